@@ -1,0 +1,199 @@
+/*
+ * sinksim_hip.h — the C ABI of the MI355X batch flooding engine (libsinksim_hip.so).
+ *
+ * One opaque batch handle owns every device buffer of a batch of independent simulations of one compiled ship
+ * and scenario. Plain pointers and sizes cross this boundary, never C++ or torch types. Every call returns an
+ * ss_status; on failure ss_last_error() holds the message (thread-local, valid until the next call on the
+ * same thread). The status classes are the exception types of the reference's C++ API so that the C++
+ * wrapper (sinksim_amd/csrc/host/batch.hpp) can rethrow the same type:
+ *   SS_INVALID_ARGUMENT  std::invalid_argument   SS_RUNTIME  std::runtime_error
+ *   SS_LOGIC             std::logic_error        SS_OUT_OF_RANGE std::out_of_range
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference checkout).
+ * The device computes the reference's "portable" (double) and "portable32" (mixed) numerics bit for bit.
+ */
+#ifndef SINKSIM_HIP_H
+#define SINKSIM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum ss_status {
+  SS_OK = 0,
+  SS_INVALID_ARGUMENT = 1,
+  SS_RUNTIME = 2,
+  SS_LOGIC = 3,
+  SS_OUT_OF_RANGE = 4
+} ss_status;
+
+/* numerics of the batch: the reference's Numerics::Portable and Numerics::PortableMixed
+ * (engine/include/sinksim/simulation.hpp:195); BatchOptions::mixedPrecision false / true
+ * (engine/include/sinksim/cuda/batch.hpp:29) */
+enum { SS_NUMERICS_PORTABLE = 1, SS_NUMERICS_PORTABLE32 = 2 };
+
+/* A compiled ship (docs/FORMATS.md "sinksim.compiled-ship"; engine/include/sinksim/model.hpp:15-35). */
+typedef struct ss_ship {
+  int32_t ncol;
+  const double *x, *y, *dxdy, *zlo, *zhi; /* [ncol] */
+  int32_t nseg;
+  const int32_t* segCol;                  /* [nseg] */
+  const double *segA, *segE;              /* [nseg] */
+  int32_t nnode;
+  const double *mu, *vmax, *amin, *aov;   /* [nnode] */
+  const int32_t *segStart, *segCount;     /* [nnode] */
+} ss_ship;
+
+/* A compiled simulation (docs/FORMATS.md "sinksim.compiled-sim"; engine/include/sinksim/model.hpp:49-96). */
+typedef struct ss_sim {
+  int32_t nnode;     /* must equal the ship's node count (engine/cuda/batch.cu:272) */
+  double rho, g, dt;
+  int32_t ngroup;
+  const int32_t *gNodeStart, *gNodeCount, *gNodes; /* [ngroup], [ngroup], [sum of counts] */
+  const uint8_t* gMode;
+  const double *gpx, *gpy, *gpz;
+  double gMargin;
+  int32_t nconn;
+  const int32_t *ca, *cb;                          /* a = -1 is the sea */
+  const uint8_t* law;                              /* 0 orifice, 1 weir */
+  const double *cx, *cy, *cz, *area, *coef;
+  const uint8_t* kind;
+  const int32_t* idx;
+  const uint8_t* en;
+  const double* tOn;
+  const int32_t *skipGroup, *monitor;
+  int32_t nmonitor;
+  double monitorThreshold;
+  int32_t nmark;
+  const double *mkx, *mky, *mkz;
+  const uint8_t* mkWhen;
+  double founder[4]; /* hullTopBelow, pitchAbsAbove, buoyancyBelow, minT */
+  double body[16];   /* ms KG g0x g0y g0z XO YO mh0 Ith0 Iph0 cDz cDth cDph qz qth qph */
+  double readout[2]; /* xFP xAP */
+  double state0[7];  /* t zO pitch roll vz wth wph */
+  const double *vol0, *level0, *cx0, *cy0, *cz0; /* [nnode] */
+} ss_sim;
+
+/* BatchOptions (engine/include/sinksim/cuda/batch.hpp:27-37), with the numerics named explicitly. */
+typedef struct ss_batch_options {
+  int32_t count;          /* simulations in the batch, >= 1 */
+  int32_t numerics;       /* SS_NUMERICS_PORTABLE or SS_NUMERICS_PORTABLE32 */
+  int32_t traced;         /* leading instances that record traces */
+  int32_t denseSteps;     /* dense records at the start of each trace */
+  double sampleEvery;     /* s between sampled records */
+  double curveEvery;      /* s between readout-curve records of every instance; 0 = none */
+  double tMax;            /* s; every instance stops here if it has not foundered */
+  int64_t stepsPerLaunch; /* steps per kernel launch; <= 0 runs every instance to its end in one launch */
+} ss_batch_options;
+
+/* DeviceInfo (engine/include/sinksim/cuda/batch.hpp:19-25) */
+typedef struct ss_device_properties {
+  char name[256];
+  char arch[64];
+  int32_t computeUnits;
+  int32_t major, minor;
+  size_t globalMemory, sharedPerBlock;
+  int32_t runtimeVersion, driverVersion;
+} ss_device_properties;
+
+/* InstanceResult (engine/include/sinksim/cuda/batch.hpp:45-54) without the per-node arrays, which
+ * ss_batch_result fills separately. */
+typedef struct ss_instance_result {
+  int32_t foundered;
+  double founderT;
+  double tEnd;
+  int64_t steps;
+  double state[7];   /* t zO pitch roll vz wth wph */
+  double outputs[7]; /* inflow Vb hullTop mw bx by bz */
+  int32_t events;    /* number of events recorded */
+} ss_instance_result;
+
+/* Event (engine/include/sinksim/kernel/types.hpp:252-258): kind 0 overflow (id = monitor), 1 mark (id = mark),
+ * 2 founder (id = -1). */
+typedef struct ss_event {
+  double t;
+  int32_t id;
+  int32_t kind;
+} ss_event;
+
+typedef struct ss_batch ss_batch;
+
+/* The message of the last failed call on this thread. */
+const char* ss_last_error(void);
+
+/* cuda::available (engine/cuda/batch.cu:71-83): SS_OK when a HIP device is present; otherwise `why` receives
+ * the reason (truncated to whyCap bytes). */
+ss_status ss_available(char* why, size_t whyCap);
+
+/* cuda::device_info (engine/cuda/batch.cu:85-95), for device 0. */
+ss_status ss_device_info(ss_device_properties* out);
+
+/* Batch::Batch (engine/cuda/batch.cu:170-209): uploads the ship and the scenario; every instance starts from
+ * the compiled initial state and connection table. SS_INVALID_ARGUMENT on a null ship, count < 1, an unknown
+ * numerics value or inconsistent arrays; SS_RUNTIME when no device is present or the model does not fit the
+ * kernel's LDS budget. */
+ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch_options* opt, ss_batch** out);
+
+/* Batch::~Batch: frees every device buffer. Null is ignored. */
+void ss_batch_destroy(ss_batch* b);
+
+/* Batch::restore (engine/cuda/batch.cu:217-227): state[7] = t zO pitch roll vz wth wph, per-node arrays of
+ * nnode values. SS_LOGIC after run, SS_OUT_OF_RANGE on a bad instance. */
+ss_status ss_batch_restore(ss_batch* b, int32_t instance, const double* state7, const double* vol, const double* level,
+                           const double* cx, const double* cy, const double* cz);
+
+/* Batch::set_connection_enabled / set_connection_area / set_connection_activation (engine/cuda/batch.cu:229-243).
+ * SS_LOGIC after run, SS_OUT_OF_RANGE on a bad instance or connection. */
+ss_status ss_batch_set_connection_enabled(ss_batch* b, int32_t instance, int32_t conn, int32_t enabled);
+ss_status ss_batch_set_connection_area(ss_batch* b, int32_t instance, int32_t conn, double area);
+ss_status ss_batch_set_connection_activation(ss_batch* b, int32_t instance, int32_t conn, double tOn);
+
+/* The three setters above for every instance at once: en, area, tOn are [count * nconn] tables (instance-major).
+ * Any pointer may be null to keep that field. No reference counterpart: it is the bulk form a driver that
+ * applies a whole sweep uses instead of count * nconn single calls. */
+ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double* area, const double* tOn);
+
+/* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the
+ * wall time of the device work (uploads excluded, as in the reference). SS_LOGIC when called twice. */
+ss_status ss_batch_run(ss_batch* b, double* seconds);
+
+/* Batch::total_steps / shared_memory_bytes (engine/include/sinksim/cuda/batch.hpp:73-74). */
+ss_status ss_batch_total_steps(const ss_batch* b, int64_t* steps);
+ss_status ss_batch_shared_memory_bytes(const ss_batch* b, size_t* bytes);
+
+/* Batch::result (engine/cuda/batch.cu:310-331). Any of the per-node pointers (nnode values each) and `events`
+ * (evCap entries) may be null. SS_LOGIC before run, SS_OUT_OF_RANGE on a bad instance. */
+ss_status ss_batch_result(const ss_batch* b, int32_t instance, ss_instance_result* res, double* vol, double* level,
+                          double* cx, double* cy, double* cz, ss_event* events, int32_t evCap);
+
+/* Every instance's summary at once ([count] arrays; any may be null): the download the reference repeats per
+ * instance (engine/cuda/batch.cu:310-331), done once. */
+ss_status ss_batch_results(const ss_batch* b, int64_t* steps, int32_t* foundered, double* founderT, double* tEnd);
+
+/* Batch::trace (engine/cuda/batch.cu:333-367): the records of a traced instance, each 10 + 5 * nnode values
+ * (t zO pitch roll vz wth wph inflow Vb hullTop, then vol level cx cy cz per node). Call with null buffers to
+ * learn the counts. SS_OUT_OF_RANGE when the instance was not traced. */
+ss_status ss_batch_trace(const ss_batch* b, int32_t instance, double* dense, int32_t denseCap, int32_t* denseCount,
+                         double* samples, int32_t samplesCap, int32_t* samplesCount);
+
+/* Batch::curve (engine/cuda/batch.cu:384-400): the readout curve (7 values per record: t trim list water
+ * inflow draftF draftA), closed by the final record. Call with a null buffer to learn the count. SS_LOGIC when
+ * curves were not recorded. */
+ss_status ss_batch_curve(const ss_batch* b, int32_t instance, double* rec, int32_t cap, int32_t* count);
+
+/* Batch::final_readouts (engine/cuda/batch.cu:369-382): the readouts at the end of the run (7 values as in a
+ * curve record), computed on the device with the portable numerics. */
+ss_status ss_batch_final_readouts(const ss_batch* b, int32_t instance, double* rec7);
+
+/* Device time of run(): the summed HIP-event duration of every kernel launch (ms) and the number of launches.
+ * No reference counterpart: the measurement hook of the benchmark. */
+ss_status ss_batch_kernel_time(const ss_batch* b, double* ms, int32_t* launches);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SINKSIM_HIP_H */

@@ -1,0 +1,302 @@
+"""ORACLE — test infrastructure only. ctypes wrapper of liboracle.so (the CPU restatement of the reference's step in
+oracle_step.hpp / oracle.cpp / fdlibm_port.c). Only tests/, __graft_entry__.smoke() and the CPU-baseline leg of
+bench.py import this module, and only as the checker.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+NUMERICS = {"oracle": 0, "portable": 1, "portable32": 2, "std": 3}
+ORDERS = {"serial": 0, "tree": 1, "tree_mixed": 2}
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_bp = C.POINTER(C.c_uint8)
+
+
+class OrcShip(C.Structure):
+    _fields_ = [("ncol", C.c_int32), ("x", _dp), ("y", _dp), ("dxdy", _dp), ("zlo", _dp), ("zhi", _dp),
+                ("nseg", C.c_int32), ("segCol", _ip), ("segA", _dp), ("segE", _dp),
+                ("nn", C.c_int32), ("mu", _dp), ("vmax", _dp), ("amin", _dp), ("aov", _dp),
+                ("segStart", _ip), ("segCount", _ip)]
+
+
+class OrcSim(C.Structure):
+    _fields_ = [("rho", C.c_double), ("g", C.c_double), ("dt", C.c_double),
+                ("ng", C.c_int32), ("gStart", _ip), ("gCount", _ip), ("gNodes", _ip), ("gMode", _bp),
+                ("gpx", _dp), ("gpy", _dp), ("gpz", _dp), ("gMargin", C.c_double),
+                ("nc", C.c_int32), ("ca", _ip), ("cb", _ip), ("law", _bp), ("cx", _dp), ("cy", _dp), ("cz", _dp),
+                ("area", _dp), ("coef", _dp), ("en", _bp), ("tOn", _dp), ("skipGroup", _ip), ("monitor", _ip),
+                ("nmon", C.c_int32), ("monThreshold", C.c_double),
+                ("nmark", C.c_int32), ("mkx", _dp), ("mky", _dp), ("mkz", _dp), ("mkWhen", _bp),
+                ("founder", C.c_double * 4), ("body", C.c_double * 16), ("readout", C.c_double * 2),
+                ("state0", C.c_double * 7),
+                ("vol0", _dp), ("level0", _dp), ("cx0", _dp), ("cy0", _dp), ("cz0", _dp)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB):
+        raise RuntimeError(f"{LIB} is missing: run make -C oracle")
+    L = C.CDLL(LIB)
+    L.orc_new.restype = C.c_void_p
+    L.orc_new.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), C.c_int]
+    L.orc_free.argtypes = [C.c_void_p]
+    L.orc_set_conn.argtypes = [C.c_void_p, C.c_int32, C.c_int, C.c_double, C.c_double]
+    L.orc_reset.argtypes = [C.c_void_p]
+    L.orc_restore.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.orc_step.argtypes = [C.c_void_p]
+    L.orc_steps.argtypes = [C.c_void_p, C.c_int64]
+    L.orc_evaluate_hydro.argtypes = [C.c_void_p]
+    L.orc_record.argtypes = [C.c_void_p, _dp]
+    L.orc_outputs.argtypes = [C.c_void_p, _dp]
+    L.orc_flows.argtypes = [C.c_void_p, _dp]
+    L.orc_scratch.argtypes = [C.c_void_p, _dp, _dp, _dp, _bp, C.c_int32]
+    L.orc_events.restype = C.c_int32
+    L.orc_events.argtypes = [C.c_void_p, _dp, _ip, _bp, C.c_int32, _ip, _dp]
+    L.orc_readouts.argtypes = [C.c_void_p, _dp]
+    L.orc_record_trace.restype = C.c_int64
+    L.orc_record_trace.argtypes = [C.c_void_p, C.c_double, C.c_int32, C.c_double, _dp, C.c_int32, _dp, C.c_int32, _ip, _ip]
+    L.orc_sweep.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), C.c_int, C.c_int32, _bp, _dp, _dp, C.c_double,
+                            C.c_double, C.c_int, C.c_int, C.POINTER(C.c_int64), _bp, _dp, _dp, _dp, C.c_int32, _ip,
+                            _dp, _ip, _bp, C.c_int32, _ip]
+    L.orc_pose.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_int, _dp]
+    L.orc_buoyancy.argtypes = [C.POINTER(OrcShip), _dp, C.c_int, _dp]
+    L.orc_node_pass.argtypes = [C.POINTER(OrcShip), _dp, C.c_int, C.c_int32, C.c_double, _dp]
+    for name in ("orc_fdl_sin", "orc_fdl_cos"):
+        getattr(L, name).restype = C.c_double
+        getattr(L, name).argtypes = [C.c_double]
+    L.orc_fdl_pow.restype = C.c_double
+    L.orc_fdl_pow.argtypes = [C.c_double, C.c_double]
+    L.orc_fdl_scalbn.restype = C.c_double
+    L.orc_fdl_scalbn.argtypes = [C.c_double, C.c_int]
+    L.orc_pow_v8.restype = C.c_double
+    L.orc_pow_v8.argtypes = [C.c_double, C.c_double]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _f(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _u(a):
+    return np.ascontiguousarray(a, dtype=np.uint8)
+
+
+class Model:
+    """The oracle's structs over contiguous copies of a CompiledShip / CompiledSim (kept alive here)."""
+
+    def __init__(self, ship, sim):
+        self.ship, self.sim = ship, sim
+        k = self.keep = {}
+
+        def kp(name, a):
+            k[name] = a
+            return a
+
+        s = OrcShip()
+        s.ncol = ship.columns
+        s.x, s.y, s.dxdy = (_p(kp(n, _f(getattr(ship, n))), C.c_double) for n in ("x", "y", "dxdy"))
+        s.zlo, s.zhi = (_p(kp(n, _f(getattr(ship, n))), C.c_double) for n in ("zlo", "zhi"))
+        s.nseg = ship.segments
+        s.segCol = _p(kp("segCol", _i(ship.segCol)), C.c_int32)
+        s.segA = _p(kp("segA", _f(ship.segA)), C.c_double)
+        s.segE = _p(kp("segE", _f(ship.segE)), C.c_double)
+        s.nn = ship.nodes
+        s.mu, s.vmax, s.amin, s.aov = (_p(kp(n, _f(getattr(ship, n))), C.c_double) for n in ("mu", "vmax", "amin", "aov"))
+        s.segStart = _p(kp("segStart", _i(ship.segStart)), C.c_int32)
+        s.segCount = _p(kp("segCount", _i(ship.segCount)), C.c_int32)
+        self.s = s
+
+        m = OrcSim()
+        m.rho, m.g, m.dt = sim.rho, sim.g, sim.dt
+        m.ng = sim.groups
+        m.gStart = _p(kp("gs", _i(sim.gNodeStart)), C.c_int32)
+        m.gCount = _p(kp("gc", _i(sim.gNodeCount)), C.c_int32)
+        m.gNodes = _p(kp("gn", _i(sim.gNodes)), C.c_int32)
+        m.gMode = _p(kp("gm", _u(sim.gMode)), C.c_uint8)
+        m.gpx, m.gpy, m.gpz = (_p(kp(n, _f(getattr(sim, n))), C.c_double) for n in ("gpx", "gpy", "gpz"))
+        m.gMargin = sim.gMargin
+        m.nc = sim.connections
+        m.ca = _p(kp("ca", _i(sim.ca)), C.c_int32)
+        m.cb = _p(kp("cb", _i(sim.cb)), C.c_int32)
+        m.law = _p(kp("law", _u(sim.cLaw)), C.c_uint8)
+        m.cx, m.cy, m.cz = (_p(kp(n, _f(getattr(sim, n))), C.c_double) for n in ("cx", "cy", "cz"))
+        m.area = _p(kp("area", _f(sim.cArea)), C.c_double)
+        m.coef = _p(kp("coef", _f(sim.cCoef)), C.c_double)
+        m.en = _p(kp("en", _u(sim.cEn)), C.c_uint8)
+        m.tOn = _p(kp("tOn", _f(sim.cTOn)), C.c_double)
+        m.skipGroup = _p(kp("skip", _i(sim.cSkipGroup)), C.c_int32)
+        m.monitor = _p(kp("mon", _i(sim.cMonitor)), C.c_int32)
+        m.nmon = sim.monitors
+        m.monThreshold = sim.monitorThreshold
+        m.nmark = sim.marks
+        m.mkx, m.mky, m.mkz = (_p(kp(n, _f(getattr(sim, n))), C.c_double) for n in ("mkx", "mky", "mkz"))
+        m.mkWhen = _p(kp("mkw", _u(sim.mkWhen)), C.c_uint8)
+        m.founder[:] = list(sim.founder)
+        m.body[:] = list(sim.body)
+        m.readout[:] = list(sim.readout)
+        m.state0[:] = list(sim.state0)
+        m.vol0, m.level0, m.cx0, m.cy0, m.cz0 = (_p(kp(n, _f(getattr(sim, n))), C.c_double)
+                                                 for n in ("vol0", "level0", "cx0", "cy0", "cz0"))
+        self.m = m
+
+
+class OracleSim:
+    """One simulation on the CPU restatement, with the reference's Simulation semantics."""
+
+    def __init__(self, ship, sim, numerics: str = "portable", model: Model | None = None):
+        self.model = model or Model(ship, sim)
+        self.ship, self.sim = ship, sim
+        self.nn = ship.nodes
+        self.h = C.c_void_p(lib().orc_new(C.byref(self.model.s), C.byref(self.model.m), NUMERICS[numerics]))
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().orc_free(self.h)
+        except Exception:
+            pass
+
+    def set_conn(self, c: int, en: bool, area: float, tOn: float) -> None:
+        lib().orc_set_conn(self.h, c, 1 if en else 0, float(area), float(tOn))
+
+    def reset(self) -> None:
+        lib().orc_reset(self.h)
+
+    def restore(self, rec: np.ndarray) -> None:
+        """Restart from a trace record (t zO pitch roll vz wth wph inflow Vb hullTop, vol level cx cy cz)."""
+        nn = self.nn
+        r = _f(rec)
+        s7 = _f(r[:7])
+        parts = [_f(r[10 + k * nn:10 + (k + 1) * nn]) for k in range(5)]
+        lib().orc_restore(self.h, _p(s7, C.c_double), *[_p(a, C.c_double) for a in parts])
+
+    def step(self, n: int = 1) -> None:
+        lib().orc_steps(self.h, n)
+
+    def record(self) -> np.ndarray:
+        out = np.zeros(10 + 5 * self.nn)
+        lib().orc_record(self.h, _p(out, C.c_double))
+        return out
+
+    def outputs(self) -> np.ndarray:
+        out = np.zeros(7)
+        lib().orc_outputs(self.h, _p(out, C.c_double))
+        return out
+
+    def flows(self) -> np.ndarray:
+        out = np.zeros(self.sim.connections)
+        lib().orc_flows(self.h, _p(out, C.c_double))
+        return out
+
+    def readouts(self) -> np.ndarray:
+        """t trimDeg listDeg draftF draftA waterM3 waterT inflowTpm"""
+        out = np.zeros(8)
+        lib().orc_readouts(self.h, _p(out, C.c_double))
+        return out
+
+    def events(self):
+        cap = self.sim.monitors + self.sim.marks + 1
+        t, i, k = np.zeros(cap), np.zeros(cap, np.int32), np.zeros(cap, np.uint8)
+        f, ft = C.c_int32(), C.c_double()
+        n = lib().orc_events(self.h, _p(t, C.c_double), _p(i, C.c_int32), _p(k, C.c_uint8), cap, C.byref(f), C.byref(ft))
+        return [(float(t[j]), int(i[j]), int(k[j])) for j in range(min(n, cap))], bool(f.value), ft.value
+
+    def record_trace(self, t_max: float, dense: int, sample_every: float):
+        rs = 10 + 5 * self.nn
+        dcap = dense + 1
+        scap = int(np.floor(t_max / sample_every)) + 2
+        d, s = np.zeros((dcap, rs)), np.zeros((scap, rs))
+        nd, ns = C.c_int32(), C.c_int32()
+        steps = lib().orc_record_trace(self.h, t_max, dense, sample_every, _p(d, C.c_double), dcap, _p(s, C.c_double),
+                                       scap, C.byref(nd), C.byref(ns))
+        return d[:nd.value], s[:ns.value], int(steps)
+
+
+def sweep(ship, sim, numerics: str, en, area, tOn, t_max: float, every: float, threads: int = 0,
+          stop_when_stable: bool = False, model: Model | None = None) -> list[dict]:
+    """sweep::run_cpu (engine/src/sweep.cpp:231-271) on per-instance connection tables [count, nconn]."""
+    model = model or Model(ship, sim)
+    en, area, tOn = _u(en), _f(area), _f(tOn)
+    count = en.shape[0]
+    curve_cap = int(np.floor(t_max / every)) + 3
+    ev_cap = sim.monitors + sim.marks + 1
+    steps = np.zeros(count, np.int64)
+    f, ft, te = np.zeros(count, np.uint8), np.zeros(count), np.zeros(count)
+    curve, cn = np.zeros((count, curve_cap, 7)), np.zeros(count, np.int32)
+    evT, evI, evK, evN = np.zeros((count, ev_cap)), np.zeros((count, ev_cap), np.int32), np.zeros((count, ev_cap), np.uint8), np.zeros(count, np.int32)
+    lib().orc_sweep(C.byref(model.s), C.byref(model.m), NUMERICS[numerics], count, _p(en, C.c_uint8), _p(area, C.c_double),
+                    _p(tOn, C.c_double), t_max, every, 1 if stop_when_stable else 0, threads,
+                    steps.ctypes.data_as(C.POINTER(C.c_int64)), _p(f, C.c_uint8), _p(ft, C.c_double), _p(te, C.c_double),
+                    _p(curve, C.c_double), curve_cap, _p(cn, C.c_int32), _p(evT, C.c_double), _p(evI, C.c_int32),
+                    _p(evK, C.c_uint8), ev_cap, _p(evN, C.c_int32))
+    out = []
+    for i in range(count):
+        n = min(int(cn[i]), curve_cap)
+        out.append({"steps": int(steps[i]), "foundered": bool(f[i]), "founderT": float(ft[i]), "tEnd": float(te[i]),
+                    "curve": curve[i, :n].copy(),
+                    "events": [(float(evT[i, k]), int(evI[i, k]), int(evK[i, k])) for k in range(min(int(evN[i]), ev_cap))]})
+    return out
+
+
+def pose(body, zO, pitch, roll, math: str = "portable") -> np.ndarray:
+    out = np.zeros(13)
+    b = _f(body)
+    lib().orc_pose(_p(b, C.c_double), zO, pitch, roll, {"portable": 0, "oracle": 1, "std": 2}[math], _p(out, C.c_double))
+    return out
+
+
+def buoyancy(model: Model, frame13, order: str) -> np.ndarray:
+    out = np.zeros(5)
+    f = _f(frame13)
+    lib().orc_buoyancy(C.byref(model.s), _p(f, C.c_double), ORDERS[order], _p(out, C.c_double))
+    return out
+
+
+def node_pass(model: Model, frame13, order: str, n: int, h: float) -> np.ndarray:
+    out = np.zeros(7)
+    f = _f(frame13)
+    lib().orc_node_pass(C.byref(model.s), _p(f, C.c_double), ORDERS[order], n, h, _p(out, C.c_double))
+    return out
+
+
+def fdl_sin(x: float) -> float:
+    return lib().orc_fdl_sin(x)
+
+
+def fdl_cos(x: float) -> float:
+    return lib().orc_fdl_cos(x)
+
+
+def fdl_pow(x: float, y: float) -> float:
+    return lib().orc_fdl_pow(x, y)
+
+
+def fdl_scalbn(x: float, n: int) -> float:
+    return lib().orc_fdl_scalbn(x, n)
+
+
+def pow_v8(x: float, y: float) -> float:
+    return lib().orc_pow_v8(x, y)

@@ -1,0 +1,200 @@
+"""The batch engine from Python: the surface of the reference's sinksim::cuda::Batch
+(engine/include/sinksim/cuda/batch.hpp:56-87) over the C ABI of include/sinksim_hip.h.
+
+    from sinksim_amd.model import load_ship, load_sim
+    from sinksim_amd.batch import Batch, BatchOptions
+    b = Batch(ship, sim, BatchOptions(count=1024, mixed_precision=False, curve_every=30, t_max=14400))
+    b.set_connection_area(3, 289, 2.4)
+    seconds = b.run()
+    b.total_steps(), b.result(3), b.curve(3)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as A
+
+
+@dataclass
+class BatchOptions:
+    """BatchOptions (engine/include/sinksim/cuda/batch.hpp:27-37)."""
+    count: int = 1
+    mixed_precision: bool = True     # portable32 numerics; False computes the portable (double) numerics
+    traced_instances: int = 1
+    dense_steps: int = 400
+    sample_every: float = 60.0
+    curve_every: float = 0.0
+    t_max: float = 5 * 3600.0
+    steps_per_launch: int = 0        # <= 0: one launch runs every instance to its end
+
+
+@dataclass
+class InstanceResult:
+    """InstanceResult (engine/include/sinksim/cuda/batch.hpp:45-54)."""
+    foundered: bool
+    founderT: float
+    tEnd: float
+    steps: int
+    state: np.ndarray                # t zO pitch roll vz wth wph
+    outputs: np.ndarray              # inflow Vb hullTop mw bx by bz
+    vol: np.ndarray
+    level: np.ndarray
+    cx: np.ndarray
+    cy: np.ndarray
+    cz: np.ndarray
+    events: list = field(default_factory=list)   # (t, id, kind)
+
+
+def available() -> tuple[bool, str]:
+    """cuda::available (engine/cuda/batch.cu:71-83)."""
+    lib = A.load_library()
+    buf = C.create_string_buffer(512)
+    ok = lib.ss_available(buf, 512) == A.SS_OK
+    return ok, buf.value.decode()
+
+
+def device_info() -> dict:
+    """cuda::device_info (engine/cuda/batch.cu:85-95)."""
+    lib = A.load_library()
+    p = A.SsDeviceProperties()
+    A.check(lib.ss_device_info(C.byref(p)))
+    return {"name": p.name.decode(), "arch": p.arch.decode(), "computeUnits": p.computeUnits, "major": p.major,
+            "minor": p.minor, "globalMemory": p.globalMemory, "sharedPerBlock": p.sharedPerBlock,
+            "runtimeVersion": p.runtimeVersion, "driverVersion": p.driverVersion}
+
+
+class Batch:
+    """Many simulations of one ship and scenario on the GPU, computing the portable numerics bit for bit."""
+
+    def __init__(self, ship, sim, opt: BatchOptions):
+        self._lib = A.load_library()
+        self.ship, self.sim, self.opt = ship, sim, opt
+        self._shipS = A.ShipStruct(ship)
+        self._simS = A.SimStruct(sim)
+        o = A.SsBatchOptions(int(opt.count), A.SS_NUMERICS_PORTABLE32 if opt.mixed_precision else A.SS_NUMERICS_PORTABLE,
+                             int(opt.traced_instances), int(opt.dense_steps), float(opt.sample_every),
+                             float(opt.curve_every), float(opt.t_max), int(opt.steps_per_launch))
+        h = C.c_void_p()
+        A.check(self._lib.ss_batch_create(C.byref(self._shipS.s), C.byref(self._simS.s), C.byref(o), C.byref(h)))
+        self._h = h
+        self.count = int(opt.count)
+        self.traced = min(int(opt.traced_instances), self.count)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._lib.ss_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ------------------------------------------------------------------ set-up (before run)
+
+    def restore(self, instance: int, state7, vol, level, cx, cy, cz) -> None:
+        arrs = [A.f64(state7), A.f64(vol), A.f64(level), A.f64(cx), A.f64(cy), A.f64(cz)]
+        A.check(self._lib.ss_batch_restore(self._h, instance, *[A.ptr(a, C.c_double) for a in arrs]))
+
+    def set_connection_enabled(self, instance: int, c: int, en: bool) -> None:
+        A.check(self._lib.ss_batch_set_connection_enabled(self._h, instance, c, 1 if en else 0))
+
+    def set_connection_area(self, instance: int, c: int, area: float) -> None:
+        A.check(self._lib.ss_batch_set_connection_area(self._h, instance, c, float(area)))
+
+    def set_connection_activation(self, instance: int, c: int, tOn: float) -> None:
+        A.check(self._lib.ss_batch_set_connection_activation(self._h, instance, c, float(tOn)))
+
+    def set_connections(self, en=None, area=None, tOn=None) -> None:
+        """Every instance's connection table at once: [count, nconn] arrays (None keeps a field)."""
+        n = self.count * self.sim.connections
+        e = A.u8(en).reshape(-1) if en is not None else None
+        a = A.f64(area).reshape(-1) if area is not None else None
+        t = A.f64(tOn).reshape(-1) if tOn is not None else None
+        for arr in (e, a, t):
+            if arr is not None and arr.size != n:
+                raise ValueError(f"connection tables must hold count * nconn = {n} values")
+        A.check(self._lib.ss_batch_set_connections(
+            self._h, A.ptr(e, C.c_uint8) if e is not None else None, A.ptr(a, C.c_double) if a is not None else None,
+            A.ptr(t, C.c_double) if t is not None else None))
+        self._keep = (e, a, t)
+
+    # ------------------------------------------------------------------ run and results
+
+    def run(self) -> float:
+        s = C.c_double()
+        A.check(self._lib.ss_batch_run(self._h, C.byref(s)))
+        return s.value
+
+    def total_steps(self) -> int:
+        v = C.c_int64()
+        A.check(self._lib.ss_batch_total_steps(self._h, C.byref(v)))
+        return v.value
+
+    def shared_memory_bytes(self) -> int:
+        v = C.c_size_t()
+        A.check(self._lib.ss_batch_shared_memory_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def kernel_time(self) -> tuple[float, int]:
+        ms, n = C.c_double(), C.c_int32()
+        A.check(self._lib.ss_batch_kernel_time(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def results(self) -> dict:
+        """Every instance's steps, founder flag, founder time and end time (one call, [count] arrays)."""
+        n = self.count
+        steps, f, ft, te = np.zeros(n, np.int64), np.zeros(n, np.int32), np.zeros(n), np.zeros(n)
+        A.check(self._lib.ss_batch_results(self._h, steps.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(f, C.c_int32),
+                                           A.ptr(ft, C.c_double), A.ptr(te, C.c_double)))
+        return {"steps": steps, "foundered": f.astype(bool), "founderT": ft, "tEnd": te}
+
+    def result(self, instance: int) -> InstanceResult:
+        nn = self.ship.nodes
+        r = A.SsInstanceResult()
+        arrs = [np.zeros(nn) for _ in range(5)]
+        cap = self.sim.monitors + self.sim.marks + 1
+        ev = (A.SsEvent * cap)()
+        A.check(self._lib.ss_batch_result(self._h, instance, C.byref(r), *[A.ptr(a, C.c_double) for a in arrs], ev, cap))
+        events = [(ev[k].t, ev[k].id, ev[k].kind) for k in range(min(r.events, cap))]
+        return InstanceResult(bool(r.foundered), r.founderT, r.tEnd, r.steps, np.array(r.state[:]),
+                              np.array(r.outputs[:]), *arrs, events=events)
+
+    def trace(self, instance: int) -> tuple[np.ndarray, np.ndarray]:
+        """Dense and sampled records of a traced instance, [n, 10 + 5 * nodes] each."""
+        nd, ns = C.c_int32(), C.c_int32()
+        A.check(self._lib.ss_batch_trace(self._h, instance, None, 0, C.byref(nd), None, 0, C.byref(ns)))
+        rs = 10 + 5 * self.ship.nodes
+        d, s = np.zeros((nd.value, rs)), np.zeros((ns.value, rs))
+        A.check(self._lib.ss_batch_trace(self._h, instance, A.ptr(d, C.c_double), nd.value, C.byref(nd),
+                                         A.ptr(s, C.c_double), ns.value, C.byref(ns)))
+        return d, s
+
+    def curve(self, instance: int) -> np.ndarray:
+        """[n, 7] records (t trim list water inflow draftF draftA), closed by the final record."""
+        n = C.c_int32()
+        A.check(self._lib.ss_batch_curve(self._h, instance, None, 0, C.byref(n)))
+        out = np.zeros((n.value, 7))
+        A.check(self._lib.ss_batch_curve(self._h, instance, A.ptr(out, C.c_double), n.value, C.byref(n)))
+        return out
+
+    def final_readouts(self, instance: int) -> np.ndarray:
+        out = np.zeros(7)
+        A.check(self._lib.ss_batch_final_readouts(self._h, instance, A.ptr(out, C.c_double)))
+        return out
+
+    def event_id(self, kind: int, id_: int) -> str:
+        return self.sim.event_id(kind, id_)
+
+    def event_label(self, kind: int, id_: int) -> str:
+        return self.sim.event_label(kind, id_)

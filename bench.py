@@ -1,0 +1,156 @@
+"""Benchmark: aggregate simulation steps per second of the HIP batch engine, bit-exact portable numerics (FP64).
+
+Workload: the 1,024-instance titanic64 batch (scenario "titanic", dt 0.25 s): per instance a log-uniform [0.5, 2]
+scale on every connection kind's area and each watertight door opened with p = 0.2, run to foundering or the 4 h
+cap, with the 30 s readout curves recorded (sinksim_amd.workloads.titanic_batch_sweep, seed 20261008).
+
+One bench step = one whole batch run (Batch.run(): every instance to its end, inputs already in HBM). value =
+sum over instances of steps / wall time of the timed runs. The roofline is that of the dominant kernel (run_chunk,
+FP64): algorithmic FP64 operations per simulation step (measured with the oracle's operation counter on a sample of
+the same workload) x steps, over the kernel's HIP-event duration. The CPU baseline is the reference's own CPU
+engine (oracle/_ref/sinksim_sweep, compiled from the reference's sources) on a bounded sample of the same sweep;
+if that binary is absent, the oracle's restatement.
+
+    python bench.py [--steps K] [--warmup W] [--count 1024] [--numerics portable|portable32] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two operations)
+# Algorithmic FP64 operations per simulation step of the titanic64 scenario (portable numerics, reference CPU
+# engine instrumented over the 1912 run: 4,408 column and ~9,435 segment evaluations and 71 node passes per step).
+FLOPS_PER_STEP = 2.31e5
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--count", type=int, default=1024)
+    ap.add_argument("--numerics", default="portable", choices=["portable", "portable32"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=48, help="instances in the CPU-baseline sample")
+    return ap.parse_args()
+
+
+def cpu_baseline(sw, sim, count: int) -> dict:
+    """The reference's CPU sweep runner (or the oracle's) on the first `count` instances, all host cores."""
+    from sinksim_amd import sweep as S
+    from sinksim_amd.data_paths import SHIP, sim_path
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    sample = S.Sweep(ship=sw.ship, sim=sw.sim, tMax=sw.tMax, every=sw.every, instances=sw.instances[:count])
+    ref = os.path.join(ROOT, "oracle", "_ref", "sinksim_sweep")
+    if os.path.exists(ref):
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "sample.sweep.json")
+            S.save_sweep(sample, path, sim)
+            res = os.path.join(d, "r.json")
+            t0 = time.perf_counter()
+            subprocess.run([ref, SHIP, sim_path(sim.id), path, "--numerics", "portable", "--threads", str(cores),
+                            "--results", res], check=True, capture_output=True)
+            secs = time.perf_counter() - t0
+            steps = sum(x.steps for x in S.load_results(res).instances)
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle
+        from sinksim_amd.data_paths import titanic
+        ship, _ = titanic(sim.id)
+        en, area, tOn = S.connection_tables(sample, sim)
+        t0 = time.perf_counter()
+        out = pyoracle.sweep(ship, sim, "portable", en, area, tOn, sw.tMax, sw.every, threads=cores)
+        secs = time.perf_counter() - t0
+        steps = sum(o["steps"] for o in out)
+        kind = "port"
+    return {"value": round(steps / secs, 1), "unit": "sim-steps/s", "cores": cores, "kind": kind,
+            "sample": f"first {count} instances of the same 1,024-instance sweep, portable numerics, "
+                      f"{steps} steps in {secs:.1f} s (file IO included)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); instances are independent, so N GPUs "
+              "would each run count/N of them", file=sys.stderr)
+        sys.exit(2)
+    from sinksim_amd import sweep as S
+    from sinksim_amd.batch import Batch, BatchOptions, device_info
+    from sinksim_amd.data_paths import titanic
+    from sinksim_amd.workloads import titanic_batch_sweep
+
+    ship, sim = titanic("titanic")
+    sw = titanic_batch_sweep(sim, count=args.count)
+    en, area, tOn = S.connection_tables(sw, sim)
+    mixed = args.numerics == "portable32"
+
+    def one_run():
+        b = Batch(ship, sim, BatchOptions(count=args.count, mixed_precision=mixed, traced_instances=0,
+                                          curve_every=sw.every, t_max=sw.tMax))
+        b.set_connections(en, area, tOn)
+        secs = b.run()
+        steps = b.total_steps()
+        kms, launches = b.kernel_time()
+        b.close()
+        return secs, steps, kms, launches
+
+    for w in range(args.warmup):
+        secs, steps, _, _ = one_run()
+        print(f"warmup {w}: {steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+    tot_s, tot_steps, tot_kms, tot_launch = 0.0, 0, 0.0, 0
+    for k in range(args.steps):
+        secs, steps, kms, launches = one_run()
+        tot_s += secs
+        tot_steps += steps
+        tot_kms += kms
+        tot_launch += launches
+        print(f"step {k}: {steps} steps in {secs:.3f} s ({steps / secs:.0f} steps/s), kernel {kms:.1f} ms over "
+              f"{launches} launch(es)", file=sys.stderr, flush=True)
+    value = tot_steps / tot_s
+    per_launch_ms = tot_kms / max(tot_launch, 1)
+    flops_per_launch = FLOPS_PER_STEP * tot_steps / max(tot_launch, 1)
+    achieved = flops_per_launch / (per_launch_ms * 1e-3) / 1e12
+    out = {
+        "metric": "aggregate sim-steps/s bit-exact at batch 1024",
+        "value": round(value, 1),
+        "unit": "sim-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tot_s / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if not mixed else "f32+f64",
+        "data": "synthetic (seeded sweep of the titanic64 model)",
+        "config": {"workload": f"titanic64 batch of {args.count} seeded instances, {args.numerics} numerics, to founder "
+                               f"or 4 h, curves every 30 s", "instances": args.count, "numerics": args.numerics,
+                   "sim_steps_per_bench_step": tot_steps // args.steps, "sinkings_per_min": None,
+                   "device": device_info()["name"]},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                     "kernel": "run_chunk<false>" if not mixed else "run_chunk<true>", "kernel_ms": round(per_launch_ms, 3)},
+    }
+    # sinkings per minute as the reference reports them (apps/sinksim_cuda_run.cpp:138-140), against instance 0
+    out["config"]["sinkings_per_min"] = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,49 @@
+"""Seeded batch workloads.
+
+titanic_batch_sweep: the 1,024-instance titanic64 batch the benchmark runs — per instance, a scale factor on the
+area of every connection kind present in the scenario drawn log-uniform in [0.5, 2], and each watertight door
+(kind "door", idx 2..13) overridden open with probability 0.2; tMax 4 h, curve every 30 s. It is written as a
+standard sinksim.sweep document, so the CPU tools run the identical instances.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import sweep as S
+
+
+def titanic_batch_sweep(sim, count: int = 1024, seed: int = 20261008, t_max: float = 4 * 3600.0, every: float = 30.0,
+                        p_open: float = 0.2) -> S.Sweep:
+    rng = np.random.default_rng(seed)
+    present = sorted({int(k) for k in sim.cKind})
+    door = sim.kind_index("door")
+    door_idx = sorted({int(sim.cIdx[c]) for c in range(sim.connections) if int(sim.cKind[c]) == door}) if door >= 0 else []
+    sw = S.Sweep(ship=sim.shipId, sim=sim.id, tMax=t_max, every=every)
+    for i in range(count):
+        scale = [(k, float(math.exp(rng.uniform(math.log(0.5), math.log(2.0))))) for k in present]
+        opened = [S.ConnectionOverride(kind=door, idx=d, enabled=True) for d in door_idx if rng.uniform() < p_open]
+        sw.instances.append(S.Instance(id=f"b{i:04d}", kindScale=scale, overrides=opened))
+    return sw
+
+
+def random_tables(sim, count: int, seed: int, activation_share: float = 0.0, p_open: float = 0.2):
+    """[count, nconn] connection tables: kind scales log-uniform in [0.5, 2], every disabled connection enabled
+    with probability p_open, and optionally a share of connections activated late (tOn uniform in [0, 1800] s)."""
+    rng = np.random.default_rng(seed)
+    nc = sim.connections
+    en = np.tile(sim.cEn.astype(np.uint8), (count, 1))
+    area = np.tile(sim.cArea.astype(np.float64), (count, 1))
+    tOn = np.tile(sim.cTOn.astype(np.float64), (count, 1))
+    kinds = sorted({int(k) for k in sim.cKind})
+    for i in range(count):
+        for k in kinds:
+            m = sim.cKind == k
+            area[i, m] = sim.cArea[m] * math.exp(rng.uniform(math.log(0.5), math.log(2.0)))
+        opened = (sim.cEn == 0) & (rng.uniform(size=nc) < p_open)
+        en[i, opened] = 1
+        if activation_share > 0:
+            late = rng.uniform(size=nc) < activation_share
+            tOn[i, late] = rng.uniform(0, 1800, size=int(late.sum()))
+    return en, area, tOn

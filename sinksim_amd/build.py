@@ -1,6 +1,8 @@
 """Build recipe of the native parts, in-tree (the built files travel with the repository to a GPU host):
 
   sinksim_amd/libsinksim_hip.so   the gfx950 batch kernel and the C ABI (include/sinksim_hip.h)
+  sinksim_amd/bin/sinksim_hip_run, sinksim_amd/bin/sinksim_sweep
+                                  the C++ command-line drop-ins over the C ABI (host code: g++, no HIP headers)
 
 Every HIP translation unit is compiled without contraction (-ffp-contract=off: each product and sum rounds on its
 own, as the reference's strict floating-point target requires), with IEEE single-precision denormals and correctly
@@ -50,8 +52,35 @@ def build_library(force: bool = True) -> str:
     return LIB
 
 
+HOST = os.path.join(CSRC, "host")
+APPS = os.path.join(CSRC, "apps")
+BIN = os.path.join(HERE, "bin")
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math"]
+HOST_SOURCES = [os.path.join(HOST, f) for f in ("json.cpp", "model.cpp", "sweep.cpp")]
+
+
+def build_tools(force: bool = True) -> list[str]:
+    """The command-line drop-ins, linked against the in-tree libsinksim_hip.so (rpath $ORIGIN/..)."""
+    os.makedirs(BIN, exist_ok=True)
+    deps = HOST_SOURCES + [os.path.join(HOST, f) for f in ("json.hpp", "model.hpp", "sweep.hpp", "batch.hpp")] + [
+        os.path.join(APPS, "cli.hpp"), os.path.join(ROOT, "include", "sinksim_hip.h")]
+    out = []
+    for app in ("sinksim_hip_run", "sinksim_sweep"):
+        exe = os.path.join(BIN, app)
+        src = os.path.join(APPS, app + ".cpp")
+        if not force and os.path.exists(exe) and all(os.path.getmtime(exe) >= os.path.getmtime(d) for d in deps + [src, LIB]):
+            out.append(exe)
+            continue
+        _run([CXX, *HOST_FLAGS, "-o", exe + ".tmp", src, *HOST_SOURCES, f"-L{HERE}", "-lsinksim_hip",
+              "-Wl,-rpath,$ORIGIN/.."])
+        os.replace(exe + ".tmp", exe)
+        out.append(exe)
+    return out
+
+
 def build_all(force: bool = True) -> list[str]:
-    return [build_library(force)]
+    return [build_library(force), *build_tools(force)]
 
 
 if __name__ == "__main__":

@@ -28,10 +28,10 @@ constexpr int kCurveValues = 7;
 // ------------------------------------------------------------------------------------------- the kernel
 
 // Writes the complete state record (the trace layout, 10 + 5 * nodes values), one thread.
-__device__ inline void write_record(double* rec, const Blk& B, int nn) {
+__device__ __noinline__ void write_record(SSG double* rec, const Blk& B, int nn) {
   for (int k = 0; k < 7; ++k) rec[k] = B.scal[S_T + k];
   rec[7] = B.scal[O_INFLOW]; rec[8] = B.scal[O_VB]; rec[9] = B.scal[O_TOP];
-  double* p = rec + 10;
+  SSG double* p = rec + 10;
   for (int n = 0; n < nn; ++n) p[n] = B.vol[n];
   p += nn;
   for (int n = 0; n < nn; ++n) p[n] = B.level[n];
@@ -47,112 +47,128 @@ __device__ inline void write_record(double* rec, const Blk& B, int nn) {
 // maxSteps steps in this launch, store it back. Traces and curves are recorded as in the reference
 // (device_step.cuh:379-416); the readouts of the state at the end of the launch go to final7.
 template <bool Mixed>
-__global__ __launch_bounds__(kThreads) void run_chunk(DevShip S, DevNet M, DevBatch Bt, LdsLayout L, long long maxSteps, double tMax,
-                                                     int firstLaunch) {
-  extern __shared__ double lds[];
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_chunk(const Params* __restrict__ params, long long maxSteps, double tMax, int firstLaunch) {
+  PC& P = *(PC*)params;
+  extern __shared__ double lds_raw[];
+  SSL double* lds = (SSL double*)lds_raw;
   const int inst = blockIdx.x;
   const int tid = threadIdx.x;
-  const int NN = S.nn, NC = M.nc;
+  const int NN = P.S.nn, NC = P.M.nc, evCap = P.M.evCap, nmon = P.M.nmon, nmark = P.M.nmark;
   const size_t iN = static_cast<size_t>(inst) * NN, iC = static_cast<size_t>(inst) * NC;
-  const Blk B = carve(lds, L);
+  const Blk B = carve(lds, P);
 
   // load the instance
   if (tid < 7) {
-    B.scal[S_T + tid] = Bt.st[inst * 7 + tid];
-    B.scal[O_INFLOW + tid] = Bt.out[inst * 7 + tid];
+    B.scal[S_T + tid] = P.Bt.st[inst * 7 + tid];
+    B.scal[O_INFLOW + tid] = P.Bt.out[inst * 7 + tid];
   }
   if (tid == 0) {
-    B.scal[S_EVCOUNT] = Bt.evCount[inst];
-    B.scal[S_FOUNDERED] = Bt.foundered[inst];
-    B.scal[S_FOUNDERT] = Bt.founderT[inst];
+    B.scal[S_EVCOUNT] = P.Bt.evCount[inst];
+    B.scal[S_FOUNDERED] = P.Bt.foundered[inst];
+    B.scal[S_FOUNDERT] = P.Bt.founderT[inst];
     B.scal[S_SEA] = 0;
+    if (P.Bt.curveEvery > 0) {
+      B.scal[S_NEXTCURVE] = P.Bt.nextCurve[inst];
+      B.scal[S_CURVECOUNT] = P.Bt.curveCount[inst];
+    }
   }
   for (int n = tid; n < NN; n += kThreads) {
-    B.vol[n] = Bt.vol[iN + n]; B.level[n] = Bt.level[iN + n];
-    B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n];
+    B.vol[n] = P.Bt.vol[iN + n]; B.level[n] = P.Bt.level[iN + n];
+    B.cx[n] = P.Bt.cx[iN + n]; B.cy[n] = P.Bt.cy[iN + n]; B.cz[n] = P.Bt.cz[iN + n];
     B.area[n] = 0; B.zmin[n] = 0; B.zmax[n] = 0; B.hEff[n] = 0; B.aEff[n] = 0; B.excess[n] = 0; B.deg[n] = 0;
     B.tX[n] = 0; B.tY[n] = 0; B.tF[n] = 0; B.tM[n] = 0; B.tIp[n] = 0; B.tIr[n] = 0;
   }
   for (int c = tid; c < NC; c += kThreads) {
-    B.cen[c] = Bt.en[iC + c]; B.carea[c] = Bt.area[iC + c]; B.ctOn[c] = Bt.tOn[iC + c];
+    B.cen[c] = P.Bt.en[iC + c]; B.carea[c] = P.Bt.area[iC + c]; B.ctOn[c] = P.Bt.tOn[iC + c];
     B.dV[c] = 0; B.q[c] = 0;
   }
-  for (int g = tid; g < M.ng; g += kThreads) B.merged[g] = 0;
-  for (int e = tid; e < M.evCap; e += kThreads) {
-    const size_t k = static_cast<size_t>(inst) * M.evCap + e;
-    B.evT[e] = Bt.evT[k]; B.evId[e] = Bt.evId[k]; B.evKind[e] = Bt.evKind[k];
+  for (int g = tid; g < P.M.ng; g += kThreads) B.merged[g] = 0;
+  for (int e = tid; e < evCap; e += kThreads) {
+    const size_t k = static_cast<size_t>(inst) * evCap + e;
+    B.evT[e] = P.Bt.evT[k]; B.evId[e] = P.Bt.evId[k]; B.evKind[e] = P.Bt.evKind[k];
   }
-  for (int m = tid; m < M.nmon; m += kThreads) { B.overT[m] = Bt.overT[static_cast<size_t>(inst) * M.nmon + m]; B.over[m] = 0; }
-  for (int m = tid; m < M.nmark; m += kThreads) B.markT[m] = Bt.markT[static_cast<size_t>(inst) * M.nmark + m];
+  for (int m = tid; m < nmon; m += kThreads) { B.overT[m] = P.Bt.overT[static_cast<size_t>(inst) * nmon + m]; B.over[m] = 0; }
+  for (int m = tid; m < nmark; m += kThreads) B.markT[m] = P.Bt.markT[static_cast<size_t>(inst) * nmark + m];
+  __syncthreads();
+  if (tid < 64) pose_wave0(P, B, tid);  // the pose of the loaded state for the first step
   __syncthreads();
 
-  const bool traced = inst < Bt.traced;
-  long long done = Bt.steps[inst];
+  const bool traced = inst < P.Bt.traced;
+  long long done = P.Bt.steps[inst];
   if (firstLaunch && traced) {
     // the initial record: buoyancy at the initial pose, no inflow yet (simulation.cpp:137-145, 204-206)
-    Frame F;
-    FrameF f;
-    pose_and_buoyancy<Mixed>(S, M, B, tid, F, f);
+    buoyancy_partials<Mixed>(P, B, tid);
     __syncthreads();
     if (tid == 0) {
       double V, bx, by, bz, top;
       finish_buoyancy(B, V, bx, by, bz, top);
       B.scal[O_INFLOW] = 0; B.scal[O_VB] = V; B.scal[O_TOP] = top; B.scal[O_BX] = bx; B.scal[O_BY] = by; B.scal[O_BZ] = bz;
-      const size_t rs = static_cast<size_t>(Bt.recSize);
-      write_record(Bt.denseRec + static_cast<size_t>(inst) * (Bt.denseSteps + 1) * rs, B, NN);
-      write_record(Bt.sampleRec + static_cast<size_t>(inst) * Bt.maxSamples * rs, B, NN);
-      Bt.denseCount[inst] = 1; Bt.sampleCount[inst] = 1; Bt.nextSample[inst] = Bt.sampleEvery;
+      const size_t rs = static_cast<size_t>(P.Bt.recSize);
+      write_record(P.Bt.denseRec + static_cast<size_t>(inst) * (P.Bt.denseSteps + 1) * rs, B, NN);
+      write_record(P.Bt.sampleRec + static_cast<size_t>(inst) * P.Bt.maxSamples * rs, B, NN);
+      P.Bt.denseCount[inst] = 1; P.Bt.sampleCount[inst] = 1; P.Bt.nextSample[inst] = P.Bt.sampleEvery;
     }
     __syncthreads();
   }
 
+  const bool curves = P.Bt.curveEvery > 0;
   for (long long s = 0; s < maxSteps; ++s) {
     if (B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= tMax) break;  // uniform: LDS values after a barrier
-    if (Bt.curveEvery > 0 && tid == 0 && B.scal[S_T] >= Bt.nextCurve[inst] && Bt.curveCount[inst] < Bt.curveMax) {
-      double* rec = Bt.curveRec + (static_cast<size_t>(inst) * Bt.curveMax + Bt.curveCount[inst]) * kCurveValues;
-      readouts7(M, B, NN, rec);
-      Bt.curveCount[inst]++;
-      Bt.nextCurve[inst] += Bt.curveEvery;
+    // a readout-curve record when due (simulation.cpp:175-176); uniform condition on LDS values
+    if (curves && B.scal[S_T] >= B.scal[S_NEXTCURVE] && B.scal[S_CURVECOUNT] < P.Bt.curveMax) {
+      if (tid == 0) {
+        SSG double* rec = P.Bt.curveRec + (static_cast<size_t>(inst) * P.Bt.curveMax + static_cast<size_t>(B.scal[S_CURVECOUNT])) * kCurveValues;
+        readouts7(P, B, rec);
+        B.scal[S_CURVECOUNT] += 1;
+        B.scal[S_NEXTCURVE] += P.Bt.curveEvery;
+      }
+      __syncthreads();  // the step below rewrites the volumes the record read
     }
-    device_step<Mixed>(S, M, B, tid);
+    device_step<Mixed>(P, B, tid);
     ++done;
-    if (traced && tid == 0) {
-      const size_t rs = static_cast<size_t>(Bt.recSize);
-      if (done <= Bt.denseSteps) {
-        const int k = Bt.denseCount[inst]++;
-        write_record(Bt.denseRec + (static_cast<size_t>(inst) * (Bt.denseSteps + 1) + k) * rs, B, NN);
+    if (traced) {
+      if (tid == 0) {
+        const size_t rs = static_cast<size_t>(P.Bt.recSize);
+        if (done <= P.Bt.denseSteps) {
+          const int k = P.Bt.denseCount[inst]++;
+          write_record(P.Bt.denseRec + (static_cast<size_t>(inst) * (P.Bt.denseSteps + 1) + k) * rs, B, NN);
+        }
+        if (B.scal[S_T] >= P.Bt.nextSample[inst] - 1e-9 && P.Bt.sampleCount[inst] < P.Bt.maxSamples) {
+          const int k = P.Bt.sampleCount[inst]++;
+          write_record(P.Bt.sampleRec + (static_cast<size_t>(inst) * P.Bt.maxSamples + k) * rs, B, NN);
+          P.Bt.nextSample[inst] += P.Bt.sampleEvery;
+        }
       }
-      if (B.scal[S_T] >= Bt.nextSample[inst] - 1e-9 && Bt.sampleCount[inst] < Bt.maxSamples) {
-        const int k = Bt.sampleCount[inst]++;
-        write_record(Bt.sampleRec + (static_cast<size_t>(inst) * Bt.maxSamples + k) * rs, B, NN);
-        Bt.nextSample[inst] += Bt.sampleEvery;
-      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // store the instance
   if (tid < 7) {
-    Bt.st[inst * 7 + tid] = B.scal[S_T + tid];
-    Bt.out[inst * 7 + tid] = B.scal[O_INFLOW + tid];
+    P.Bt.st[inst * 7 + tid] = B.scal[S_T + tid];
+    P.Bt.out[inst * 7 + tid] = B.scal[O_INFLOW + tid];
   }
   if (tid == 0) {
-    Bt.evCount[inst] = static_cast<int32_t>(B.scal[S_EVCOUNT]);
-    Bt.foundered[inst] = static_cast<uint8_t>(B.scal[S_FOUNDERED]);
-    Bt.founderT[inst] = B.scal[S_FOUNDERT];
-    Bt.steps[inst] = done;
-    readouts7(M, B, NN, Bt.final7 + static_cast<size_t>(inst) * 7);
+    P.Bt.evCount[inst] = static_cast<int32_t>(B.scal[S_EVCOUNT]);
+    P.Bt.foundered[inst] = static_cast<uint8_t>(B.scal[S_FOUNDERED]);
+    P.Bt.founderT[inst] = B.scal[S_FOUNDERT];
+    P.Bt.steps[inst] = done;
+    if (P.Bt.curveEvery > 0) {
+      P.Bt.nextCurve[inst] = B.scal[S_NEXTCURVE];
+      P.Bt.curveCount[inst] = static_cast<int32_t>(B.scal[S_CURVECOUNT]);
+    }
+    readouts7(P, B, P.Bt.final7 + static_cast<size_t>(inst) * 7);
   }
   for (int n = tid; n < NN; n += kThreads) {
-    Bt.vol[iN + n] = B.vol[n]; Bt.level[iN + n] = B.level[n];
-    Bt.cx[iN + n] = B.cx[n]; Bt.cy[iN + n] = B.cy[n]; Bt.cz[iN + n] = B.cz[n];
+    P.Bt.vol[iN + n] = B.vol[n]; P.Bt.level[iN + n] = B.level[n];
+    P.Bt.cx[iN + n] = B.cx[n]; P.Bt.cy[iN + n] = B.cy[n]; P.Bt.cz[iN + n] = B.cz[n];
   }
-  for (int e = tid; e < M.evCap; e += kThreads) {
-    const size_t k = static_cast<size_t>(inst) * M.evCap + e;
-    Bt.evT[k] = B.evT[e]; Bt.evId[k] = B.evId[e]; Bt.evKind[k] = B.evKind[e];
+  for (int e = tid; e < evCap; e += kThreads) {
+    const size_t k = static_cast<size_t>(inst) * evCap + e;
+    P.Bt.evT[k] = B.evT[e]; P.Bt.evId[k] = B.evId[e]; P.Bt.evKind[k] = B.evKind[e];
   }
-  for (int m = tid; m < M.nmon; m += kThreads) Bt.overT[static_cast<size_t>(inst) * M.nmon + m] = B.overT[m];
-  for (int m = tid; m < M.nmark; m += kThreads) Bt.markT[static_cast<size_t>(inst) * M.nmark + m] = B.markT[m];
+  for (int m = tid; m < nmon; m += kThreads) P.Bt.overT[static_cast<size_t>(inst) * nmon + m] = B.overT[m];
+  for (int m = tid; m < nmark; m += kThreads) P.Bt.markT[static_cast<size_t>(inst) * nmark + m] = B.markT[m];
 }
 
 // ------------------------------------------------------------------------------------------- host side
@@ -160,6 +176,12 @@ __global__ __launch_bounds__(kThreads) void run_chunk(DevShip S, DevNet M, DevBa
 namespace {
 
 thread_local std::string g_error;
+
+// device addresses as the kernel's address-space-qualified pointer types
+template <class T>
+const SSG T* G(const T* p) { return (const SSG T*)p; }
+template <class T>
+SSG T* W(T* p) { return (SSG T*)p; }
 
 struct Fail {
   ss_status status;
@@ -270,6 +292,7 @@ struct ss_batch {
       nextCurve;
   Dev<int32_t> evId, evCount, denseCount, sampleCount, curveCount;
   Dev<long long> steps;
+  Dev<Params> params;
   // host staging of the per-instance tables until run()
   std::vector<uint8_t> hEn;
   std::vector<double> hArea, hTOn, hSt, hVol, hLevel, hCx, hCy, hCz;
@@ -282,26 +305,29 @@ struct ss_batch {
   DevShip shipView() const {
     DevShip s{};
     s.ncol = ncol; s.nseg = nseg; s.nn = NN;
-    s.x = x.p; s.y = y.p; s.dxdy = dxdy.p; s.zlo = zlo.p; s.zhi = zhi.p;
-    s.xf = xf.p; s.yf = yf.p; s.dxdyf = dxdyf.p; s.zlof = zlof.p; s.zhif = zhif.p;
-    s.segStart = segStart.p; s.segCount = segCount.p;
-    s.sx = sx.p; s.sy = sy.p; s.sA = sA.p; s.sa = sa.p; s.se = se.p;
-    s.sxf = sxf.p; s.syf = syf.p; s.sAf = sAf.p; s.saf = saf.p; s.sef = sef.p;
-    s.mu = mu.p; s.vmax = vmax.p; s.amin = amin.p; s.aov = aov.p;
+    s.x = G(x.p); s.y = G(y.p); s.dxdy = G(dxdy.p); s.zlo = G(zlo.p); s.zhi = G(zhi.p);
+    s.xf = G(xf.p); s.yf = G(yf.p); s.dxdyf = G(dxdyf.p); s.zlof = G(zlof.p); s.zhif = G(zhif.p);
+    s.segStart = G(segStart.p); s.segCount = G(segCount.p);
+    s.sx = G(sx.p); s.sy = G(sy.p); s.sA = G(sA.p); s.sa = G(sa.p); s.se = G(se.p);
+    s.sxf = G(sxf.p); s.syf = G(syf.p); s.sAf = G(sAf.p); s.saf = G(saf.p); s.sef = G(sef.p);
+    s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
     return s;
   }
 
   DevBatch batchView() const {
     DevBatch b{};
     b.count = opt.count;
-    b.en = en.p; b.area = area.p; b.tOn = tOn.p;
-    b.st = st.p; b.out = out.p;
-    b.vol = vol.p; b.level = level.p; b.cx = ccx.p; b.cy = ccy.p; b.cz = ccz.p;
-    b.evT = evT.p; b.evId = evId.p; b.evKind = evKind.p; b.evCount = evCount.p;
-    b.overT = overT.p; b.markT = markT.p; b.foundered = foundered.p; b.founderT = founderT.p; b.steps = steps.p; b.final7 = final7.p;
+    b.en = G(en.p); b.area = G(area.p); b.tOn = G(tOn.p);
+    b.st = W(st.p); b.out = W(out.p);
+    b.vol = W(vol.p); b.level = W(level.p); b.cx = W(ccx.p); b.cy = W(ccy.p); b.cz = W(ccz.p);
+    b.evT = W(evT.p); b.evId = W(evId.p); b.evKind = W(evKind.p); b.evCount = W(evCount.p);
+    b.overT = W(overT.p); b.markT = W(markT.p); b.foundered = W(foundered.p); b.founderT = W(founderT.p); b.steps = W(steps.p);
+    b.final7 = W(final7.p);
     b.traced = opt.traced; b.denseSteps = opt.denseSteps; b.maxSamples = maxSamples; b.recSize = recSize; b.sampleEvery = opt.sampleEvery;
-    b.denseRec = denseRec.p; b.sampleRec = sampleRec.p; b.denseCount = denseCount.p; b.sampleCount = sampleCount.p; b.nextSample = nextSample.p;
-    b.curveEvery = opt.curveEvery; b.curveMax = curveMax; b.curveRec = curveRec.p; b.curveCount = curveCount.p; b.nextCurve = nextCurve.p;
+    b.denseRec = W(denseRec.p); b.sampleRec = W(sampleRec.p); b.denseCount = W(denseCount.p); b.sampleCount = W(sampleCount.p);
+    b.nextSample = W(nextSample.p);
+    b.curveEvery = opt.curveEvery; b.curveMax = curveMax; b.curveRec = W(curveRec.p); b.curveCount = W(curveCount.p);
+    b.nextCurve = W(nextCurve.p);
     return b;
   }
 };
@@ -321,7 +347,7 @@ LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t n
   L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC);
   L.passes = take(7 * kUnits * maxGroup);
   L.warpHyd = take(5 * kUnits);
-  L.trig = take(4);
+  L.frame = take(F_COUNT);
   L.scal = take(S_COUNT);
   L.over = take(nmon); L.overT = take(nmon); L.markT = take(nmark); L.evT = take(evCap);
   L.deg = take(ints(NN)); L.evId = take(ints(evCap));
@@ -544,14 +570,14 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
 
     DevNet& d = b.net;
     d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
-    d.ng = M.ngroup; d.gStart = b.gStart.p; d.gCount = b.gCount.p; d.gNodes = b.gNodes.p; d.gMode = b.gMode.p;
-    d.gpx = b.gpx.p; d.gpy = b.gpy.p; d.gpz = b.gpz.p; d.gMargin = M.gMargin;
-    d.nc = M.nconn; d.ca = b.ca.p; d.cb = b.cb.p; d.law = b.law.p; d.cx = b.cx.p; d.cy = b.cy.p; d.cz = b.cz.p; d.coef = b.coef.p;
-    d.skipGroup = b.skipGroup.p;
-    d.incStart = b.incStart.p; d.incCount = b.incCount.p; d.incConn = b.incConn.p; d.incSide = b.incSide.p;
-    d.seaCount = static_cast<int32_t>(sea.size()); d.sea = b.sea.p;
-    d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = b.monStart.p; d.monCount = b.monCount.p; d.monConn = b.monConn.p;
-    d.nmark = M.nmark; d.mkx = b.mkx.p; d.mky = b.mky.p; d.mkz = b.mkz.p; d.mkWhen = b.mkWhen.p;
+    d.ng = M.ngroup; d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
+    d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
+    d.nc = M.nconn; d.ca = G(b.ca.p); d.cb = G(b.cb.p); d.law = G(b.law.p); d.cx = G(b.cx.p); d.cy = G(b.cy.p); d.cz = G(b.cz.p); d.coef = G(b.coef.p);
+    d.skipGroup = G(b.skipGroup.p);
+    d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.incConn = G(b.incConn.p); d.incSide = G(b.incSide.p);
+    d.seaCount = static_cast<int32_t>(sea.size()); d.sea = G(b.sea.p);
+    d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = G(b.monStart.p); d.monCount = G(b.monCount.p); d.monConn = G(b.monConn.p);
+    d.nmark = M.nmark; d.mkx = G(b.mkx.p); d.mky = G(b.mky.p); d.mkz = G(b.mkz.p); d.mkWhen = G(b.mkWhen.p);
     d.hullTopBelow = M.founder[0]; d.pitchAbsAbove = M.founder[1]; d.buoyancyBelow = M.founder[2]; d.minT = M.founder[3];
     const double* bd = M.body;
     d.ms = bd[0]; d.g0x = bd[2]; d.g0y = bd[3]; d.g0z = bd[4]; d.XO = bd[5]; d.YO = bd[6]; d.mh0 = bd[7]; d.Ith0 = bd[8]; d.Iph0 = bd[9];
@@ -638,8 +664,12 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
       b->nextCurve.fill(0.0, count);  // the first record is the initial state, as in the oracle's run
     }
     (void)nn;
-    const DevShip ship = b->shipView();
-    const DevBatch batch = b->batchView();
+    Params prm{};
+    prm.S = b->shipView();
+    prm.M = b->net;
+    prm.Bt = b->batchView();
+    prm.L = b->layout;
+    b->params.upload(&prm, 1);
     const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
     auto kernel = b->mixed ? run_chunk<true> : run_chunk<false>;
     if (b->ldsBytes > 64 * 1024)
@@ -657,8 +687,8 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
     std::vector<double> st(count * 7);
     for (;;) {
       hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
-      hipLaunchKernelGGL(kernel, dim3(b->opt.count), dim3(kThreads), b->ldsBytes, nullptr, ship, b->net, batch, b->layout, maxSteps, b->opt.tMax,
-                         first);
+      hipLaunchKernelGGL(kernel, dim3(b->opt.count), dim3(kThreads), b->ldsBytes, nullptr, static_cast<const Params*>(b->params.p), maxSteps,
+                         b->opt.tMax, first);
       hip_check(hipGetLastError(), "kernel launch");
       hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
       hip_check(hipEventSynchronize(e1), "kernel execution");

@@ -6,16 +6,18 @@
 // have a canonical order and are reproduced as such:
 //   - buoyancy: 256 logical partials, partial t holding columns t, t+256, ...; each 32-partial group reduced by
 //     the halving tree (l absorbs l+16, l+8, ... l+1); the 8 group results combined 0..7 in order
-//     (reduce.hpp:12-107, hydro.hpp:242-258). Here a logical 32-lane group is one half of a wave64.
+//     (reduce.hpp:12-107, hydro.hpp:242-258). A logical 32-lane group is one half of a wave64.
 //   - node pass: 32 lane partials over the node's segments with stride 32, then the same tree (hydro.hpp:306-316).
-//   - every other sum (sea inflow, node gathers, loads, monitors) is serial in the reference's index order.
-// All other arithmetic is element-wise and is written operation for operation as in the reference; compile with
-// -ffp-contract=off so that nothing is fused.
+//   - every other sum (sea inflow, node gathers, loads, monitors) is serial in the reference's index order, one
+//     chain per lane where several independent chains exist.
+// All other arithmetic is element-wise and written operation for operation as in the reference; the library is
+// compiled with -ffp-contract=off so that nothing is fused.
 //
-// Work layout per step (barriers marked |):
-//   pose (4 lanes evaluate cos/sin of pitch and roll) | buoyancy partials + one level-solve unit per 32 lanes
-//   | per node: effective heads, degrees | per connection: flows | per node: gather, volume update, load terms;
-//   monitor sums; sea inflow | lanes 0-7 of wave 0: the serial tail (loads, integration, events) |
+// Work layout of a step (| = workgroup barrier):
+//   buoyancy partials + the level solve, one logical 32-lane group per free-surface group | per node: effective
+//   heads and limiter degrees | per connection: flows | per node: gather, volume update, load terms; per monitor:
+//   flow sums; one lane: sea inflow | wave 0: the tail (loads as seven parallel chains, integration, events, and
+//   the pose of the next step) |
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,10 +26,14 @@
 
 #include "fdlibm_dev.hpp"
 
+#define SSG __attribute__((address_space(1)))  // global memory
+#define SSC __attribute__((address_space(4)))  // constant memory (scalar loads)
+#define SSL __attribute__((address_space(3)))  // LDS
+
 namespace ssa {
 
-constexpr int kThreads = 256;           // logical block of the canonical buoyancy order
-constexpr int kLanes = 32;              // logical warp of the canonical trees
+constexpr int kThreads = 256;  // logical block of the canonical buoyancy order
+constexpr int kLanes = 32;     // logical warp of the canonical trees
 constexpr int kUnits = kThreads / kLanes;
 
 // scheme v1 (reference: engine/include/sinksim/kernel/scheme.hpp:11-28)
@@ -51,6 +57,11 @@ constexpr double kPi = 3.141592653589793;
 __device__ __forceinline__ double jmax(double a, double b) { return (b > a) ? b : a; }
 __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b : a; }
 
+using gcd = const SSG double;
+using gcf = const SSG float;
+using gci = const SSG int32_t;
+using gcb = const SSG uint8_t;
+
 // ------------------------------------------------------------------------------------------- device views
 
 // The hull, shared by every simulation. Segment arrays are pre-gathered per segment (x, y and footprint of the
@@ -58,37 +69,37 @@ __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b 
 // rounded once on the host, which is what the portable32 arithmetic converts them to anyway.
 struct DevShip {
   int32_t ncol, nseg, nn;
-  const double *x, *y, *dxdy, *zlo, *zhi;
-  const float *xf, *yf, *dxdyf, *zlof, *zhif;
-  const int32_t *segStart, *segCount;
-  const double *sx, *sy, *sA, *sa, *se;
-  const float *sxf, *syf, *sAf, *saf, *sef;
-  const double *mu, *vmax, *amin, *aov;
+  gcd *x, *y, *dxdy, *zlo, *zhi;
+  gcf *xf, *yf, *dxdyf, *zlof, *zhif;
+  gci *segStart, *segCount;
+  gcd *sx, *sy, *sA, *sa, *se;
+  gcf *sxf, *syf, *sAf, *saf, *sef;
+  gcd *mu, *vmax, *amin, *aov;
 };
 
 // The scenario's immutable network and constants.
 struct DevNet {
   double rho, g, sq2g, dt;
   int32_t ng;
-  const int32_t *gStart, *gCount, *gNodes;
-  const uint8_t* gMode;
-  const double *gpx, *gpy, *gpz;
+  gci *gStart, *gCount, *gNodes;
+  gcb* gMode;
+  gcd *gpx, *gpy, *gpz;
   double gMargin;
   int32_t nc;
-  const int32_t *ca, *cb;
-  const uint8_t* law;
-  const double *cx, *cy, *cz, *coef;
-  const int32_t* skipGroup;
-  const int32_t *incStart, *incCount, *incConn;
-  const uint8_t* incSide;
+  gci *ca, *cb;
+  gcb* law;
+  gcd *cx, *cy, *cz, *coef;
+  gci* skipGroup;
+  gci *incStart, *incCount, *incConn;
+  gcb* incSide;
   int32_t seaCount;
-  const int32_t* sea;
+  gci* sea;
   int32_t nmon;
   double monThreshold;
-  const int32_t *monStart, *monCount, *monConn;  // per monitor, its connections in increasing order
+  gci *monStart, *monCount, *monConn;  // per monitor, its connections in increasing order
   int32_t nmark;
-  const double *mkx, *mky, *mkz;
-  const uint8_t* mkWhen;
+  gcd *mkx, *mky, *mkz;
+  gcb* mkWhen;
   double hullTopBelow, pitchAbsAbove, buoyancyBelow, minT;
   double ms, g0x, g0y, g0z, XO, YO, mh0, Ith0, Iph0, cDz, cDth, cDph, qz, qth, qph;
   double xFP, xAP;
@@ -98,109 +109,159 @@ struct DevNet {
 // Per-instance global arrays of the batch.
 struct DevBatch {
   int32_t count;
-  const uint8_t* en;
-  const double *area, *tOn;  // [count * nc]
-  double* st;                // [count * 7]: t zO pitch roll vz wth wph
-  double* out;               // [count * 7]: inflow Vb hullTop mw bx by bz
-  double *vol, *level, *cx, *cy, *cz;  // [count * nn]
-  double* evT;
-  int32_t* evId;
-  uint8_t* evKind;
-  int32_t* evCount;
-  double *overT, *markT;
-  uint8_t* foundered;
-  double* founderT;
-  long long* steps;
-  double* final7;            // [count * 7] readouts at the end of the last launch
+  gcb* en;
+  gcd *area, *tOn;                      // [count * nc]
+  SSG double* st;                       // [count * 7]: t zO pitch roll vz wth wph
+  SSG double* out;                      // [count * 7]: inflow Vb hullTop mw bx by bz
+  SSG double *vol, *level, *cx, *cy, *cz;  // [count * nn]
+  SSG double* evT;
+  SSG int32_t* evId;
+  SSG uint8_t* evKind;
+  SSG int32_t* evCount;
+  SSG double *overT, *markT;
+  SSG uint8_t* foundered;
+  SSG double* founderT;
+  SSG long long* steps;
+  SSG double* final7;                   // [count * 7] readouts at the end of the last launch
   int32_t traced, denseSteps, maxSamples, recSize;
   double sampleEvery;
-  double *denseRec, *sampleRec;
-  int32_t *denseCount, *sampleCount;
-  double* nextSample;
+  SSG double *denseRec, *sampleRec;
+  SSG int32_t *denseCount, *sampleCount;
+  SSG double* nextSample;
   double curveEvery;
   int32_t curveMax;
-  double* curveRec;
-  int32_t* curveCount;
-  double* nextCurve;
+  SSG double* curveRec;
+  SSG int32_t* curveCount;
+  SSG double* nextCurve;
 };
 
 // LDS layout of one simulation, as offsets in doubles (the host computes it: lds_layout in batch.hip).
 struct LdsLayout {
   int32_t vol, level, cx, cy, cz, area, zmin, zmax, hEff, aEff, excess;
   int32_t tX, tY, tF, tM, tIp, tIr;
-  int32_t dV, q, carea, ctOn, passes, warpHyd, trig, scal, over, overT, markT, evT;
-  int32_t deg, evId;      // int32 arrays, offsets in doubles
+  int32_t dV, q, carea, ctOn, passes, warpHyd, frame, scal, over, overT, markT, evT;
+  int32_t deg, evId;            // int32 arrays, offsets in doubles
   int32_t merged, cen, evKind;  // byte arrays, offsets in doubles
-  int32_t total;          // doubles
+  int32_t total;                // doubles
 };
+
+// Everything a launch needs, in constant memory; device code reads fields on demand with scalar loads.
+struct Params {
+  DevShip S;
+  DevNet M;
+  DevBatch Bt;
+  LdsLayout L;
+};
+using PC = const SSC Params;
+
+// frame slots: R00 R01 R02 R10 R11 R12 R20 R21 R22 tx ty tz cth iR, then the f32 frame R20 R21 R22 tz iR
+enum : int { F_R00 = 0, F_R01, F_R02, F_R10, F_R11, F_R12, F_R20, F_R21, F_R22, F_TX, F_TY, F_TZ, F_CTH, F_IR, F_F32, F_COUNT = F_F32 + 3 };
 
 // scalar slots of the `scal` block
 enum : int { S_T = 0, S_ZO, S_PITCH, S_ROLL, S_VZ, S_WTH, S_WPH, O_INFLOW, O_VB, O_TOP, O_MW, O_BX, O_BY, O_BZ, S_SEA, S_FOUNDERT, S_EVCOUNT,
-             S_FOUNDERED, S_COUNT };
+             S_FOUNDERED, S_NEXTCURVE, S_CURVECOUNT, S_COUNT };
 
 struct Blk {
-  double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
-  double *tX, *tY, *tF, *tM, *tIp, *tIr;
-  double *dV, *q, *carea, *ctOn, *passes, *warpHyd, *trig, *scal, *over, *overT, *markT, *evT;
-  int32_t *deg, *evId;
-  uint8_t *merged, *cen, *evKind;
+  SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
+  SSL double *tX, *tY, *tF, *tM, *tIp, *tIr;
+  SSL double *dV, *q, *carea, *ctOn, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
+  SSL int32_t *deg, *evId;
+  SSL uint8_t *merged, *cen, *evKind;
 };
 
-__device__ inline Blk carve(double* base, const LdsLayout& L) {
+__device__ inline Blk carve(SSL double* base, PC& P) {
+  const SSC LdsLayout& L = P.L;
   Blk b;
   b.vol = base + L.vol; b.level = base + L.level; b.cx = base + L.cx; b.cy = base + L.cy; b.cz = base + L.cz;
   b.area = base + L.area; b.zmin = base + L.zmin; b.zmax = base + L.zmax; b.hEff = base + L.hEff; b.aEff = base + L.aEff;
   b.excess = base + L.excess;
   b.tX = base + L.tX; b.tY = base + L.tY; b.tF = base + L.tF; b.tM = base + L.tM; b.tIp = base + L.tIp; b.tIr = base + L.tIr;
   b.dV = base + L.dV; b.q = base + L.q; b.carea = base + L.carea; b.ctOn = base + L.ctOn; b.passes = base + L.passes;
-  b.warpHyd = base + L.warpHyd; b.trig = base + L.trig; b.scal = base + L.scal; b.over = base + L.over; b.overT = base + L.overT;
+  b.warpHyd = base + L.warpHyd; b.frame = base + L.frame; b.scal = base + L.scal; b.over = base + L.over; b.overT = base + L.overT;
   b.markT = base + L.markT; b.evT = base + L.evT;
-  b.deg = reinterpret_cast<int32_t*>(base + L.deg); b.evId = reinterpret_cast<int32_t*>(base + L.evId);
-  b.merged = reinterpret_cast<uint8_t*>(base + L.merged); b.cen = reinterpret_cast<uint8_t*>(base + L.cen);
-  b.evKind = reinterpret_cast<uint8_t*>(base + L.evKind);
+  b.deg = reinterpret_cast<SSL int32_t*>(base + L.deg);
+  b.evId = reinterpret_cast<SSL int32_t*>(base + L.evId);
+  b.merged = reinterpret_cast<SSL uint8_t*>(base + L.merged);
+  b.cen = reinterpret_cast<SSL uint8_t*>(base + L.cen);
+  b.evKind = reinterpret_cast<SSL uint8_t*>(base + L.evKind);
   return b;
 }
 
 // ------------------------------------------------------------------------------------------- pose
 
-struct Frame {
-  double R00, R01, R02, R10, R11, R12, R20, R21, R22, tx, ty, tz, cth, iR;
-};
-struct FrameF {
-  float R20, R21, R22, tz, iR;
-};
+// Out of line: called four lanes at a time once per step, and for the 30 s readouts.
+__device__ __noinline__ double trig(double x, int wantCos) { return fdl::sin_or_cos(x, wantCos); }
+__device__ __noinline__ double dpow(double x, double y) { return fdl::pow(x, y); }
 
-// reference: kernel/frame.hpp:10-21, from the four trigonometric values
-__device__ inline Frame make_frame(const DevNet& M, double zO, double cth, double sth, double cph, double sph) {
-  Frame F;
-  F.R00 = cth; F.R01 = sth * sph; F.R02 = sth * cph;
-  F.R10 = 0;   F.R11 = cph;       F.R12 = -sph;
-  F.R20 = -sth; F.R21 = cth * sph; F.R22 = cth * cph;
-  F.tx = M.XO - (F.R00 * M.g0x + F.R01 * M.g0y + F.R02 * M.g0z);
-  F.ty = M.YO - (F.R10 * M.g0x + F.R11 * M.g0y + F.R12 * M.g0z);
-  F.tz = zO - (F.R20 * M.g0x + F.R21 * M.g0y + F.R22 * M.g0z);
-  F.cth = cth;
-  F.iR = 1 / F.R22;  // hydro.hpp:209,245,264,309
-  return F;
+// reference: kernel/frame.hpp:10-21, written to the frame slots, with iR = 1 / R22 (hydro.hpp:209,245,264,309)
+// and the f32 frame of reduce.hpp:118-126
+__device__ inline void write_frame(PC& P, SSL double* fr, double zO, double cth, double sth, double cph, double sph) {
+  const double R00 = cth, R01 = sth * sph, R02 = sth * cph;
+  const double R10 = 0, R11 = cph, R12 = -sph;
+  const double R20 = -sth, R21 = cth * sph, R22 = cth * cph;
+  const double tx = P.M.XO - (R00 * P.M.g0x + R01 * P.M.g0y + R02 * P.M.g0z);
+  const double ty = P.M.YO - (R10 * P.M.g0x + R11 * P.M.g0y + R12 * P.M.g0z);
+  const double tz = zO - (R20 * P.M.g0x + R21 * P.M.g0y + R22 * P.M.g0z);
+  fr[F_R00] = R00; fr[F_R01] = R01; fr[F_R02] = R02; fr[F_R10] = R10; fr[F_R11] = R11; fr[F_R12] = R12;
+  fr[F_R20] = R20; fr[F_R21] = R21; fr[F_R22] = R22; fr[F_TX] = tx; fr[F_TY] = ty; fr[F_TZ] = tz; fr[F_CTH] = cth;
+  fr[F_IR] = 1 / R22;
+  const float f20 = static_cast<float>(R20), f21 = static_cast<float>(R21), f22 = static_cast<float>(R22), ftz = static_cast<float>(tz);
+  SSL float* ff = reinterpret_cast<SSL float*>(fr + F_F32);
+  ff[0] = f20; ff[1] = f21; ff[2] = f22; ff[3] = ftz; ff[4] = 1.0f / f22;
 }
 
-// reference: kernel/reduce.hpp:118-126
-__device__ inline FrameF frame_f(const Frame& F) {
-  FrameF f;
-  f.R20 = static_cast<float>(F.R20);
-  f.R21 = static_cast<float>(F.R21);
-  f.R22 = static_cast<float>(F.R22);
-  f.tz = static_cast<float>(F.tz);
-  f.iR = 1.0f / f.R22;
-  return f;
+// The pose of the current state into the frame slots: lanes 0..3 of wave 0 evaluate cos/sin of pitch and roll,
+// lane 0 writes the frame. Call from wave 0 only; visible to the workgroup after the next barrier.
+__device__ inline void pose_wave0(PC& P, const Blk& B, int lane) {
+  double v = 0;
+  if (lane < 4) v = trig(lane < 2 ? B.scal[S_PITCH] : B.scal[S_ROLL], (lane & 1) == 0 ? 1 : 0);
+  const double cth = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 0) << 32) |
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 0));
+  const double sth = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 1) << 32) |
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 1));
+  const double cph = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 2) << 32) |
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 2));
+  const double sph = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 3) << 32) |
+                                                    (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 3));
+  if (lane == 0) write_frame(P, B.frame, B.scal[S_ZO], cth, sth, cph, sph);
 }
 
-__device__ inline double world_z(const Frame& F, double x, double y, double z) { return F.R20 * x + F.R21 * y + F.R22 * z + F.tz; }
-
-// cos(pitch), sin(pitch), cos(roll), sin(roll) for slot 0..3 (frame.hpp:11)
-__device__ inline double pose_trig(int slot, double pitch, double roll) {
-  return fdl::sin_or_cos(slot < 2 ? pitch : roll, (slot & 1) == 0 ? 1 : 0);
+__device__ __forceinline__ double world_z(const SSL double* fr, double x, double y, double z) {
+  return fr[F_R20] * x + fr[F_R21] * y + fr[F_R22] * z + fr[F_TZ];
 }
+
+// ------------------------------------------------------------------------------------------- cross-lane
+
+// Lane l of the logical 32-lane group receives lane l + off (off = 8, 4, 2, 1 by DPP row shifts inside a row of
+// 16; off = 16 by an LDS-unit swizzle that swaps the two rows of each group). Only lanes l < off use the result.
+template <int OFF>
+__device__ __forceinline__ uint32_t down32(uint32_t v) {
+  if constexpr (OFF == 16) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x401F));
+  else return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x100 + OFF, 0xF, 0xF, false));
+}
+template <int OFF>
+__device__ __forceinline__ double down(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = down32<OFF>(static_cast<uint32_t>(b)), hi = down32<OFF>(static_cast<uint32_t>(b >> 32));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+template <int OFF>
+__device__ __forceinline__ float down(float v) {
+  return __builtin_bit_cast(float, down32<OFF>(__builtin_bit_cast(uint32_t, v)));
+}
+
+// lane 0 of the logical group (wave lane 0 or 32) to every lane of the group
+__device__ __forceinline__ uint32_t lead32(uint32_t v, bool upper) {
+  const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+  const uint32_t b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+  return upper ? b : a;
+}
+__device__ __forceinline__ double lead(double v, bool upper) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(lead32(static_cast<uint32_t>(b >> 32), upper)) << 32) |
+                                        lead32(static_cast<uint32_t>(b), upper));
+}
+__device__ __forceinline__ float lead(float v, bool upper) { return __builtin_bit_cast(float, lead32(__builtin_bit_cast(uint32_t, v), upper)); }
 
 // ------------------------------------------------------------------------------------------- partials
 
@@ -213,81 +274,83 @@ struct PP {
   T vol, dv, mx, my, mz, zmin, zmax;
 };
 
-template <class T>
-__device__ __forceinline__ void absorb(HP<T>& a, const HP<T>& b) {
-  a.V += b.V; a.Mx += b.Mx; a.My += b.My; a.Mz += b.Mz;
-  if (b.top > a.top) a.top = b.top;
-}
-template <class T>
-__device__ __forceinline__ void absorb(PP<T>& a, const PP<T>& b) {
-  a.vol += b.vol; a.dv += b.dv; a.mx += b.mx; a.my += b.my; a.mz += b.mz;
-  if (b.zmin < a.zmin) a.zmin = b.zmin;
-  if (b.zmax > a.zmax) a.zmax = b.zmax;
-}
-
-// The halving tree of a logical 32-lane group (one half of the wave64): for off = 16 ... 1 lane l < off
-// absorbs lane l + off. Lanes >= off keep stale values that the tree never reads again.
-template <class T>
-__device__ __forceinline__ T down(T v, int off) { return __shfl_down(v, off, kLanes); }
-
-template <class T>
-__device__ __forceinline__ void tree32(HP<T>& p, int lane) {
-#pragma unroll
-  for (int off = kLanes / 2; off > 0; off >>= 1) {
-    HP<T> o;
-    o.V = down(p.V, off); o.Mx = down(p.Mx, off); o.My = down(p.My, off); o.Mz = down(p.Mz, off); o.top = down(p.top, off);
-    if (lane < off) absorb(p, o);
+template <int OFF, class T>
+__device__ __forceinline__ void tree_level(HP<T>& p, int lane) {
+  const T V = down<OFF>(p.V), Mx = down<OFF>(p.Mx), My = down<OFF>(p.My), Mz = down<OFF>(p.Mz), top = down<OFF>(p.top);
+  if (lane < OFF) {
+    p.V += V; p.Mx += Mx; p.My += My; p.Mz += Mz;
+    if (top > p.top) p.top = top;
   }
 }
-template <class T>
-__device__ __forceinline__ void tree32(PP<T>& p, int lane) {
-#pragma unroll
-  for (int off = kLanes / 2; off > 0; off >>= 1) {
-    PP<T> o;
-    o.vol = down(p.vol, off); o.dv = down(p.dv, off); o.mx = down(p.mx, off); o.my = down(p.my, off); o.mz = down(p.mz, off);
-    o.zmin = down(p.zmin, off); o.zmax = down(p.zmax, off);
-    if (lane < off) absorb(p, o);
+template <int OFF, class T>
+__device__ __forceinline__ void tree_level(PP<T>& p, int lane) {
+  const T vol = down<OFF>(p.vol), dv = down<OFF>(p.dv), mx = down<OFF>(p.mx), my = down<OFF>(p.my), mz = down<OFF>(p.mz);
+  const T zmin = down<OFF>(p.zmin), zmax = down<OFF>(p.zmax);
+  if (lane < OFF) {
+    p.vol += vol; p.dv += dv; p.mx += mx; p.my += my; p.mz += mz;
+    if (zmin < p.zmin) p.zmin = zmin;
+    if (zmax > p.zmax) p.zmax = zmax;
   }
 }
 
-// lane 0 of the logical group to every lane of it
-template <class T>
-__device__ __forceinline__ T from_lead(T v) { return __shfl(v, 0, kLanes); }
+// The halving tree of a logical 32-lane group (reduce.hpp:102-107): l absorbs l + 16, 8, 4, 2, 1; the total is in
+// lane 0 of the group.
+template <class P_>
+__device__ __forceinline__ void tree32(P_& p, int lane) {
+  tree_level<16>(p, lane);
+  tree_level<8>(p, lane);
+  tree_level<4>(p, lane);
+  tree_level<2>(p, lane);
+  tree_level<1>(p, lane);
+}
 
 // one column into a buoyancy partial (reduce.hpp:67-77 / 159-170)
-__device__ __forceinline__ void column(const DevShip& S, const Frame& F, int i, HP<double>& p) {
-  const double x = S.x[i], y = S.y[i], lo = S.zlo[i], hi = S.zhi[i];
+struct Fr64 {
+  double R20, R21, R22, tz, iR;
+};
+struct Fr32 {
+  float R20, R21, R22, tz, iR;
+};
+
+__device__ __forceinline__ Fr64 fr64(const SSL double* fr) { return Fr64{fr[F_R20], fr[F_R21], fr[F_R22], fr[F_TZ], fr[F_IR]}; }
+__device__ __forceinline__ Fr32 fr32(const SSL double* fr) {
+  const SSL float* f = reinterpret_cast<const SSL float*>(fr + F_F32);
+  return Fr32{f[0], f[1], f[2], f[3], f[4]};
+}
+
+__device__ __forceinline__ void column(PC& P, const Fr64& F, int i, HP<double>& p) {
+  const double x = P.S.x[i], y = P.S.y[i], lo = P.S.zlo[i], hi = P.S.zhi[i];
   const double base = F.R20 * x + F.R21 * y + F.tz;
   const double wt = base + F.R22 * hi;
   if (wt > p.top) p.top = wt;
   const double zp = -base * F.iR;
   if (zp <= lo) return;
   const double s = zp >= hi ? hi - lo : zp - lo;
-  const double v = S.dxdy[i] * s;
+  const double v = P.S.dxdy[i] * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
 }
-__device__ __forceinline__ void column(const DevShip& S, const FrameF& f, int i, HP<float>& p) {
-  const float x = S.xf[i], y = S.yf[i], lo = S.zlof[i], hi = S.zhif[i];
+__device__ __forceinline__ void column(PC& P, const Fr32& f, int i, HP<float>& p) {
+  const float x = P.S.xf[i], y = P.S.yf[i], lo = P.S.zlof[i], hi = P.S.zhif[i];
   const float base = f.R20 * x + f.R21 * y + f.tz;
   const float wt = base + f.R22 * hi;
   if (wt > p.top) p.top = wt;
   const float zp = -base * f.iR;
   if (zp <= lo) return;
   const float s = zp >= hi ? hi - lo : zp - lo;
-  const float v = S.dxdyf[i] * s;
+  const float v = P.S.dxdyf[i] * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
 }
 
 // one segment into a node-pass partial (reduce.hpp:80-99 / 172-192)
-__device__ __forceinline__ void segment(const DevShip& S, const Frame& F, double h, int j, PP<double>& p) {
-  const double x = S.sx[j], y = S.sy[j], a = S.sa[j], b = S.se[j];
+__device__ __forceinline__ void segment(PC& P, const Fr64& F, double h, int j, PP<double>& p) {
+  const double x = P.S.sx[j], y = P.S.sy[j], a = P.S.sa[j], b = P.S.se[j];
   const double base = F.R20 * x + F.R21 * y + F.tz;
   const double wa = base + F.R22 * a, wb = base + F.R22 * b;
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const double zp = (h - base) * F.iR;
   if (zp <= a) return;
-  const double A = S.sA[j];
+  const double A = P.S.sA[j];
   double L;
   if (zp >= b) {
     L = b - a;
@@ -298,15 +361,15 @@ __device__ __forceinline__ void segment(const DevShip& S, const Frame& F, double
   const double v = A * L;
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
 }
-__device__ __forceinline__ void segment(const DevShip& S, const FrameF& f, float h, int j, PP<float>& p) {
-  const float x = S.sxf[j], y = S.syf[j], a = S.saf[j], b = S.sef[j];
+__device__ __forceinline__ void segment(PC& P, const Fr32& f, float h, int j, PP<float>& p) {
+  const float x = P.S.sxf[j], y = P.S.syf[j], a = P.S.saf[j], b = P.S.sef[j];
   const float base = f.R20 * x + f.R21 * y + f.tz;
   const float wa = base + f.R22 * a, wb = base + f.R22 * b;
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const float zp = (h - base) * f.iR;
   if (zp <= a) return;
-  const float A = S.sAf[j];
+  const float A = P.S.sAf[j];
   float L;
   if (zp >= b) {
     L = b - a;
@@ -326,32 +389,35 @@ struct Pass {
 // (hydro.hpp:306-316 / 338-349; finish_pass hydro.hpp:295-303)
 template <bool Mixed>
 struct UnitPass {
-  const DevShip& S;
-  const Frame& F;
-  const FrameF& f;
+  PC& P;
+  const SSL double* fr;
   int lane;
+  bool upper;
   __device__ __forceinline__ Pass operator()(int n, double h) const {
-    const int s0 = S.segStart[n], s1 = s0 + S.segCount[n];
+    const int s0 = P.S.segStart[n], s1 = s0 + P.S.segCount[n];
     double vol, dv, mx, my, mz, zmin, zmax;
     if constexpr (Mixed) {
+      const Fr32 f = fr32(fr);
       const float hf = static_cast<float>(h);
       PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
-      for (int j = s0 + lane; j < s1; j += kLanes) segment(S, f, hf, j, p);
+      for (int j = s0 + lane; j < s1; j += kLanes) segment(P, f, hf, j, p);
       tree32(p, lane);
-      vol = static_cast<double>(from_lead(p.vol)); dv = static_cast<double>(from_lead(p.dv));
-      mx = static_cast<double>(from_lead(p.mx)); my = static_cast<double>(from_lead(p.my)); mz = static_cast<double>(from_lead(p.mz));
-      zmin = static_cast<double>(from_lead(p.zmin)); zmax = static_cast<double>(from_lead(p.zmax));
+      vol = static_cast<double>(lead(p.vol, upper)); dv = static_cast<double>(lead(p.dv, upper));
+      mx = static_cast<double>(lead(p.mx, upper)); my = static_cast<double>(lead(p.my, upper)); mz = static_cast<double>(lead(p.mz, upper));
+      zmin = static_cast<double>(lead(p.zmin, upper)); zmax = static_cast<double>(lead(p.zmax, upper));
     } else {
+      const Fr64 F = fr64(fr);
       PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
-      for (int j = s0 + lane; j < s1; j += kLanes) segment(S, F, h, j, p);
+      for (int j = s0 + lane; j < s1; j += kLanes) segment(P, F, h, j, p);
       tree32(p, lane);
-      vol = from_lead(p.vol); dv = from_lead(p.dv); mx = from_lead(p.mx); my = from_lead(p.my); mz = from_lead(p.mz);
-      zmin = from_lead(p.zmin); zmax = from_lead(p.zmax);
+      vol = lead(p.vol, upper); dv = lead(p.dv, upper); mx = lead(p.mx, upper); my = lead(p.my, upper); mz = lead(p.mz, upper);
+      zmin = lead(p.zmin, upper); zmax = lead(p.zmax, upper);
     }
-    const double mu = S.mu[n];
+    const double mu = P.S.mu[n];
+    const double iR = fr[F_IR];
     Pass r;
     r.vol = vol * mu;
-    r.dv = dv * mu * F.iR;
+    r.dv = dv * mu * iR;
     r.mx = mx * mu; r.my = my * mu; r.mz = mz * mu;
     r.zmin = zmin; r.zmax = zmax;
     return r;
@@ -364,32 +430,32 @@ __device__ __forceinline__ void put_centroid(const Blk& B, int n, const Pass& p)
   if (p.vol > kCentroidMinVolume) { B.cx[n] = p.mx / p.vol; B.cy[n] = p.my / p.vol; B.cz[n] = p.mz / p.vol; }
 }
 
-__device__ __forceinline__ void put_pass(double* P, int k, const Pass& p) {
-  double* q = P + 7 * k;
+__device__ __forceinline__ void put_pass(SSL double* Pp, int k, const Pass& p) {
+  SSL double* q = Pp + 7 * k;
   q[0] = p.vol; q[1] = p.dv; q[2] = p.mx; q[3] = p.my; q[4] = p.mz; q[5] = p.zmin; q[6] = p.zmax;
 }
-__device__ __forceinline__ Pass get_pass(const double* P, int k) {
-  const double* q = P + 7 * k;
+__device__ __forceinline__ Pass get_pass(const SSL double* Pp, int k) {
+  const SSL double* q = Pp + 7 * k;
   return Pass{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
 }
 
 // levels.hpp:13-49 with full = false (the step never asks for the 40-iteration solve)
 template <class PassFn>
-__device__ void solve_single(const DevShip& S, const PassFn& pass, int n, const Blk& B, bool lead) {
-  const double V = B.vol[n], Vmax = S.vmax[n];
+__device__ void solve_single(PC& P, const PassFn& pass, int n, const Blk& B, bool lead_) {
+  const double V = B.vol[n], Vmax = P.S.vmax[n];
   double h = B.level[n];
   Pass p = pass(n, h);
   const double lo = p.zmin, hi = p.zmax;
-  if (lead) { B.zmin[n] = lo; B.zmax[n] = hi; }
+  if (lead_) { B.zmin[n] = lo; B.zmax[n] = hi; }
   if (V <= kEmptyVolume) {
-    if (lead) { B.level[n] = lo; B.area[n] = 0; }
+    if (lead_) { B.level[n] = lo; B.area[n] = 0; }
     return;
   }
   if (V >= Vmax * kFullFraction) {
     p = pass(n, hi);
-    if (lead) {
-      B.level[n] = hi + jmax(0.0, V - Vmax) / S.aov[n];
-      B.area[n] = S.aov[n];
+    if (lead_) {
+      B.level[n] = hi + jmax(0.0, V - Vmax) / P.S.aov[n];
+      B.area[n] = P.S.aov[n];
       put_centroid(B, n, p);
     }
     return;
@@ -409,17 +475,17 @@ __device__ void solve_single(const DevShip& S, const PassFn& pass, int n, const 
     h = hn;
     p = pass(n, h);
   }
-  if (lead) {
+  if (lead_) {
     B.level[n] = h; B.area[n] = p.dv;
     put_centroid(B, n, p);
   }
 }
 
-// levels.hpp:54-162 with full = false. P is this unit's pass scratch in LDS (maxGroup entries).
+// levels.hpp:54-162 with full = false. Pp is this unit's pass scratch in LDS (maxGroup entries).
 template <class PassFn>
-__device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t* members, int N, const Blk& B, double* P, bool lead) {
+__device__ void solve_group(PC& P, const PassFn& pass, gci* members, int N, const Blk& B, SSL double* Pp, bool lead_) {
   double V = 0, Vmax = 0;
-  for (int k = 0; k < N; ++k) { V += B.vol[members[k]]; Vmax += S.vmax[members[k]]; }
+  for (int k = 0; k < N; ++k) { V += B.vol[members[k]]; Vmax += P.S.vmax[members[k]]; }
   bool allWet = true;
   int firstWet = -1;
   for (int k = 0; k < N; ++k) {
@@ -437,33 +503,33 @@ __device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t*
   double lo = 0, hi = 0;
   for (int k = 0; k < N; ++k) {
     const Pass p = pass(members[k], h);
-    if (lead) { put_pass(P, k, p); B.zmin[members[k]] = p.zmin; B.zmax[members[k]] = p.zmax; }
+    if (lead_) { put_pass(Pp, k, p); B.zmin[members[k]] = p.zmin; B.zmax[members[k]] = p.zmax; }
     if (k == 0) { lo = p.zmin; hi = p.zmax; }
     else { if (p.zmin < lo) lo = p.zmin; if (p.zmax > hi) hi = p.zmax; }
   }
   if (V <= kEmptyVolume) {
-    if (lead) {
-      for (int k = 0; k < N; ++k) { const int n = members[k]; B.vol[n] = 0; B.level[n] = get_pass(P, k).zmin; B.area[n] = 0; }
+    if (lead_) {
+      for (int k = 0; k < N; ++k) { const int n = members[k]; B.vol[n] = 0; B.level[n] = Pp[7 * k + 5]; B.area[n] = 0; }
     }
     return;
   }
   if (V >= Vmax * kFullFraction) {
     const double over = jmax(0.0, V - Vmax);
     double aSum = 0;
-    for (int k = 0; k < N; ++k) aSum += S.aov[members[k]];
+    for (int k = 0; k < N; ++k) aSum += P.S.aov[members[k]];
     const double dh = over / aSum;
     double assigned = 0;
     for (int k = 0; k < N - 1; ++k) {
-      const double v = S.vmax[members[k]] + dh * S.aov[members[k]];
-      if (lead) B.vol[members[k]] = v;
+      const double v = P.S.vmax[members[k]] + dh * P.S.aov[members[k]];
+      if (lead_) B.vol[members[k]] = v;
       assigned += v;
     }
-    if (lead) B.vol[members[N - 1]] = V - assigned;
+    if (lead_) B.vol[members[N - 1]] = V - assigned;
     for (int k = 0; k < N; ++k) {
       const int n = members[k];
       const Pass p = pass(n, hi);
-      if (lead) {
-        B.level[n] = hi + dh; B.area[n] = S.aov[n];
+      if (lead_) {
+        B.level[n] = hi + dh; B.area[n] = P.S.aov[n];
         put_centroid(B, n, p);
       }
     }
@@ -473,39 +539,39 @@ __device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t*
     h = jmin(hi, jmax(lo, h));
     for (int k = 0; k < N; ++k) {
       const Pass p = pass(members[k], h);
-      if (lead) put_pass(P, k, p);
+      if (lead_) put_pass(Pp, k, p);
     }
   }
   const double tol = jmax(1e-3 * Vmax, 0.5);
   double a = lo, b = hi;
   for (int it = 0; it < kNewtonFast; ++it) {
     double sum = 0;
-    for (int k = 0; k < N; ++k) sum += P[7 * k + 0];
+    for (int k = 0; k < N; ++k) sum += Pp[7 * k + 0];
     const double err = sum - V;
     if (__builtin_fabs(err) < tol) break;
     if (err > 0) b = h; else a = h;
     double dv = 0;
-    for (int k = 0; k < N; ++k) dv += P[7 * k + 1];
+    for (int k = 0; k < N; ++k) dv += Pp[7 * k + 1];
     double hn = dv > kSlopeMin ? h - err / dv : 0.5 * (a + b);
     if (!(hn > a && hn < b)) hn = 0.5 * (a + b);
     h = hn;
     for (int k = 0; k < N; ++k) {
       const Pass p = pass(members[k], h);
-      if (lead) put_pass(P, k, p);
+      if (lead_) put_pass(Pp, k, p);
     }
   }
   // share the water in proportion to the geometry at this level (levels.hpp:125-161)
-  if (!lead) return;
+  if (!lead_) return;
   double tot = 0;
-  for (int k = 0; k < N; ++k) tot += P[7 * k + 0];
+  for (int k = 0; k < N; ++k) tot += Pp[7 * k + 0];
   double assigned = 0;
   for (int k = 0; k < N - 1; ++k) {
-    double v = tot > 0 ? V * P[7 * k + 0] / tot : V / static_cast<double>(N);
-    if (v > S.vmax[members[k]]) v = S.vmax[members[k]];
+    double v = tot > 0 ? V * Pp[7 * k + 0] / tot : V / static_cast<double>(N);
+    if (v > P.S.vmax[members[k]]) v = P.S.vmax[members[k]];
     B.vol[members[k]] = v;
     assigned += v;
   }
-  const double vmaxLast = S.vmax[members[N - 1]];
+  const double vmaxLast = P.S.vmax[members[N - 1]];
   if (V - assigned > vmaxLast) {
     const double need = V - vmaxLast;
     double capped = 0;
@@ -513,9 +579,9 @@ __device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t*
       double prior = 0;
       for (int j = 0; j < k; ++j) prior += B.vol[members[j]];
       const double v = (need - capped) - prior;
-      if (v > S.vmax[members[k]] && k > 0) {
-        B.vol[members[k]] = S.vmax[members[k]];
-        capped += S.vmax[members[k]];
+      if (v > P.S.vmax[members[k]] && k > 0) {
+        B.vol[members[k]] = P.S.vmax[members[k]];
+        capped += P.S.vmax[members[k]];
         continue;
       }
       B.vol[members[k]] = v;
@@ -527,7 +593,7 @@ __device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t*
   B.vol[members[N - 1]] = V - assigned;
   for (int k = 0; k < N; ++k) {
     const int n = members[k];
-    const Pass p = get_pass(P, k);
+    const Pass p = get_pass(Pp, k);
     B.level[n] = h; B.area[n] = p.dv;
     put_centroid(B, n, p);
   }
@@ -535,27 +601,26 @@ __device__ void solve_group(const DevShip& S, const PassFn& pass, const int32_t*
 
 // levels.hpp:166-188: the merge decision, then the solve, for group g
 template <class PassFn>
-__device__ void solve_group_entry(const DevShip& S, const DevNet& M, const Frame& F, const PassFn& pass, int g, const Blk& B, double* P,
-                                  bool lead) {
-  const int N = M.gCount[g];
-  const int32_t* members = M.gNodes + M.gStart[g];
-  const uint8_t mode = M.gMode[g];
+__device__ void solve_group_entry(PC& P, const PassFn& pass, int g, const Blk& B, SSL double* Pp, bool lead_) {
+  const int N = P.M.gCount[g];
+  gci* members = P.M.gNodes + P.M.gStart[g];
+  const uint8_t mode = P.M.gMode[g];
   bool merged = false;
   if (N >= 2) {
     if (mode == 1) {
       merged = true;
     } else if (mode == 2) {
-      const double zMw = world_z(F, M.gpx[g], M.gpy[g], M.gpz[g]);
+      const double zMw = world_z(B.frame, P.M.gpx[g], P.M.gpy[g], P.M.gpz[g]);
       merged = true;
       for (int k = 0; k < N; ++k) {
         const int n = members[k];
-        if (!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + M.gMargin)) { merged = false; break; }
+        if (!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + P.M.gMargin)) { merged = false; break; }
       }
     }
   }
-  if (lead) B.merged[g] = merged ? 1 : 0;
-  if (merged) solve_group(S, pass, members, N, B, P, lead);
-  else for (int k = 0; k < N; ++k) solve_single(S, pass, members[k], B, lead);
+  if (lead_) B.merged[g] = merged ? 1 : 0;
+  if (merged) solve_group(P, pass, members, N, B, Pp, lead_);
+  else for (int k = 0; k < N; ++k) solve_single(P, pass, members[k], B, lead_);
 }
 
 // ------------------------------------------------------------------------------------------- (c) network
@@ -564,17 +629,17 @@ __device__ __forceinline__ double head_of(const Blk& B, int n) { return B.vol[n]
 
 // flows.hpp:211-231: effective head and area of node n, and the number of connections that count toward the
 // limiter there (an enabled connection not between two empty spaces). The heads of the other ends are
-// recomputed here from vol and level, so that one pass over the nodes suffices.
-__device__ __forceinline__ void node_heads(const DevShip& S, const DevNet& M, const Blk& B, int n) {
+// recomputed from vol and level, so that one pass over the nodes suffices.
+__device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
   const double v = B.vol[n];
   B.hEff[n] = v > kEmptyVolume ? B.level[n] : kNoHead;
-  B.aEff[n] = v >= S.vmax[n] ? S.aov[n] : jmax(B.area[n], S.amin[n]);
-  const int s = M.incStart[n], e = s + M.incCount[n];
+  B.aEff[n] = v >= P.S.vmax[n] ? P.S.aov[n] : jmax(B.area[n], P.S.amin[n]);
+  const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   int d = 0;
   for (int k = s; k < e; ++k) {
-    const int c = M.incConn[k];
+    const int c = P.M.incConn[k];
     if (!B.cen[c]) continue;
-    const int a = M.ca[c], b = M.cb[c];
+    const int a = P.M.ca[c], b = P.M.cb[c];
     if (a >= 0 && head_of(B, a) < kNoHeadTest && head_of(B, b) < kNoHeadTest) continue;
     ++d;
   }
@@ -582,17 +647,18 @@ __device__ __forceinline__ void node_heads(const DevShip& S, const DevNet& M, co
 }
 
 // flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter
-__device__ inline double connection_flow(const DevNet& M, const Frame& F, double t, const Blk& B, int c) {
+__device__ inline double connection_flow(PC& P, const SSL double* fr, double t, const Blk& B, int c) {
   if (!B.cen[c] || B.ctOn[c] > t) { B.q[c] = 0; return 0; }
-  if (M.skipGroup[c] >= 0 && B.merged[M.skipGroup[c]]) { B.q[c] = 0; return 0; }
-  const int a = M.ca[c], b = M.cb[c];
-  const double zo = F.R20 * M.cx[c] + F.R21 * M.cy[c] + F.R22 * M.cz[c] + F.tz;
+  if (P.M.skipGroup[c] >= 0 && B.merged[P.M.skipGroup[c]]) { B.q[c] = 0; return 0; }
+  const int a = P.M.ca[c], b = P.M.cb[c];
+  const double zo = fr[F_R20] * P.M.cx[c] + fr[F_R21] * P.M.cy[c] + fr[F_R22] * P.M.cz[c] + fr[F_TZ];
   const double ha = a < 0 ? 0.0 : B.hEff[a], hbv = B.hEff[b];
   const double Ha = ha > zo ? ha - zo : 0.0, Hb = hbv > zo ? hbv - zo : 0.0;
+  const double sq2g = P.M.sq2g;
   double q = 0;
-  if (M.law[c] == 0) {
+  if (P.M.law[c] == 0) {
     const double d = Ha - Hb;
-    if (d != 0) q = M.coef[c] * B.carea[c] * M.sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
+    if (d != 0) q = P.M.coef[c] * B.carea[c] * sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
   } else {
     const double H1 = Ha > Hb ? Ha : Hb;
     if (H1 > 0) {
@@ -600,13 +666,14 @@ __device__ inline double connection_flow(const DevNet& M, const Frame& F, double
       double f = 1;
       if (H2 > 0) {
         const double r = H2 / H1;
-        f = fdl::pow(jmax(0.0, 1 - fdl::pow(r, kWeirExponent)), kVillemonteExponent);
+        f = dpow(jmax(0.0, 1 - dpow(r, kWeirExponent)), kVillemonteExponent);
       }
-      q = M.coef[c] * B.carea[c] * kWeirCoefficient * M.sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
+      q = P.M.coef[c] * B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
     }
   }
   if (q == 0) { B.q[c] = 0; return 0; }
-  double dV = q * M.dt;
+  const double dt = P.M.dt;
+  double dV = q * dt;
   const int src = dV > 0 ? a : b, dst = dV > 0 ? b : a;
   const double hs = src < 0 ? 0.0 : B.hEff[src];
   const double hd = dst < 0 ? 0.0 : B.hEff[dst];
@@ -616,38 +683,39 @@ __device__ inline double connection_flow(const DevNet& M, const Frame& F, double
   double lim = inv > 0 ? share * (hs - jmax(hd, zo)) / inv : __builtin_fabs(dV);
   if (lim < 0) lim = 0;
   if (__builtin_fabs(dV) > lim) dV = dV > 0 ? lim : -lim;
-  B.q[c] = dV / M.dt;
+  B.q[c] = dV / dt;
   return dV;
 }
 
 // flows.hpp:277-307 for node n (the gather in increasing connection order and the capped volume update), then
-// the node's terms of the loads (body.hpp:21-34), which only depend on this node once its volume is final.
-__device__ inline void node_update(const DevShip& S, const DevNet& M, const Frame& F, double zO, const Blk& B, int n) {
-  const int s = M.incStart[n], e = s + M.incCount[n];
+// the node's terms of the loads (body.hpp:21-34), which depend on this node only once its volume is final.
+__device__ inline void node_update(PC& P, const Blk& B, int n) {
+  const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   double acc = 0;
   for (int k = s; k < e; ++k) {
-    const double d = B.dV[M.incConn[k]];
-    acc = M.incSide[k] ? acc + d : acc - d;
+    const double d = B.dV[P.M.incConn[k]];
+    acc = P.M.incSide[k] ? acc + d : acc - d;
   }
   double v = B.vol[n] + acc;
   if (v < 0) v = 0;
-  const double vm = S.vmax[n];
+  const double vm = P.S.vmax[n];
   double excess = 0;
   if (v > vm) {
     const double head = jmax(0.0, -B.zmax[n]);
-    const double cap = vm + S.aov[n] * head;
+    const double cap = vm + P.S.aov[n] * head;
     if (v > cap) { excess = v - cap; v = cap; }
   }
   B.excess[n] = excess;
   B.vol[n] = v;
   if (v > 0) {
-    const double W = M.rho * M.g;
+    const SSL double* fr = B.frame;
+    const double W = P.M.rho * P.M.g;
     const double x = B.cx[n], y = B.cy[n], z = B.cz[n];
-    const double wx = F.R00 * x + F.R01 * y + F.R02 * z + F.tx - M.XO;
-    const double wy = F.R10 * x + F.R11 * y + F.R12 * z + F.ty - M.YO;
-    const double wz = F.R20 * x + F.R21 * y + F.R22 * z + F.tz - zO;
+    const double wx = fr[F_R00] * x + fr[F_R01] * y + fr[F_R02] * z + fr[F_TX] - P.M.XO;
+    const double wy = fr[F_R10] * x + fr[F_R11] * y + fr[F_R12] * z + fr[F_TY] - P.M.YO;
+    const double wz = fr[F_R20] * x + fr[F_R21] * y + fr[F_R22] * z + fr[F_TZ] - B.scal[S_ZO];
     const double f = -W * v;
-    const double m = M.rho * v;
+    const double m = P.M.rho * v;
     B.tX[n] = wy * f;
     B.tY[n] = wx * f;
     B.tF[n] = f;
@@ -660,23 +728,25 @@ __device__ inline void node_update(const DevShip& S, const DevNet& M, const Fram
 // ------------------------------------------------------------------------------------------- readouts
 
 // readouts.hpp:175-191 (7 values: t trim list water(t) inflow(t/min) draftF draftA), one thread
-__device__ inline void readouts7(const DevNet& M, const Blk& B, int nn, double* r) {
+__device__ __noinline__ void readouts7(PC& P, const Blk& B, SSG double* r) {
   const double t = B.scal[S_T], zO = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
-  const Frame F = make_frame(M, zO, fdl::cos(pitch), fdl::sin(pitch), fdl::cos(roll), fdl::sin(roll));
+  const double cth = trig(pitch, 1), sth = trig(pitch, 0), cph = trig(roll, 1), sph = trig(roll, 0);
+  const double R20 = -sth, R21 = cth * sph, R22 = cth * cph;
+  const double tz = zO - (R20 * P.M.g0x + R21 * P.M.g0y + R22 * P.M.g0z);
   double water = 0;
-  for (int n = 0; n < nn; ++n) water += B.vol[n];
+  for (int n = 0; n < P.S.nn; ++n) water += B.vol[n];
   r[0] = t;
   r[1] = pitch * 180 / kPi;
   r[2] = roll * 180 / kPi;
-  r[3] = water * M.rho / 1000;
-  r[4] = B.scal[O_INFLOW] * M.rho / 1000 * 60;
-  r[5] = -world_z(F, M.xFP, 0, 0);
-  r[6] = -world_z(F, M.xAP, 0, 0);
+  r[3] = water * P.M.rho / 1000;
+  r[4] = B.scal[O_INFLOW] * P.M.rho / 1000 * 60;
+  r[5] = -(R20 * P.M.xFP + R21 * 0.0 + R22 * 0.0 + tz);
+  r[6] = -(R20 * P.M.xAP + R21 * 0.0 + R22 * 0.0 + tz);
 }
 
-__device__ inline void push_event(const DevNet& M, const Blk& B, double t, int id, int kind) {
+__device__ inline void push_event(PC& P, const Blk& B, double t, int id, int kind) {
   const int k = static_cast<int>(B.scal[S_EVCOUNT]);
-  if (k < M.evCap) {
+  if (k < P.M.evCap) {
     B.evT[k] = t; B.evId[k] = id; B.evKind[k] = static_cast<uint8_t>(kind);
     B.scal[S_EVCOUNT] = k + 1;
   }
@@ -684,31 +754,23 @@ __device__ inline void push_event(const DevNet& M, const Blk& B, double t, int i
 
 // ------------------------------------------------------------------------------------------- the step
 
-// Pose and buoyancy partials: after this every thread holds the frame, and the lead lane of each 32-lane group
-// has written its group's tree result to warpHyd. Ends without a barrier after the partials.
+// Buoyancy partials of this thread's 32-lane group; the lead lane stores the group's tree result.
 template <bool Mixed>
-__device__ inline void pose_and_buoyancy(const DevShip& S, const DevNet& M, const Blk& B, int tid, Frame& F, FrameF& f) {
-  if (tid < 4) B.trig[tid] = pose_trig(tid, B.scal[S_PITCH], B.scal[S_ROLL]);
-  __syncthreads();
-  F = make_frame(M, B.scal[S_ZO], B.trig[0], B.trig[1], B.trig[2], B.trig[3]);
-  f = frame_f(F);
+__device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
+  SSL double* w = B.warpHyd + 5 * unit;
   if constexpr (Mixed) {
+    const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
-    for (int i = tid; i < S.ncol; i += kThreads) column(S, f, i, p);
+    for (int i = tid; i < P.S.ncol; i += kThreads) column(P, f, i, p);
     tree32(p, lane);
-    if (lane == 0) {
-      double* w = B.warpHyd + 5 * unit;
-      w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top;
-    }
+    if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   } else {
+    const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
-    for (int i = tid; i < S.ncol; i += kThreads) column(S, F, i, p);
+    for (int i = tid; i < P.S.ncol; i += kThreads) column(P, F, i, p);
     tree32(p, lane);
-    if (lane == 0) {
-      double* w = B.warpHyd + 5 * unit;
-      w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top;
-    }
+    if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   }
 }
 
@@ -716,7 +778,7 @@ __device__ inline void pose_and_buoyancy(const DevShip& S, const DevNet& M, cons
 __device__ inline void finish_buoyancy(const Blk& B, double& V, double& bx, double& by, double& bz, double& top) {
   double tV = B.warpHyd[0], tMx = B.warpHyd[1], tMy = B.warpHyd[2], tMz = B.warpHyd[3], tTop = B.warpHyd[4];
   for (int w = 1; w < kUnits; ++w) {
-    const double* q = B.warpHyd + 5 * w;
+    const SSL double* q = B.warpHyd + 5 * w;
     tV += q[0]; tMx += q[1]; tMy += q[2]; tMz += q[3];
     if (q[4] > tTop) tTop = q[4];
   }
@@ -727,117 +789,135 @@ __device__ inline void finish_buoyancy(const Blk& B, double& V, double& bx, doub
   top = tTop;
 }
 
-template <bool Mixed>
-__device__ void device_step(const DevShip& S, const DevNet& M, const Blk& B, int tid) {
-  const int lane = tid & (kLanes - 1), unit = tid / kLanes;
-  const int NN = S.nn, NC = M.nc, NG = M.ng;
-  Frame F;
-  FrameF f;
-  pose_and_buoyancy<Mixed>(S, M, B, tid, F, f);
+__device__ __forceinline__ double bcast(double v, int from) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), from));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), from));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
 
-  // (b) free surfaces: logical group `unit` solves groups unit, unit + 8, ...
+// The tail of a step on wave 0 (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp): lanes 0..6 run the
+// seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess), lane 0
+// integrates and tracks events, then lanes 0..3 evaluate the pose of the next step.
+__device__ void step_tail(PC& P, const Blk& B, int lane) {
+  const SSL double* fr = B.frame;
+  double hV, hx, hy, hz, htop;
+  finish_buoyancy(B, hV, hx, hy, hz, htop);
+  const double W = P.M.rho * P.M.g;
+  const double Fb = W * hV;
+  const double bxw = fr[F_R00] * hx + fr[F_R01] * hy + fr[F_R02] * hz + fr[F_TX];
+  const double byw = fr[F_R10] * hx + fr[F_R11] * hy + fr[F_R12] * hz + fr[F_TY];
+  const int NN = P.S.nn;
+  double acc = 0;
+  if (lane < 7) {
+    const SSL double* src = lane == 0 ? B.tX : lane == 1 ? B.tY : lane == 2 ? B.tF : lane == 3 ? B.tM : lane == 4 ? B.tIp : lane == 5 ? B.tIr : B.excess;
+    const SSL double* gate = lane < 6 ? B.vol : B.excess;
+    const bool subtract = lane == 1 || lane == 6;
+    acc = lane == 0 ? (byw - P.M.YO) * Fb : lane == 1 ? -(bxw - P.M.XO) * Fb : lane == 2 ? Fb - P.M.g * P.M.ms : lane == 6 ? B.scal[S_SEA] : 0.0;
+    for (int n = 0; n < NN; ++n) {
+      const double gv = gate[n];
+      if (lane < 6 ? !(gv <= 0) : gv > 0) {
+        const double v = src[n];
+        acc = subtract ? acc - v : acc + v;
+      }
+    }
+  }
+  const double tauX = bcast(acc, 0), tauY = bcast(acc, 1), Fz = bcast(acc, 2), mw = bcast(acc, 3);
+  const double Iw_p = bcast(acc, 4), Iw_r = bcast(acc, 5), seaIn = bcast(acc, 6);
+  if (lane == 0) {
+    B.scal[O_INFLOW] = seaIn / P.M.dt;
+    const double Mth = tauY, Mph = tauX * fr[F_CTH];
+    // integrate (body.hpp:42-57)
+    double vz = B.scal[S_VZ], wth = B.scal[S_WTH], wph = B.scal[S_WPH];
+    double zO = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
+    const double dt = P.M.dt;
+    const double mh = P.M.mh0 + mw;
+    const double Ith = P.M.Ith0 + Iw_p, Iph = P.M.Iph0 + Iw_r;
+    const double az = (Fz - P.M.cDz * vz - P.M.qz * __builtin_fabs(vz) * vz) / mh;
+    const double ath = (Mth - P.M.cDth * wth - P.M.qth * __builtin_fabs(wth) * wth) / Ith;
+    const double aph = (Mph - P.M.cDph * wph - P.M.qph * __builtin_fabs(wph) * wph) / Iph;
+    vz += az * dt; wth += ath * dt; wph += aph * dt;
+    zO += vz * dt; pitch += wth * dt; roll += wph * dt;
+    if (pitch > kPitchClamp) { pitch = kPitchClamp; wth = 0; }
+    if (pitch < -kPitchClamp) { pitch = -kPitchClamp; wth = 0; }
+    if (roll > kRollClamp) { roll = kRollClamp; wph = 0; }
+    if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; }
+    const double tn = B.scal[S_T] + dt;
+    B.scal[S_T] = tn; B.scal[S_ZO] = zO; B.scal[S_PITCH] = pitch; B.scal[S_ROLL] = roll;
+    B.scal[S_VZ] = vz; B.scal[S_WTH] = wth; B.scal[S_WPH] = wph;
+    B.scal[O_VB] = hV; B.scal[O_TOP] = htop; B.scal[O_MW] = mw; B.scal[O_BX] = hx; B.scal[O_BY] = hy; B.scal[O_BZ] = hz;
+    // events (events.hpp:11-38), with the pose of the step and the time after it
+    for (int m = 0; m < P.M.nmon; ++m) {
+      if (B.over[m] > P.M.monThreshold && B.overT[m] < 0) {
+        B.overT[m] = tn;
+        push_event(P, B, tn, m, 0);
+      }
+    }
+    for (int i = 0; i < P.M.nmark; ++i) {
+      if (B.markT[i] >= 0) continue;
+      const double wz = world_z(fr, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]);
+      const bool under = P.M.mkWhen[i] == 0;
+      if ((under && wz < 0) || (!under && wz > 0)) {
+        B.markT[i] = tn;
+        push_event(P, B, tn, i, 1);
+      }
+    }
+    if (B.scal[S_FOUNDERED] == 0 &&
+        (htop < P.M.hullTopBelow || __builtin_fabs(pitch) > P.M.pitchAbsAbove || (hV < P.M.buoyancyBelow && tn > P.M.minT))) {
+      B.scal[S_FOUNDERED] = 1;
+      B.scal[S_FOUNDERT] = tn;
+      push_event(P, B, tn, -1, 2);
+    }
+  }
+  // the pose of the next step (frame.hpp:9-21); the frame slots are free: every reader of this step is done
+  pose_wave0(P, B, lane);
+}
+
+// One step. On entry the frame slots hold the pose of the current state (written by the previous step's tail or
+// at load time); every thread may read them. Ends with a barrier.
+template <bool Mixed>
+__device__ void device_step(PC& P, const Blk& B, int tid) {
+  const int lane = tid & (kLanes - 1), unit = tid / kLanes;
+  const int NN = P.S.nn, NC = P.M.nc, NG = P.M.ng;
+
+  // (a) buoyancy partials, then (b) the free surfaces: logical group `unit` solves groups unit, unit + 8, ...
+  buoyancy_partials<Mixed>(P, B, tid);
   {
-    const UnitPass<Mixed> pass{S, F, f, lane};
-    double* P = B.passes + 7 * M.maxGroup * unit;
-    for (int g = unit; g < NG; g += kUnits) solve_group_entry(S, M, F, pass, g, B, P, lane == 0);
+    const UnitPass<Mixed> pass{P, B.frame, lane, (tid & 32) != 0};
+    SSL double* Pp = B.passes + 7 * P.M.maxGroup * unit;
+    for (int g = unit; g < NG; g += kUnits) solve_group_entry(P, pass, g, B, Pp, lane == 0);
   }
   __syncthreads();
 
   // (c) heads and degrees per node
-  for (int n = tid; n < NN; n += kThreads) node_heads(S, M, B, n);
+  for (int n = tid; n < NN; n += kThreads) node_heads(P, B, n);
   __syncthreads();
 
   // (c) flows per connection
   const double t = B.scal[S_T];
-  for (int c = tid; c < NC; c += kThreads) B.dV[c] = connection_flow(M, F, t, B, c);
+  for (int c = tid; c < NC; c += kThreads) B.dV[c] = connection_flow(P, B.frame, t, B, c);
   __syncthreads();
 
-  // (c) gathers, volume updates and load terms per node; monitor sums (events.hpp:73-75, per monitor in
-  // increasing connection order); the sea inflow (flows.hpp:288-292) on the last thread
-  const double zO = B.scal[S_ZO];
-  for (int n = tid; n < NN; n += kThreads) node_update(S, M, F, zO, B, n);
-  for (int m = tid; m < M.nmon; m += kThreads) {
+  // (c) gathers, volume updates and load terms per node; monitor sums (events.hpp:73-75, per monitor in increasing
+  // connection order); the sea inflow (flows.hpp:288-292) on the last thread
+  for (int n = tid; n < NN; n += kThreads) node_update(P, B, n);
+  for (int m = tid; m < P.M.nmon; m += kThreads) {
     double o = 0;
-    const int s = M.monStart[m], e = s + M.monCount[m];
+    const int s = P.M.monStart[m], e = s + P.M.monCount[m];
     for (int k = s; k < e; ++k) {
-      const double qc = B.q[M.monConn[k]];
+      const double qc = B.q[P.M.monConn[k]];
       if (qc > 0) o += qc;
     }
     B.over[m] = o;
   }
   if (tid == kThreads - 1) {
     double s = 0;
-    for (int k = 0; k < M.seaCount; ++k) s += B.dV[M.sea[k]];
+    for (int k = 0; k < P.M.seaCount; ++k) s += B.dV[P.M.sea[k]];
     B.scal[S_SEA] = s;
   }
   __syncthreads();
 
-  // the serial tail (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp) on one lane
-  if (tid == 0) {
-    double seaIn = B.scal[S_SEA];
-    for (int n = 0; n < NN; ++n)
-      if (B.excess[n] > 0) seaIn -= B.excess[n];
-    B.scal[O_INFLOW] = seaIn / M.dt;
-    double hV, hx, hy, hz, htop;
-    finish_buoyancy(B, hV, hx, hy, hz, htop);
-    // loads (body.hpp:11-40)
-    const double W = M.rho * M.g;
-    const double Fb = W * hV;
-    const double bxw = F.R00 * hx + F.R01 * hy + F.R02 * hz + F.tx;
-    const double byw = F.R10 * hx + F.R11 * hy + F.R12 * hz + F.ty;
-    double tauX = (byw - M.YO) * Fb, tauY = -(bxw - M.XO) * Fb;
-    double Fz = Fb - M.g * M.ms;
-    double mw = 0, Iw_p = 0, Iw_r = 0;
-    for (int n = 0; n < NN; ++n) {
-      if (B.vol[n] <= 0) continue;
-      tauX += B.tX[n]; tauY -= B.tY[n]; Fz += B.tF[n];
-      mw += B.tM[n];
-      Iw_p += B.tIp[n];
-      Iw_r += B.tIr[n];
-    }
-    const double Mth = tauY, Mph = tauX * F.cth;
-    // integrate (body.hpp:42-57)
-    double vz = B.scal[S_VZ], wth = B.scal[S_WTH], wph = B.scal[S_WPH];
-    double zOn = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
-    const double dt = M.dt;
-    const double mh = M.mh0 + mw;
-    const double Ith = M.Ith0 + Iw_p, Iph = M.Iph0 + Iw_r;
-    const double az = (Fz - M.cDz * vz - M.qz * __builtin_fabs(vz) * vz) / mh;
-    const double ath = (Mth - M.cDth * wth - M.qth * __builtin_fabs(wth) * wth) / Ith;
-    const double aph = (Mph - M.cDph * wph - M.qph * __builtin_fabs(wph) * wph) / Iph;
-    vz += az * dt; wth += ath * dt; wph += aph * dt;
-    zOn += vz * dt; pitch += wth * dt; roll += wph * dt;
-    if (pitch > kPitchClamp) { pitch = kPitchClamp; wth = 0; }
-    if (pitch < -kPitchClamp) { pitch = -kPitchClamp; wth = 0; }
-    if (roll > kRollClamp) { roll = kRollClamp; wph = 0; }
-    if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; }
-    const double tn = t + dt;
-    B.scal[S_T] = tn; B.scal[S_ZO] = zOn; B.scal[S_PITCH] = pitch; B.scal[S_ROLL] = roll;
-    B.scal[S_VZ] = vz; B.scal[S_WTH] = wth; B.scal[S_WPH] = wph;
-    B.scal[O_VB] = hV; B.scal[O_TOP] = htop; B.scal[O_MW] = mw; B.scal[O_BX] = hx; B.scal[O_BY] = hy; B.scal[O_BZ] = hz;
-    // events (events.hpp:11-38), with the pose of the step and the time after it
-    for (int m = 0; m < M.nmon; ++m) {
-      if (B.over[m] > M.monThreshold && B.overT[m] < 0) {
-        B.overT[m] = tn;
-        push_event(M, B, tn, m, 0);
-      }
-    }
-    for (int i = 0; i < M.nmark; ++i) {
-      if (B.markT[i] >= 0) continue;
-      const double wz = world_z(F, M.mkx[i], M.mky[i], M.mkz[i]);
-      const bool under = M.mkWhen[i] == 0;
-      if ((under && wz < 0) || (!under && wz > 0)) {
-        B.markT[i] = tn;
-        push_event(M, B, tn, i, 1);
-      }
-    }
-    if (B.scal[S_FOUNDERED] == 0 &&
-        (htop < M.hullTopBelow || __builtin_fabs(pitch) > M.pitchAbsAbove || (hV < M.buoyancyBelow && tn > M.minT))) {
-      B.scal[S_FOUNDERED] = 1;
-      B.scal[S_FOUNDERT] = tn;
-      push_event(M, B, tn, -1, 2);
-    }
-  }
+  if (tid < 64) step_tail(P, B, tid);
   __syncthreads();
 }
 

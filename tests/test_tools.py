@@ -1,0 +1,105 @@
+"""The C++ command-line drop-ins on the host (no GPU): usage errors and exit codes, numerics refusals, and the exact
+results comparison, cross-checked against the reference's own sinksim_sweep --compare where that build exists."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle
+from conftest import ROOT
+from sinksim_amd import sweep as S
+from sinksim_amd.data_paths import SHIP, sim_path
+
+BIN = os.path.join(ROOT, "sinksim_amd", "bin")
+REF_SWEEP = os.path.join(pyoracle.REF_DIR, "sinksim_sweep")
+
+
+def _tool(name):
+    p = os.path.join(BIN, name)
+    if not os.path.exists(p):
+        pytest.fail(f"{p} is not built (run build())")
+    return p
+
+
+def _run(*cmd):
+    return subprocess.run(list(cmd), capture_output=True, text=True)
+
+
+def test_usage_errors_exit_2():
+    assert _run(_tool("sinksim_hip_run")).returncode == 2
+    assert _run(_tool("sinksim_sweep")).returncode == 2
+    r = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), "x.json", "--numerics", "oracle")
+    assert r.returncode == 2 and "CPU only" in r.stderr
+    r = _run(_tool("sinksim_hip_run"), SHIP, sim_path("titanic"), "--precision", "quad")
+    assert r.returncode == 2 and "unknown --precision" in r.stderr
+    r = _run(_tool("sinksim_hip_run"), SHIP, sim_path("titanic"), "--count")
+    assert r.returncode == 2 and "needs a value" in r.stderr
+
+
+def _results(titanic, count=3):
+    """Oracle results of a small sweep, as a Results document."""
+    ship, sim = titanic
+    from sinksim_amd.workloads import titanic_batch_sweep
+    sw = titanic_batch_sweep(sim, count=count, t_max=600.0)
+    en, area, tOn = S.connection_tables(sw, sim)
+    out = pyoracle.sweep(ship, sim, "portable32", en, area, tOn, sw.tMax, sw.every, threads=4)
+    r = S.Results(ship.id, sim.id, "portable32", "oracle", sw.tMax, sw.every)
+    for inst, o in zip(sw.instances, out):
+        ev = [(t, sim.event_id(k, i), sim.event_label(k, i)) for t, i, k in o["events"]]
+        r.instances.append(S.InstanceResult(inst.id, o["foundered"], o["founderT"], o["tEnd"], o["steps"],
+                                            S.final_dict(o["curve"][-1]), ev, o["curve"]))
+    return r
+
+
+def test_compare_identical_and_different(titanic, tmp_path):
+    r = _results(titanic)
+    a, b = os.path.join(tmp_path, "a.json"), os.path.join(tmp_path, "b.json")
+    S.save_results(r, a)
+    S.save_results(r, b)
+    res = _run(_tool("sinksim_sweep"), "--compare", a, b)
+    assert res.returncode == 0 and "RESULT: IDENTICAL" in res.stdout
+    r.instances[1].curve = r.instances[1].curve.copy()
+    r.instances[1].curve[3, 1] = np.nextafter(r.instances[1].curve[3, 1], 1e9)
+    S.save_results(r, b)
+    res = _run(_tool("sinksim_sweep"), "--compare", a, b)
+    assert res.returncode == 1 and "RESULT: DIFFERENT" in res.stdout and "curve[3] trim" in res.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SWEEP), reason="reference CPU build absent (make -C oracle ref)")
+def test_reference_compare_reads_our_results(titanic, tmp_path):
+    """The reference's own --compare parses results files written here and agrees on identity and difference."""
+    r = _results(titanic, 2)
+    a, b = os.path.join(tmp_path, "a.json"), os.path.join(tmp_path, "b.json")
+    S.save_results(r, a)
+    S.save_results(r, b)
+    assert _run(REF_SWEEP, "--compare", a, b).returncode == 0
+    r.instances[0].founderT += 0.25
+    S.save_results(r, b)
+    res = _run(REF_SWEEP, "--compare", a, b)
+    assert res.returncode == 1 and "founderT" in res.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SWEEP), reason="reference CPU build absent (make -C oracle ref)")
+@pytest.mark.parametrize("numerics", ["portable32", "portable", "oracle"])
+def test_oracle_sweep_equals_reference_sweep(titanic, tmp_path, numerics):
+    """The oracle's sweep runner against the reference's CPU sweep runner on the reference's smoke sweep, value for
+    value (the sweep file, apply order, curves, events and final readouts)."""
+    from conftest import GOLDEN
+    ship, sim = titanic
+    sw_path = os.path.join(GOLDEN, "smoke.sweep.json")
+    ref_out = os.path.join(tmp_path, "ref.json")
+    res = _run(REF_SWEEP, SHIP, sim_path("titanic"), sw_path, "--numerics", numerics, "--threads", "8", "--results", ref_out)
+    assert res.returncode == 0, res.stderr
+    ref = S.load_results(ref_out)
+    sw = S.load_sweep(sw_path, sim)
+    en, area, tOn = S.connection_tables(sw, sim)
+    out = pyoracle.sweep(ship, sim, numerics, en, area, tOn, sw.tMax, sw.every, threads=8)
+    mine = S.Results(ship.id, sim.id, numerics, "oracle", sw.tMax, sw.every)
+    for inst, o in zip(sw.instances, out):
+        ev = [(t, sim.event_id(k, i), sim.event_label(k, i)) for t, i, k in o["events"]]
+        mine.instances.append(S.InstanceResult(inst.id, o["foundered"], o["founderT"], o["tEnd"], o["steps"],
+                                               S.final_dict(o["curve"][-1]), ev, o["curve"]))
+    d = S.compare(ref, mine)
+    assert d.differing == 0, d.first
+    assert d.compared > 1000

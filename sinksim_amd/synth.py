@@ -72,3 +72,279 @@ def box_barge(L=100.0, B=20.0, H=10.0, T=4.0, dx=5.0, dy=1.0, parts=2, shared_su
         0.05, [], [], [], [], f64([]), f64([]), f64([]), u8([]), f64([-1e9, 10, -1, 1e9]), body, f64([L / 2, -L / 2]),
         f64([0, -T, 0, 0, 0, 0, 0]), zeros.copy(), zeros.copy(), zeros.copy(), zeros.copy(), zeros.copy())
     return ship, sim
+
+
+# ------------------------------------------------------------------------------------------- uneven synthetic hulls
+
+KIND_NAMES = ["door", "over", "down", "xlow", "xmerge", "xup", "top", "breach", "open"]
+
+
+def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = None, rho: float = 1025.0, g: float = 9.81,
+                   dt: float = 0.25):
+    """A seeded ship and flooding scenario of the kind the uneven batch runs: length 100-300 m, parabolic sections,
+    `columns` (1,000-16,000) hull columns, `spaces` (8-128) nodes, a network of doors, bulkhead-top weirs, deck
+    openings, cross-flow ducts, deck-edge weirs to the sea and 1-6 breaches of 0.01-1 m^2 below the waterline.
+
+    Compartments are bounded by transverse bulkheads; each is split port/starboard and lower/upper at a flat deck, so
+    a compartment holds four nodes. Lower pairs share a surface always, above a threshold point, never, or are
+    solved singly; some compartments join all four nodes in one group (a shared surface over N = 4 members). The
+    body floats level at its design draft: mass = displaced mass there, centre of gravity above the centre of
+    buoyancy with a positive metacentric height, added mass and damping as the reference's box-barge rig."""
+    rng = np.random.default_rng(seed)
+    L = float(rng.uniform(100.0, 300.0))
+    B = L / float(rng.uniform(7.5, 10.0))
+    D = B * float(rng.uniform(0.55, 0.8))
+    T = D * float(rng.uniform(0.45, 0.62))
+    ncol_target = int(columns if columns is not None else rng.integers(1000, 16001))
+    nsp_target = int(spaces if spaces is not None else rng.integers(8, 129))
+    ncomp = max(2, nsp_target // 4)
+    # grid: the plan fills about 0.78 of the bounding box
+    cells = ncol_target / 0.78
+    ny = max(6, int(round(math.sqrt(cells * B / L))))
+    nx = max(2 * ncomp, int(round(cells / ny)))
+    dx, dy = L / nx, B / ny
+    zdeck = D * float(rng.uniform(0.4, 0.55))
+
+    def half_breadth(x):
+        return 0.5 * B * max(0.0, 1.0 - abs(2.0 * x / L) ** 2.4) ** 0.5
+
+    def keel_rise(x):
+        return 0.35 * D * abs(2.0 * x / L) ** 5
+
+    cols = []  # (x, y, dxdy, zlo, zhi, compartment, side)
+    bulkheads = np.linspace(-L / 2, L / 2, ncomp + 1)
+    for i in range(nx):
+        x = -L / 2 + (i + 0.5) * dx
+        hb = half_breadth(x)
+        comp = min(ncomp - 1, int(np.searchsorted(bulkheads, x, side="right") - 1))
+        for j in range(ny):
+            y = -B / 2 + (j + 0.5) * dy
+            if abs(y) >= hb or hb <= 0:
+                continue
+            rise = keel_rise(x)
+            zlo = rise + (D - rise) * (abs(y) / hb) ** 2
+            zhi = D + 0.02 * L * (2.0 * x / L) ** 2
+            if zhi - zlo < 0.05:
+                continue
+            cols.append((x, y, dx * dy, zlo, zhi, comp, 1 if y >= 0 else 0))
+    ncol = len(cols)
+    cx_ = np.array([c[0] for c in cols]); cy_ = np.array([c[1] for c in cols]); cA = np.array([c[2] for c in cols])
+    czlo = np.array([c[3] for c in cols]); czhi = np.array([c[4] for c in cols])
+    ccomp = np.array([c[5] for c in cols]); cside = np.array([c[6] for c in cols])
+
+    # nodes: compartment k -> (P lower, P upper, S lower, S upper); side 1 = port (y >= 0)
+    node_of = {}
+    labels, mu, vmax, amin, aov, segStart, segCount, segCol, segA, segE, ncent = [], [], [], [], [], [], [], [], [], [], []
+    for k in range(ncomp):
+        for side in (1, 0):
+            for upper in (False, True):
+                n = len(labels)
+                node_of[(k, side, upper)] = n
+                labels.append(f"C{k}/{'P' if side else 'S'}/{'upper' if upper else 'lower'}")
+                segStart.append(len(segCol))
+                perm = float(rng.uniform(0.85, 0.98))
+                vol = footprint = 0.0
+                mom = np.zeros(3)
+                for c in np.flatnonzero((ccomp == k) & (cside == side)):
+                    a, e = (max(czlo[c], zdeck), czhi[c]) if upper else (czlo[c], min(czhi[c], zdeck))
+                    if e - a <= 1e-3:
+                        continue
+                    segCol.append(int(c)); segA.append(float(a)); segE.append(float(e))
+                    v = cA[c] * (e - a)
+                    vol += v
+                    footprint += cA[c]
+                    mom += v * np.array([cx_[c], cy_[c], 0.5 * (a + e)])
+                segCount.append(len(segCol) - segStart[-1])
+                if segCount[-1] == 0:
+                    raise RuntimeError("synthetic hull produced an empty node; change the seed")
+                mu.append(perm); vmax.append(perm * vol); amin.append(max(1.0, 0.01 * footprint))
+                aov.append(max(1.0, 0.03 * footprint)); ncent.append(mom / vol)
+    nn = len(labels)
+
+    # groups
+    gS, gC, gN, gM, gpx, gpy, gpz = [], [], [], [], [], [], []
+
+    def group(members, mode, p=(0.0, 0.0, 0.0)):
+        gS.append(len(gN)); gC.append(len(members)); gN.extend(members); gM.append(mode)
+        gpx.append(p[0]); gpy.append(p[1]); gpz.append(p[2])
+        return len(gC) - 1
+
+    lower_group = {}
+    for k in range(ncomp):
+        xc = 0.5 * (bulkheads[k] + bulkheads[k + 1])
+        lp, up, ls, us = node_of[(k, 1, False)], node_of[(k, 1, True)], node_of[(k, 0, False)], node_of[(k, 0, True)]
+        r = rng.uniform()
+        if r < 0.1:
+            lower_group[k] = group([lp, ls, up, us], 1)
+            continue
+        if r < 0.55:
+            lower_group[k] = group([lp, ls], 1)
+        elif r < 0.8:
+            lower_group[k] = group([lp, ls], 2, (xc, 0.0, 0.3 * zdeck))
+        elif r < 0.9:
+            lower_group[k] = group([lp, ls], 3)
+        else:
+            group([lp], 0); group([ls], 0)
+        group([up, us], 1)
+
+    # connections: (a, b, law, x, y, z, area/width, coef, kind, idx, en, tOn, skipGroup, monitor)
+    conns = []
+    K = {name: i for i, name in enumerate(KIND_NAMES)}
+    for k in range(ncomp):
+        xc = 0.5 * (bulkheads[k] + bulkheads[k + 1])
+        hb = half_breadth(xc)
+        lp, up, ls, us = node_of[(k, 1, False)], node_of[(k, 1, True)], node_of[(k, 0, False)], node_of[(k, 0, True)]
+        skip = lower_group.get(k, -1) if (lower_group.get(k, -1) >= 0 and gC[lower_group[k]] == 2) else -1
+        conns.append((lp, ls, 0, xc, 0.0, 0.6, float(rng.uniform(0.5, 3.0)), 0.6, K["xlow"], k, 1, 0.0, skip, -1))
+        for side, lo_, up_ in ((1, lp, up), (0, ls, us)):
+            ysgn = 1.0 if side else -1.0
+            conns.append((up_, lo_, 0, xc, ysgn * 0.25 * hb, zdeck, float(rng.uniform(0.2, 2.0)), 0.6, K["down"], k, 1, 0.0, -1, -1))
+            conns.append((-1, up_, 1, xc, ysgn * 0.6 * hb, D + 0.02 * L * (2 * xc / L) ** 2, float(rng.uniform(1.0, 5.0)), 0.6, K["top"], k,
+                          1, 0.0, -1, -1))
+    for k in range(ncomp - 1):
+        xb = float(bulkheads[k + 1])
+        hb = half_breadth(xb)
+        top = zdeck + (D - zdeck) * float(rng.uniform(0.3, 0.8))
+        for side in (1, 0):
+            ysgn = 1.0 if side else -1.0
+            a_lo, b_lo = node_of[(k, side, False)], node_of[(k + 1, side, False)]
+            a_up, b_up = node_of[(k, side, True)], node_of[(k + 1, side, True)]
+            conns.append((a_lo, b_lo, 0, xb, ysgn * 0.25 * hb, 1.0, float(rng.uniform(0.5, 4.0)), 0.6, K["door"], k + 2,
+                          int(rng.uniform() < 0.3), 0.0, -1, -1))
+            conns.append((a_up, b_up, 1, xb, ysgn * 0.5 * hb, top, float(rng.uniform(2.0, 10.0)), 0.6, K["over"], k, 1, 0.0, -1, k))
+    nbreach = int(rng.integers(1, 7))
+    for m in range(nbreach):
+        k = int(rng.integers(max(1, ncomp // 3))) if m else 0
+        side = int(rng.integers(2))
+        xc = float(rng.uniform(bulkheads[k], bulkheads[k + 1]))
+        z = float(max(0.3, T - rng.uniform(0.5, 0.8 * T)))
+        tOn = 0.0 if m == 0 or rng.uniform() < 0.6 else float(rng.uniform(0, 1800))
+        conns.append((-1, node_of[(k, side, False)], 0, xc, (1 if side else -1) * 0.9 * half_breadth(xc), z, float(rng.uniform(0.01, 1.0)),
+                      0.6, K["breach"], m, 1, tOn, -1, -1))
+    nc = len(conns)
+    col = lambda q, dtp: np.array([c[q] for c in conns], dtype=dtp)
+
+    # hydrostatics at the design draft, and the body floating level there
+    wet = np.clip(T - czlo, 0.0, czhi - czlo)
+    V = float((cA * wet).sum())
+    xB = float((cA * wet * cx_).sum() / V)
+    KB = float((cA * wet * (czlo + 0.5 * wet)).sum() / V)
+    wp = czlo < T
+    IT = float((cA[wp] * cy_[wp] ** 2).sum())
+    IL = float((cA[wp] * (cx_[wp] - xB) ** 2).sum())
+    Awp = float(cA[wp].sum())
+    BMT, BML = IT / V, IL / V
+    KG = min(0.6 * D, KB + BMT - 0.04 * B)
+    GM = KB + BMT - KG
+    ms = rho * V
+    mh0, Ith0, Iph0 = 2 * ms, 2 * ms * (0.25 * L) ** 2, 1.2 * ms * (0.38 * B) ** 2
+    Kz, Kth, Kph = rho * g * Awp, rho * g * V * BML, rho * g * V * GM
+    body = np.array([ms, KG, xB, 0.0, KG, xB, 0.0, mh0, Ith0, Iph0, 2 * 0.6 * math.sqrt(Kz * mh0), 2 * 0.6 * math.sqrt(Kth * Ith0),
+                     2 * 0.12 * math.sqrt(Kph * Iph0), 0.0, 0.0, 0.0])
+    zO = KG - T
+    floor = np.array([min(segA[s] for s in range(segStart[n], segStart[n] + segCount[n])) for n in range(nn)]) - T
+    cen = np.array(ncent)
+    f64 = lambda v: np.asarray(v, dtype=np.float64)
+    i32 = lambda v: np.asarray(v, dtype=np.int32)
+    u8 = lambda v: np.asarray(v, dtype=np.uint8)
+    sid = f"synth{seed}"
+    ship = CompiledShip(sid, f"synthetic hull {seed}", "", cx_, cy_, cA, czlo, czhi, i32(ccomp), i32(cside), labels, f64(mu), f64(vmax),
+                        f64(amin), f64(aov), i32(segStart), i32(segCount), i32(segCol), f64(segA), f64(segE))
+    mk = [(0.45 * L, 0.0, D + 0.02 * L * 0.81, 0), (0.0, 0.0, D, 0), (-0.48 * L, 0.0, 0.5, 1)]
+    sim = CompiledSim(
+        "flood", f"synthetic flooding {seed}", sid, "", "", rho, g, dt, i32(gS), i32(gC), i32(gN), u8(gM), f64(gpx), f64(gpy), f64(gpz), 0.05,
+        col(0, np.int32), col(1, np.int32), col(2, np.uint8), col(3, np.float64), col(4, np.float64), col(5, np.float64),
+        col(6, np.float64), col(7, np.float64), col(8, np.uint8), col(9, np.int32), col(10, np.uint8), col(11, np.float64),
+        col(12, np.int32), col(13, np.int32), list(KIND_NAMES), 0.05, [f"over{k}" for k in range(ncomp - 1)],
+        [f"Water flows over bulkhead {k}" for k in range(ncomp - 1)], ["bow", "midship", "stern"],
+        ["Bow deck goes under", "Midship deck goes under", "Stern keel clears the water"], f64([m[0] for m in mk]),
+        f64([m[1] for m in mk]), f64([m[2] for m in mk]), u8([m[3] for m in mk]), f64([-0.5, 1.2, 1.0, 10.0]), body,
+        f64([L / 2, -L / 2]), f64([0.0, zO, 0.0, 0.0, 0.0, 0.0, 0.0]), np.zeros(nn), floor, cen[:, 0].copy(), cen[:, 1].copy(),
+        cen[:, 2].copy())
+    _hash_in_place(ship, sim)
+    return ship, sim
+
+
+def _ship_arrays(ship):
+    return [("columns.x", ship.x), ("columns.y", ship.y), ("columns.dxdy", ship.dxdy), ("columns.zlo", ship.zlo),
+            ("columns.zhi", ship.zhi), ("columns.zone", ship.zone), ("columns.side", ship.side), ("nodes.mu", ship.mu),
+            ("nodes.vmax", ship.vmax), ("nodes.amin", ship.amin), ("nodes.aov", ship.aov), ("nodes.segStart", ship.segStart),
+            ("nodes.segCount", ship.segCount), ("segments.col", ship.segCol), ("segments.a", ship.segA), ("segments.e", ship.segE)]
+
+
+def _sim_arrays(sim):
+    return [("groups.nodeStart", sim.gNodeStart), ("groups.nodeCount", sim.gNodeCount), ("groups.nodes", sim.gNodes),
+            ("groups.mode", sim.gMode), ("groups.px", sim.gpx), ("groups.py", sim.gpy), ("groups.pz", sim.gpz),
+            ("connections.a", sim.ca), ("connections.b", sim.cb), ("connections.type", sim.cLaw), ("connections.x", sim.cx),
+            ("connections.y", sim.cy), ("connections.z", sim.cz), ("connections.area", sim.cArea),
+            ("connections.coef", sim.cCoef), ("connections.kind", sim.cKind), ("connections.idx", sim.cIdx),
+            ("connections.en", sim.cEn), ("connections.tOn", sim.cTOn), ("connections.skipGroup", sim.cSkipGroup),
+            ("connections.monitor", sim.cMonitor), ("marks.x", sim.mkx), ("marks.y", sim.mky), ("marks.z", sim.mkz),
+            ("marks.when", sim.mkWhen), ("state0.vol", sim.vol0), ("state0.level", sim.level0), ("state0.cx", sim.cx0),
+            ("state0.cy", sim.cy0), ("state0.cz", sim.cz0)]
+
+
+def _hashes(arrays):
+    from .fnv import Fnv1a64
+    out, chain = {}, Fnv1a64()
+    for name, a in arrays:
+        one = Fnv1a64()
+        one.update_array(a)
+        chain.update_array(a)
+        out[name] = one.hex()
+    out["all"] = chain.hex()
+    return out
+
+
+def _hash_in_place(ship, sim):
+    ship.hashAll = _hashes(_ship_arrays(ship))["all"]
+    sim.shipHash = ship.hashAll
+    sim.hashAll = _hashes(_sim_arrays(sim))["all"]
+
+
+def _nums(a):
+    return [float(v) if isinstance(v, (float, np.floating)) else int(v) for v in np.asarray(a).tolist()]
+
+
+def write_compiled(ship, sim, ship_path: str, sim_path: str) -> None:
+    """The pair as sinksim.compiled-ship / sinksim.compiled-sim v1 files with every array hashed (docs/FORMATS.md),
+    readable by the reference's loaders and by this package's."""
+    import json
+    sh = _hashes(_ship_arrays(ship))
+    ship_doc = {
+        "format": "sinksim.compiled-ship", "formatVersion": 1, "id": ship.id,
+        "meta": {"title": ship.title, "generator": "sinksim_amd.synth", "engine": "python"},
+        "columns": {"n": ship.columns, "x": _nums(ship.x), "y": _nums(ship.y), "dxdy": _nums(ship.dxdy), "zlo": _nums(ship.zlo),
+                    "zhi": _nums(ship.zhi), "zone": _nums(ship.zone), "side": _nums(ship.side)},
+        "nodes": {"n": ship.nodes, "label": list(ship.nodeLabel), "mu": _nums(ship.mu), "vmax": _nums(ship.vmax),
+                  "amin": _nums(ship.amin), "aov": _nums(ship.aov), "segStart": _nums(ship.segStart), "segCount": _nums(ship.segCount)},
+        "segments": {"n": ship.segments, "col": _nums(ship.segCol), "a": _nums(ship.segA), "e": _nums(ship.segE)},
+        "hash": sh,
+    }
+    b = sim.body
+    mh = _hashes(_sim_arrays(sim))
+    sim_doc = {
+        "format": "sinksim.compiled-sim", "formatVersion": 1, "id": sim.id, "ship": ship.id, "shipHash": sh["all"],
+        "meta": {"title": sim.title, "generator": "sinksim_amd.synth", "engine": "python"},
+        "constants": {"rho": sim.rho, "g": sim.g}, "scheme": {"dt": sim.dt}, "params": {},
+        "groups": {"n": sim.groups, "nodeStart": _nums(sim.gNodeStart), "nodeCount": _nums(sim.gNodeCount), "nodes": _nums(sim.gNodes),
+                   "mode": _nums(sim.gMode), "px": _nums(sim.gpx), "py": _nums(sim.gpy), "pz": _nums(sim.gpz), "margin": sim.gMargin},
+        "connections": {"n": sim.connections, "a": _nums(sim.ca), "b": _nums(sim.cb), "type": _nums(sim.cLaw), "x": _nums(sim.cx),
+                        "y": _nums(sim.cy), "z": _nums(sim.cz), "area": _nums(sim.cArea), "coef": _nums(sim.cCoef),
+                        "kind": _nums(sim.cKind), "idx": _nums(sim.cIdx), "en": _nums(sim.cEn), "tOn": _nums(sim.cTOn),
+                        "skipGroup": _nums(sim.cSkipGroup), "monitor": _nums(sim.cMonitor), "kindNames": list(sim.kindNames)},
+        "monitors": {"n": sim.monitors, "threshold": sim.monitorThreshold, "id": list(sim.monitorId), "label": list(sim.monitorLabel)},
+        "marks": {"n": sim.marks, "id": list(sim.markId), "label": list(sim.markLabel), "x": _nums(sim.mkx), "y": _nums(sim.mky),
+                  "z": _nums(sim.mkz), "when": _nums(sim.mkWhen)},
+        "founder": dict(zip(("hullTopBelow", "pitchAbsAbove", "buoyancyBelow", "minT"), _nums(sim.founder))),
+        "body": {"ms": float(b[0]), "KG": float(b[1]), "g0": _nums(b[2:5]), "XO": float(b[5]), "YO": float(b[6]),
+                 **dict(zip(("mh0", "Ith0", "Iph0", "cDz", "cDth", "cDph", "qz", "qth", "qph"), _nums(b[7:16])))},
+        "readout": {"xFP": float(sim.readout[0]), "xAP": float(sim.readout[1])},
+        "state0": {**dict(zip(("t", "zO", "pitch", "roll", "vz", "wth", "wph"), _nums(sim.state0))), "vol": _nums(sim.vol0),
+                   "level": _nums(sim.level0), "cx": _nums(sim.cx0), "cy": _nums(sim.cy0), "cz": _nums(sim.cz0)},
+        "hash": mh,
+    }
+    with open(ship_path, "w") as f:
+        json.dump(ship_doc, f)
+    with open(sim_path, "w") as f:
+        json.dump(sim_doc, f)

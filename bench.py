@@ -27,9 +27,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two operations)
-# Algorithmic FP64 operations per simulation step of the titanic64 scenario (portable numerics, reference CPU
-# engine instrumented over the 1912 run: 4,408 column and ~9,435 segment evaluations and 71 node passes per step).
-FLOPS_PER_STEP = 2.31e5
+# Algorithmic FP64 operations per simulation step on this workload, counted by the oracle's operation counter over
+# the first 32 instances of the sweep (per step: 4,074 wet and 335 dry column evaluations, 2,760 partial, 1,799 full
+# and 5,135 dry segment evaluations, 72.8 node passes, 51.6 active connections, 34.5 pow calls) and priced by
+# oracle/pyoracle.py:flops_per_step: 2.44e5. tests/test_bench_constants.py re-counts a sample and pins this figure.
+FLOPS_PER_STEP = 2.44e5
 
 
 def parse():
@@ -53,6 +55,7 @@ def cpu_baseline(sw, sim, count: int) -> dict:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         pass
+    cores = min(cores, 16)  # a one-GPU host's CPU share
     sample = S.Sweep(ship=sw.ship, sim=sw.sim, tMax=sw.tMax, every=sw.every, instances=sw.instances[:count])
     ref = os.path.join(ROOT, "oracle", "_ref", "sinksim_sweep")
     if os.path.exists(ref):

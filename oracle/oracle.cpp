@@ -419,4 +419,16 @@ void orc_node_pass(const orc_ship* s, const double* f13, int order, int32_t n, d
 
 double orc_pow_v8(double x, double y) { return MathOracle::pow(x, y); }
 
+// Operation counting on the calling thread (counts: cwet cdry spart sfull sdry passes steps active pows wetNodes).
+void orc_count_begin(void) {
+  g_ops = OpCount{};
+  g_counting = true;
+}
+void orc_count_end(int64_t* out10) {
+  g_counting = false;
+  const long long v[10] = {g_ops.cwet, g_ops.cdry, g_ops.spart, g_ops.sfull, g_ops.sdry, g_ops.passes, g_ops.steps, g_ops.active, g_ops.pows,
+                           g_ops.wetNodes};
+  for (int k = 0; k < 10; ++k) out10[k] = v[k];
+}
+
 }  // extern "C"

@@ -86,6 +86,18 @@ struct MathStd {
 
 enum class Order { Serial, Tree, TreeMixed };
 
+// Operation counters of the algorithmic work (per thread), read by the benchmark's roofline: column evaluations
+// wet/dry, segment evaluations partial/full/dry, node passes, steps, active connections, pow calls, wet nodes.
+struct OpCount {
+  long long cwet, cdry, spart, sfull, sdry, passes, steps, active, pows, wetNodes;
+};
+inline thread_local OpCount g_ops{};
+inline thread_local bool g_counting = false;
+#define ORC_COUNT(field) \
+  do {                   \
+    if (g_counting) ++g_ops.field; \
+  } while (0)
+
 // ---------------------------------------------------------------- views (kernel/types.hpp:13-185)
 
 struct Ship {
@@ -266,7 +278,8 @@ inline void column(const Ship& S, const Frame& F, Real iR, Index i, HP<Real>& p)
   const Real wt = base + F.R22 * hi;
   if (wt > p.top) p.top = wt;
   const Real zp = -base * iR;
-  if (zp <= lo) return;
+  if (zp <= lo) { ORC_COUNT(cdry); return; }
+  ORC_COUNT(cwet);
   const Real s = zp >= hi ? hi - lo : zp - lo;
   const Real v = S.dxdy[i] * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
@@ -280,7 +293,8 @@ inline void column_f(const Ship& S, const FrameF& f, Index i, HP<float>& p) {
   const float wt = base + f.R22 * hi;
   if (wt > p.top) p.top = wt;
   const float zp = -base * f.iR;
-  if (zp <= lo) return;
+  if (zp <= lo) { ORC_COUNT(cdry); return; }
+  ORC_COUNT(cwet);
   const float s = zp >= hi ? hi - lo : zp - lo;
   const float v = static_cast<float>(S.dxdy[i]) * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
@@ -295,12 +309,14 @@ inline void segment(const Ship& S, const Frame& F, Real iR, Real h, Index j, PP<
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const Real zp = (h - base) * iR;
-  if (zp <= a) return;
+  if (zp <= a) { ORC_COUNT(sdry); return; }
   const Real A = S.dxdy[i];
   Real L;
   if (zp >= b) {
+    ORC_COUNT(sfull);
     L = b - a;
   } else {
+    ORC_COUNT(spart);
     L = zp - a;
     p.dv += A;
   }
@@ -318,12 +334,14 @@ inline void segment_f(const Ship& S, const FrameF& f, float h, Index j, PP<float
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const float zp = (h - base) * f.iR;
-  if (zp <= a) return;
+  if (zp <= a) { ORC_COUNT(sdry); return; }
   const float A = static_cast<float>(S.dxdy[i]);
   float L;
   if (zp >= b) {
+    ORC_COUNT(sfull);
     L = b - a;
   } else {
+    ORC_COUNT(spart);
     L = zp - a;
     p.dv += A;
   }
@@ -404,6 +422,7 @@ inline Buoy buoyancy(const Ship& S, const Frame& F) {
 
 template <Order O>
 inline Pass node_pass(const Ship& S, const Frame& F, Index n, Real h) {
+  ORC_COUNT(passes);
   const Index s0 = S.segStart[n], s1 = s0 + S.segCount[n];
   if constexpr (O == Order::Serial) {
     // hydro.hpp:261-293
@@ -638,12 +657,14 @@ inline Real flow(const Net& M, const Frame& F, const State& st, const Scratch& s
       Real f = 1;
       if (H2 > 0) {
         const Real r = H2 / H1;
+        if (g_counting) g_ops.pows += 2;
         f = Mth::pow(jmax(Real(0), 1 - Mth::pow(r, kWeirExponent)), kVillemonteExponent);
       }
       q = M.coef[c] * M.area[c] * kWeirCoefficient * sq2g * H1 * std::sqrt(H1) * f * (Ha >= Hb ? Real(1) : Real(-1));
     }
   }
   if (q == 0) { M.q[c] = 0; return 0; }
+  ORC_COUNT(active);
   Real dV = q * M.dt;
   const Index src = dV > 0 ? a : b, dst = dV > 0 ? b : a;
   const Real hs = src < 0 ? Real(0) : sc.hEff[src];
@@ -716,6 +737,7 @@ inline Loads loads(const Ship& S, const Net& M, const Frame& F, const Buoy& hb, 
   for (Index n = 0; n < S.nn; ++n) {
     const Real V = st.vol[n];
     if (V <= 0) continue;
+    ORC_COUNT(wetNodes);
     const Real x = sc.cx[n], y = sc.cy[n], z = sc.cz[n];
     const Real wx = F.R00 * x + F.R01 * y + F.R02 * z + F.tx - M.body.XO;
     const Real wy = F.R10 * x + F.R11 * y + F.R12 * z + F.ty - M.body.YO;
@@ -784,6 +806,7 @@ inline void track_events(const Net& M, const Frame& F, const State& st, const Ou
 
 template <class Mth, Order O>
 inline void step(const Ship& S, const Net& M, State& st, Scratch& sc, Outputs& out, Events& ev) {
+  ORC_COUNT(steps);
   const Frame F = pose<Mth>(M.body, st.zO, st.pitch, st.roll);
   const Buoy hb = buoyancy<O>(S, F);
   solve_levels<O>(S, M, F, false, st, sc);

@@ -300,3 +300,41 @@ def fdl_scalbn(x: float, n: int) -> float:
 
 def pow_v8(x: float, y: float) -> float:
     return lib().orc_pow_v8(x, y)
+
+
+def op_counts(ship, sim, numerics, en, area, tOn, t_max, every, model: Model | None = None) -> dict:
+    """Algorithmic operation counts of running every instance (one thread): column evaluations wet/dry, segment
+    evaluations partial/full/dry, node passes, steps, active connections, pow calls, wet nodes in the loads."""
+    model = model or Model(ship, sim)
+    L = lib()
+    L.orc_count_begin.argtypes = []
+    L.orc_count_end.argtypes = [C.POINTER(C.c_int64)]
+    en, area, tOn = _u(en), _f(area), _f(tOn)
+    L.orc_count_begin()
+    for i in range(en.shape[0]):
+        S = OracleSim(ship, sim, numerics, model)
+        for c in range(sim.connections):
+            S.set_conn(c, bool(en[i, c]), float(area[i, c]), float(tOn[i, c]))
+        S.reset()
+        while True:
+            ev, f, _ = S.events()
+            t = S.record()[0]
+            if f or t >= t_max:
+                break
+            S.step(min(4096, max(1, int((t_max - t) / sim.dt))))
+    out = np.zeros(10, np.int64)
+    L.orc_count_end(out.ctypes.data_as(C.POINTER(C.c_int64)))
+    keys = ("cwet", "cdry", "spart", "sfull", "sdry", "passes", "steps", "active", "pows", "wetNodes")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def flops_per_step(c: dict, nn: int, nconn: int, nsea: int) -> float:
+    """Algorithmic FP64 operations (add, sub, mul, div and sqrt each one; no FMA exists under -ffp-contract=off) per
+    step from the counts: 18 per wet column, 7 per dry column, 22 / 21 / 10 per partial / full / dry segment, 162 per
+    node pass (31 tree adds for each of 5 sums, 7 to finish), 1,020 buoyancy tree adds and 3 divisions, about 35 per
+    active connection plus about 60 per fdlibm pow, 37 per wet node in the loads, the gathers (2 x nconn), the sea sum,
+    and about 250 for the pose, integration and events."""
+    s = c["steps"]
+    main = 18 * c["cwet"] + 7 * c["cdry"] + 22 * c["spart"] + 21 * c["sfull"] + 10 * c["sdry"] + 162 * c["passes"]
+    net = 35 * c["active"] + 60 * c["pows"] + 37 * c["wetNodes"]
+    return (main + net) / s + 1023 + 2 * nconn + nsea + 250

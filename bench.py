@@ -85,6 +85,12 @@ def cpu_baseline(sw, sim, count: int) -> dict:
                       f"{steps} steps in {secs:.1f} s (file IO included)"}
 
 
+def _device() -> str:
+    from sinksim_amd.batch import device_info
+    d = device_info()
+    return f"{d['name'] or 'AMD GPU'} ({d['arch']}, {d['computeUnits']} CUs)"
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -92,7 +98,7 @@ def main():
               "would each run count/N of them", file=sys.stderr)
         sys.exit(2)
     from sinksim_amd import sweep as S
-    from sinksim_amd.batch import Batch, BatchOptions, device_info
+    from sinksim_amd.batch import Batch, BatchOptions
     from sinksim_amd.data_paths import titanic
     from sinksim_amd.workloads import titanic_batch_sweep
 
@@ -108,15 +114,16 @@ def main():
         secs = b.run()
         steps = b.total_steps()
         kms, launches = b.kernel_time()
+        steps0 = b.result(0).steps
         b.close()
-        return secs, steps, kms, launches
+        return secs, steps, kms, launches, steps0
 
     for w in range(args.warmup):
-        secs, steps, _, _ = one_run()
+        secs, steps, _, _, _ = one_run()
         print(f"warmup {w}: {steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
     tot_s, tot_steps, tot_kms, tot_launch = 0.0, 0, 0.0, 0
     for k in range(args.steps):
-        secs, steps, kms, launches = one_run()
+        secs, steps, kms, launches, steps0 = one_run()
         tot_s += secs
         tot_steps += steps
         tot_kms += kms
@@ -142,14 +149,13 @@ def main():
         "data": "synthetic (seeded sweep of the titanic64 model)",
         "config": {"workload": f"titanic64 batch of {args.count} seeded instances, {args.numerics} numerics, to founder "
                                f"or 4 h, curves every 30 s", "instances": args.count, "numerics": args.numerics,
-                   "sim_steps_per_bench_step": tot_steps // args.steps, "sinkings_per_min": None,
-                   "device": device_info()["name"]},
+                   "sim_steps_per_bench_step": tot_steps // args.steps,
+                   # full sinkings per minute at instance 0's length, as apps/sinksim_cuda_run.cpp:138-140 reports it
+                   "sinkings_per_min": round(value / max(steps0, 1) * 60.0, 1), "device": _device()},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
                      "kernel": "run_chunk<false>" if not mixed else "run_chunk<true>", "kernel_ms": round(per_launch_ms, 3)},
     }
-    # sinkings per minute as the reference reports them (apps/sinksim_cuda_run.cpp:138-140), against instance 0
-    out["config"]["sinkings_per_min"] = None
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample)
     print(json.dumps(out), flush=True)

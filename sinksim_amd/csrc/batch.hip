@@ -90,7 +90,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   for (int m = tid; m < nmon; m += kThreads) { B.overT[m] = P.Bt.overT[static_cast<size_t>(inst) * nmon + m]; B.over[m] = 0; }
   for (int m = tid; m < nmark; m += kThreads) B.markT[m] = P.Bt.markT[static_cast<size_t>(inst) * nmark + m];
   __syncthreads();
-  if (tid < 64) pose_wave0(P, B, tid);  // the pose of the loaded state for the first step
+  if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);  // the loaded state's pose
   __syncthreads();
 
   const bool traced = inst < P.Bt.traced;
@@ -114,15 +114,20 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   const bool curves = P.Bt.curveEvery > 0;
   for (long long s = 0; s < maxSteps; ++s) {
     if (B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= tMax) break;  // uniform: LDS values after a barrier
-    // a readout-curve record when due (simulation.cpp:175-176); uniform condition on LDS values
+    // a readout-curve record when due (simulation.cpp:175-176). Every wave evaluates the condition on the same
+    // LDS values; the barrier inside orders all of those reads (and the record's reads of the volumes) before
+    // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
     if (curves && B.scal[S_T] >= B.scal[S_NEXTCURVE] && B.scal[S_CURVECOUNT] < P.Bt.curveMax) {
       if (tid == 0) {
         SSG double* rec = P.Bt.curveRec + (static_cast<size_t>(inst) * P.Bt.curveMax + static_cast<size_t>(B.scal[S_CURVECOUNT])) * kCurveValues;
         readouts7(P, B, rec);
+      }
+      __syncthreads();
+      if (tid == 0) {
         B.scal[S_CURVECOUNT] += 1;
         B.scal[S_NEXTCURVE] += P.Bt.curveEvery;
       }
-      __syncthreads();  // the step below rewrites the volumes the record read
+      // read again only at the next iteration, after the step's barriers
     }
     device_step<Mixed>(P, B, tid);
     ++done;
@@ -282,9 +287,11 @@ struct ss_batch {
   Dev<float> xf, yf, dxdyf, zlof, zhif, sxf, syf, sAf, saf, sef;
   Dev<int32_t> segStart, segCount;
   // scenario
-  Dev<int32_t> gStart, gCount, gNodes, ca, cb, skipGroup, incStart, incCount, incConn, sea, monStart, monCount, monConn;
-  Dev<uint8_t> gMode, law, incSide, mkWhen;
-  Dev<double> gpx, gpy, gpz, cx, cy, cz, coef, mkx, mky, mkz;
+  Dev<int32_t> gStart, gCount, gNodes, incStart, incCount, sea, monStart, monCount, monConn;
+  Dev<int4> inc, conI;
+  Dev<double4> conD;
+  Dev<uint8_t> gMode, mkWhen;
+  Dev<double> gpx, gpy, gpz, mkx, mky, mkz;
   DevNet net{};
   // per instance
   Dev<uint8_t> en, foundered, evKind;
@@ -534,8 +541,13 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     for (size_t g = 0; g < ng; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
     b.gStart.upload(M.gNodeStart, ng); b.gCount.upload(M.gNodeCount, ng); b.gNodes.upload(M.gNodes, members); b.gMode.upload(M.gMode, ng);
     b.gpx.upload(M.gpx, ng); b.gpy.upload(M.gpy, ng); b.gpz.upload(M.gpz, ng);
-    b.ca.upload(M.ca, ncn); b.cb.upload(M.cb, ncn); b.law.upload(M.law, ncn); b.cx.upload(M.cx, ncn); b.cy.upload(M.cy, ncn);
-    b.cz.upload(M.cz, ncn); b.coef.upload(M.coef, ncn); b.skipGroup.upload(M.skipGroup, ncn);
+    std::vector<int4> conI(std::max<size_t>(ncn, 1));
+    std::vector<double4> conD(std::max<size_t>(ncn, 1));
+    for (size_t c = 0; c < ncn; ++c) {
+      conI[c] = make_int4(M.ca[c], M.cb[c], M.skipGroup[c], M.law[c]);
+      conD[c] = make_double4(M.cx[c], M.cy[c], M.cz[c], M.coef[c]);
+    }
+    b.conI.upload(conI); b.conD.upload(conD);
     std::vector<int32_t> incCount(nn, 0), incStart(nn, 0), fill(nn, 0), sea;
     for (size_t c = 0; c < ncn; ++c) {
       if (M.ca[c] >= 0) incCount[M.ca[c]]++;
@@ -543,21 +555,19 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     }
     int32_t total = 0;
     for (size_t n = 0; n < nn; ++n) { incStart[n] = total; total += incCount[n]; }
-    std::vector<int32_t> incConn(total, -1);
-    std::vector<uint8_t> incSide(total, 0);
+    std::vector<int4> inc(std::max<int32_t>(total, 1), make_int4(0, 0, 0, 0));
     for (size_t c = 0; c < ncn; ++c) {
+      const int32_t ci = static_cast<int32_t>(c);
       if (M.ca[c] >= 0) {
         const int32_t n = M.ca[c];
-        const int32_t k = incStart[n] + fill[n]++;
-        incConn[k] = static_cast<int32_t>(c); incSide[k] = 0;
+        inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 0);
       } else {
-        sea.push_back(static_cast<int32_t>(c));
+        sea.push_back(ci);
       }
       const int32_t n = M.cb[c];
-      const int32_t k = incStart[n] + fill[n]++;
-      incConn[k] = static_cast<int32_t>(c); incSide[k] = 1;
+      inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 1);
     }
-    b.incStart.upload(incStart); b.incCount.upload(incCount); b.incConn.upload(incConn); b.incSide.upload(incSide); b.sea.upload(sea);
+    b.incStart.upload(incStart); b.incCount.upload(incCount); b.inc.upload(inc); b.sea.upload(sea);
     std::vector<int32_t> monCount(std::max<int32_t>(b.nmon, 0), 0), monStart(std::max<int32_t>(b.nmon, 0), 0), monConn;
     for (int32_t m = 0; m < b.nmon; ++m) {
       monStart[m] = static_cast<int32_t>(monConn.size());
@@ -572,9 +582,8 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
     d.ng = M.ngroup; d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
     d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
-    d.nc = M.nconn; d.ca = G(b.ca.p); d.cb = G(b.cb.p); d.law = G(b.law.p); d.cx = G(b.cx.p); d.cy = G(b.cy.p); d.cz = G(b.cz.p); d.coef = G(b.coef.p);
-    d.skipGroup = G(b.skipGroup.p);
-    d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.incConn = G(b.incConn.p); d.incSide = G(b.incSide.p);
+    d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p;
+    d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.inc = (const SSG i4v*)b.inc.p;
     d.seaCount = static_cast<int32_t>(sea.size()); d.sea = G(b.sea.p);
     d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = G(b.monStart.p); d.monCount = G(b.monCount.p); d.monConn = G(b.monConn.p);
     d.nmark = M.nmark; d.mkx = G(b.mkx.p); d.mky = G(b.mky.p); d.mkz = G(b.mkz.p); d.mkWhen = G(b.mkWhen.p);

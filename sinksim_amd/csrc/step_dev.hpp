@@ -57,6 +57,10 @@ constexpr double kPi = 3.141592653589793;
 __device__ __forceinline__ double jmax(double a, double b) { return (b > a) ? b : a; }
 __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b : a; }
 
+// four 32-bit integers or four doubles loaded as one vector (the packed per-connection and incidence records)
+typedef int32_t i4v __attribute__((ext_vector_type(4)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
 using gcd = const SSG double;
 using gcf = const SSG float;
 using gci = const SSG int32_t;
@@ -86,12 +90,10 @@ struct DevNet {
   gcd *gpx, *gpy, *gpz;
   double gMargin;
   int32_t nc;
-  gci *ca, *cb;
-  gcb* law;
-  gcd *cx, *cy, *cz, *coef;
-  gci* skipGroup;
-  gci *incStart, *incCount, *incConn;
-  gcb* incSide;
+  gci *incStart, *incCount;
+  const SSG i4v* inc;   // per incidence entry (node order, increasing connection): connection, its a, its b, side (1 = b)
+  const SSG i4v* conI;  // per connection: a, b, skipGroup, law
+  const SSG d4v* conD;  // per connection: x, y, z, coef
   int32_t seaCount;
   gci* sea;
   int32_t nmon;
@@ -210,11 +212,12 @@ __device__ inline void write_frame(PC& P, SSL double* fr, double zO, double cth,
   ff[0] = f20; ff[1] = f21; ff[2] = f22; ff[3] = ftz; ff[4] = 1.0f / f22;
 }
 
-// The pose of the current state into the frame slots: lanes 0..3 of wave 0 evaluate cos/sin of pitch and roll,
-// lane 0 writes the frame. Call from wave 0 only; visible to the workgroup after the next barrier.
-__device__ inline void pose_wave0(PC& P, const Blk& B, int lane) {
+// The pose (zO, pitch, roll: the same value in every lane) into the frame slots: lanes 0..3 of wave 0 evaluate
+// cos/sin of pitch and roll, lane 0 writes the frame. Call from wave 0 only; visible to the workgroup after the
+// next barrier.
+__device__ inline void pose_wave0(PC& P, const Blk& B, int lane, double zO, double pitch, double roll) {
   double v = 0;
-  if (lane < 4) v = trig(lane < 2 ? B.scal[S_PITCH] : B.scal[S_ROLL], (lane & 1) == 0 ? 1 : 0);
+  if (lane < 4) v = trig(lane < 2 ? pitch : roll, (lane & 1) == 0 ? 1 : 0);
   const double cth = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 0) << 32) |
                                                     (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 0));
   const double sth = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 1) << 32) |
@@ -223,7 +226,7 @@ __device__ inline void pose_wave0(PC& P, const Blk& B, int lane) {
                                                     (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 2));
   const double sph = __builtin_bit_cast(double, ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(__builtin_bit_cast(uint64_t, v) >> 32), 3) << 32) |
                                                     (uint32_t)__builtin_amdgcn_readlane((int)__builtin_bit_cast(uint64_t, v), 3));
-  if (lane == 0) write_frame(P, B.frame, B.scal[S_ZO], cth, sth, cph, sph);
+  if (lane == 0) write_frame(P, B.frame, zO, cth, sth, cph, sph);
 }
 
 __device__ __forceinline__ double world_z(const SSL double* fr, double x, double y, double z) {
@@ -318,39 +321,52 @@ __device__ __forceinline__ Fr32 fr32(const SSL double* fr) {
   return Fr32{f[0], f[1], f[2], f[3], f[4]};
 }
 
-__device__ __forceinline__ void column(PC& P, const Fr64& F, int i, HP<double>& p) {
-  const double x = P.S.x[i], y = P.S.y[i], lo = P.S.zlo[i], hi = P.S.zhi[i];
+// A column's or a segment's geometry, fetched ahead of its evaluation so that a lane's loads of several
+// columns or segments are in flight together.
+template <class T>
+struct Geo {
+  T x, y, lo, hi, A;
+};
+__device__ __forceinline__ Geo<double> column_geo(PC& P, const double*, int i) { return Geo<double>{P.S.x[i], P.S.y[i], P.S.zlo[i], P.S.zhi[i], P.S.dxdy[i]}; }
+__device__ __forceinline__ Geo<float> column_geo(PC& P, const float*, int i) { return Geo<float>{P.S.xf[i], P.S.yf[i], P.S.zlof[i], P.S.zhif[i], P.S.dxdyf[i]}; }
+__device__ __forceinline__ Geo<double> segment_geo(PC& P, const double*, int j) { return Geo<double>{P.S.sx[j], P.S.sy[j], P.S.sa[j], P.S.se[j], P.S.sA[j]}; }
+__device__ __forceinline__ Geo<float> segment_geo(PC& P, const float*, int j) { return Geo<float>{P.S.sxf[j], P.S.syf[j], P.S.saf[j], P.S.sef[j], P.S.sAf[j]}; }
+
+constexpr int kFetch = 4;  // columns or segments a lane fetches before evaluating them in order
+
+__device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, HP<double>& p) {
+  const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
   const double wt = base + F.R22 * hi;
   if (wt > p.top) p.top = wt;
   const double zp = -base * F.iR;
   if (zp <= lo) return;
   const double s = zp >= hi ? hi - lo : zp - lo;
-  const double v = P.S.dxdy[i] * s;
+  const double v = g.A * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
 }
-__device__ __forceinline__ void column(PC& P, const Fr32& f, int i, HP<float>& p) {
-  const float x = P.S.xf[i], y = P.S.yf[i], lo = P.S.zlof[i], hi = P.S.zhif[i];
+__device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, HP<float>& p) {
+  const float x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const float base = f.R20 * x + f.R21 * y + f.tz;
   const float wt = base + f.R22 * hi;
   if (wt > p.top) p.top = wt;
   const float zp = -base * f.iR;
   if (zp <= lo) return;
   const float s = zp >= hi ? hi - lo : zp - lo;
-  const float v = P.S.dxdyf[i] * s;
+  const float v = g.A * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
 }
 
 // one segment into a node-pass partial (reduce.hpp:80-99 / 172-192)
-__device__ __forceinline__ void segment(PC& P, const Fr64& F, double h, int j, PP<double>& p) {
-  const double x = P.S.sx[j], y = P.S.sy[j], a = P.S.sa[j], b = P.S.se[j];
+__device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<double>& g, PP<double>& p) {
+  const double x = g.x, y = g.y, a = g.lo, b = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
   const double wa = base + F.R22 * a, wb = base + F.R22 * b;
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const double zp = (h - base) * F.iR;
   if (zp <= a) return;
-  const double A = P.S.sA[j];
+  const double A = g.A;
   double L;
   if (zp >= b) {
     L = b - a;
@@ -361,15 +377,15 @@ __device__ __forceinline__ void segment(PC& P, const Fr64& F, double h, int j, P
   const double v = A * L;
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
 }
-__device__ __forceinline__ void segment(PC& P, const Fr32& f, float h, int j, PP<float>& p) {
-  const float x = P.S.sxf[j], y = P.S.syf[j], a = P.S.saf[j], b = P.S.sef[j];
+__device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>& g, PP<float>& p) {
+  const float x = g.x, y = g.y, a = g.lo, b = g.hi;
   const float base = f.R20 * x + f.R21 * y + f.tz;
   const float wa = base + f.R22 * a, wb = base + f.R22 * b;
   if (wa < p.zmin) p.zmin = wa;
   if (wb > p.zmax) p.zmax = wb;
   const float zp = (h - base) * f.iR;
   if (zp <= a) return;
-  const float A = P.S.sAf[j];
+  const float A = g.A;
   float L;
   if (zp >= b) {
     L = b - a;
@@ -400,7 +416,14 @@ struct UnitPass {
       const Fr32 f = fr32(fr);
       const float hf = static_cast<float>(h);
       PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
-      for (int j = s0 + lane; j < s1; j += kLanes) segment(P, f, hf, j, p);
+      for (int j0 = s0 + lane; j0 < s1; j0 += kFetch * kLanes) {
+        Geo<float> g[kFetch];
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u) g[u] = segment_geo(P, static_cast<const float*>(nullptr), j0 + u * kLanes < s1 ? j0 + u * kLanes : j0);
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u)
+          if (j0 + u * kLanes < s1) segment(f, hf, g[u], p);
+      }
       tree32(p, lane);
       vol = static_cast<double>(lead(p.vol, upper)); dv = static_cast<double>(lead(p.dv, upper));
       mx = static_cast<double>(lead(p.mx, upper)); my = static_cast<double>(lead(p.my, upper)); mz = static_cast<double>(lead(p.mz, upper));
@@ -408,7 +431,14 @@ struct UnitPass {
     } else {
       const Fr64 F = fr64(fr);
       PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
-      for (int j = s0 + lane; j < s1; j += kLanes) segment(P, F, h, j, p);
+      for (int j0 = s0 + lane; j0 < s1; j0 += kFetch * kLanes) {
+        Geo<double> g[kFetch];
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u) g[u] = segment_geo(P, static_cast<const double*>(nullptr), j0 + u * kLanes < s1 ? j0 + u * kLanes : j0);
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u)
+          if (j0 + u * kLanes < s1) segment(F, h, g[u], p);
+      }
       tree32(p, lane);
       vol = lead(p.vol, upper); dv = lead(p.dv, upper); mx = lead(p.mx, upper); my = lead(p.my, upper); mz = lead(p.mz, upper);
       zmin = lead(p.zmin, upper); zmax = lead(p.zmax, upper);
@@ -501,9 +531,12 @@ __device__ void solve_group(PC& P, const PassFn& pass, gci* members, int N, cons
     h = B.level[members[firstWet >= 0 ? firstWet : N - 1]];
   }
   double lo = 0, hi = 0;
+  // Pp is written by every lane of the unit (each lane holds the same pass result) and read back by each lane
+  // after its own write, so no lane depends on another lane's LDS store between barriers.
   for (int k = 0; k < N; ++k) {
     const Pass p = pass(members[k], h);
-    if (lead_) { put_pass(Pp, k, p); B.zmin[members[k]] = p.zmin; B.zmax[members[k]] = p.zmax; }
+    put_pass(Pp, k, p);
+    if (lead_) { B.zmin[members[k]] = p.zmin; B.zmax[members[k]] = p.zmax; }
     if (k == 0) { lo = p.zmin; hi = p.zmax; }
     else { if (p.zmin < lo) lo = p.zmin; if (p.zmax > hi) hi = p.zmax; }
   }
@@ -539,7 +572,7 @@ __device__ void solve_group(PC& P, const PassFn& pass, gci* members, int N, cons
     h = jmin(hi, jmax(lo, h));
     for (int k = 0; k < N; ++k) {
       const Pass p = pass(members[k], h);
-      if (lead_) put_pass(Pp, k, p);
+      put_pass(Pp, k, p);
     }
   }
   const double tol = jmax(1e-3 * Vmax, 0.5);
@@ -557,7 +590,7 @@ __device__ void solve_group(PC& P, const PassFn& pass, gci* members, int N, cons
     h = hn;
     for (int k = 0; k < N; ++k) {
       const Pass p = pass(members[k], h);
-      if (lead_) put_pass(Pp, k, p);
+      put_pass(Pp, k, p);
     }
   }
   // share the water in proportion to the geometry at this level (levels.hpp:125-161)
@@ -636,29 +669,36 @@ __device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
   B.aEff[n] = v >= P.S.vmax[n] ? P.S.aov[n] : jmax(B.area[n], P.S.amin[n]);
   const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   int d = 0;
-  for (int k = s; k < e; ++k) {
-    const int c = P.M.incConn[k];
-    if (!B.cen[c]) continue;
-    const int a = P.M.ca[c], b = P.M.cb[c];
-    if (a >= 0 && head_of(B, a) < kNoHeadTest && head_of(B, b) < kNoHeadTest) continue;
-    ++d;
+  for (int k0 = s; k0 < e; k0 += kFetch) {
+    i4v q[kFetch];
+#pragma unroll
+    for (int u = 0; u < kFetch; ++u) q[u] = P.M.inc[k0 + u < e ? k0 + u : k0];
+#pragma unroll
+    for (int u = 0; u < kFetch; ++u) {
+      if (k0 + u >= e || !B.cen[q[u].x]) continue;
+      const int a = q[u].y, b = q[u].z;
+      if (a >= 0 && head_of(B, a) < kNoHeadTest && head_of(B, b) < kNoHeadTest) continue;
+      ++d;
+    }
   }
   B.deg[n] = d;
 }
 
 // flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter
 __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, const Blk& B, int c) {
+  const i4v ci = P.M.conI[c];
+  const d4v cd = P.M.conD[c];
   if (!B.cen[c] || B.ctOn[c] > t) { B.q[c] = 0; return 0; }
-  if (P.M.skipGroup[c] >= 0 && B.merged[P.M.skipGroup[c]]) { B.q[c] = 0; return 0; }
-  const int a = P.M.ca[c], b = P.M.cb[c];
-  const double zo = fr[F_R20] * P.M.cx[c] + fr[F_R21] * P.M.cy[c] + fr[F_R22] * P.M.cz[c] + fr[F_TZ];
+  if (ci.z >= 0 && B.merged[ci.z]) { B.q[c] = 0; return 0; }
+  const int a = ci.x, b = ci.y;
+  const double zo = fr[F_R20] * cd.x + fr[F_R21] * cd.y + fr[F_R22] * cd.z + fr[F_TZ];
   const double ha = a < 0 ? 0.0 : B.hEff[a], hbv = B.hEff[b];
   const double Ha = ha > zo ? ha - zo : 0.0, Hb = hbv > zo ? hbv - zo : 0.0;
   const double sq2g = P.M.sq2g;
   double q = 0;
-  if (P.M.law[c] == 0) {
+  if (ci.w == 0) {
     const double d = Ha - Hb;
-    if (d != 0) q = P.M.coef[c] * B.carea[c] * sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
+    if (d != 0) q = cd.w * B.carea[c] * sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
   } else {
     const double H1 = Ha > Hb ? Ha : Hb;
     if (H1 > 0) {
@@ -668,7 +708,7 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
         const double r = H2 / H1;
         f = dpow(jmax(0.0, 1 - dpow(r, kWeirExponent)), kVillemonteExponent);
       }
-      q = P.M.coef[c] * B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
+      q = cd.w * B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
     }
   }
   if (q == 0) { B.q[c] = 0; return 0; }
@@ -692,9 +732,16 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
 __device__ inline void node_update(PC& P, const Blk& B, int n) {
   const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   double acc = 0;
-  for (int k = s; k < e; ++k) {
-    const double d = B.dV[P.M.incConn[k]];
-    acc = P.M.incSide[k] ? acc + d : acc - d;
+  for (int k0 = s; k0 < e; k0 += kFetch) {
+    i4v q[kFetch];
+    double d[kFetch];
+#pragma unroll
+    for (int u = 0; u < kFetch; ++u) q[u] = P.M.inc[k0 + u < e ? k0 + u : k0];
+#pragma unroll
+    for (int u = 0; u < kFetch; ++u) d[u] = B.dV[q[u].x];
+#pragma unroll
+    for (int u = 0; u < kFetch; ++u)
+      if (k0 + u < e) acc = q[u].w ? acc + d[u] : acc - d[u];
   }
   double v = B.vol[n] + acc;
   if (v < 0) v = 0;
@@ -762,13 +809,27 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   if constexpr (Mixed) {
     const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
-    for (int i = tid; i < P.S.ncol; i += kThreads) column(P, f, i, p);
+    for (int i0 = tid; i0 < P.S.ncol; i0 += kFetch * kThreads) {
+      Geo<float> g[kFetch];
+#pragma unroll
+      for (int u = 0; u < kFetch; ++u) g[u] = column_geo(P, static_cast<const float*>(nullptr), i0 + u * kThreads < P.S.ncol ? i0 + u * kThreads : i0);
+#pragma unroll
+      for (int u = 0; u < kFetch; ++u)
+        if (i0 + u * kThreads < P.S.ncol) column(f, g[u], p);
+    }
     tree32(p, lane);
     if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   } else {
     const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
-    for (int i = tid; i < P.S.ncol; i += kThreads) column(P, F, i, p);
+    for (int i0 = tid; i0 < P.S.ncol; i0 += kFetch * kThreads) {
+      Geo<double> g[kFetch];
+#pragma unroll
+      for (int u = 0; u < kFetch; ++u) g[u] = column_geo(P, static_cast<const double*>(nullptr), i0 + u * kThreads < P.S.ncol ? i0 + u * kThreads : i0);
+#pragma unroll
+      for (int u = 0; u < kFetch; ++u)
+        if (i0 + u * kThreads < P.S.ncol) column(F, g[u], p);
+    }
     tree32(p, lane);
     if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   }
@@ -796,9 +857,11 @@ __device__ __forceinline__ double bcast(double v, int from) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// The tail of a step on wave 0 (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp): lanes 0..6 run the
-// seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess), lane 0
-// integrates and tracks events, then lanes 0..3 evaluate the pose of the next step.
+// The tail of a step on wave 0 (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp). Lanes 0..6 run the
+// seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess), eight
+// nodes' values fetched at a time; every lane then integrates the same uniform values, the lanes test the monitors
+// and marks side by side and lane 0 records the crossings in order; lanes 0..3 evaluate the pose of the next step.
+// Values flow between lanes through registers only: LDS written here is read after the step's closing barrier.
 __device__ void step_tail(PC& P, const Blk& B, int lane) {
   const SSL double* fr = B.frame;
   double hV, hx, hy, hz, htop;
@@ -807,70 +870,96 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
   const double Fb = W * hV;
   const double bxw = fr[F_R00] * hx + fr[F_R01] * hy + fr[F_R02] * hz + fr[F_TX];
   const double byw = fr[F_R10] * hx + fr[F_R11] * hy + fr[F_R12] * hz + fr[F_TY];
+  const double t0 = B.scal[S_T];
+  double vz = B.scal[S_VZ], wth = B.scal[S_WTH], wph = B.scal[S_WPH];
+  double zO = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
+  const bool wasFoundered = B.scal[S_FOUNDERED] != 0;
   const int NN = P.S.nn;
   double acc = 0;
   if (lane < 7) {
     const SSL double* src = lane == 0 ? B.tX : lane == 1 ? B.tY : lane == 2 ? B.tF : lane == 3 ? B.tM : lane == 4 ? B.tIp : lane == 5 ? B.tIr : B.excess;
     const SSL double* gate = lane < 6 ? B.vol : B.excess;
     const bool subtract = lane == 1 || lane == 6;
+    const bool loads = lane < 6;
     acc = lane == 0 ? (byw - P.M.YO) * Fb : lane == 1 ? -(bxw - P.M.XO) * Fb : lane == 2 ? Fb - P.M.g * P.M.ms : lane == 6 ? B.scal[S_SEA] : 0.0;
-    for (int n = 0; n < NN; ++n) {
-      const double gv = gate[n];
-      if (lane < 6 ? !(gv <= 0) : gv > 0) {
-        const double v = src[n];
-        acc = subtract ? acc - v : acc + v;
+    for (int n0 = 0; n0 < NN; n0 += 8) {
+      double gv[8], sv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int n = n0 + k < NN ? n0 + k : NN - 1;
+        gv[k] = gate[n];
+        sv[k] = src[n];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        if (n0 + k < NN && (loads ? !(gv[k] <= 0) : gv[k] > 0)) acc = subtract ? acc - sv[k] : acc + sv[k];
       }
     }
   }
   const double tauX = bcast(acc, 0), tauY = bcast(acc, 1), Fz = bcast(acc, 2), mw = bcast(acc, 3);
   const double Iw_p = bcast(acc, 4), Iw_r = bcast(acc, 5), seaIn = bcast(acc, 6);
+  // integrate (body.hpp:42-57)
+  const double Mth = tauY, Mph = tauX * fr[F_CTH];
+  const double dt = P.M.dt;
+  const double mh = P.M.mh0 + mw;
+  const double Ith = P.M.Ith0 + Iw_p, Iph = P.M.Iph0 + Iw_r;
+  const double az = (Fz - P.M.cDz * vz - P.M.qz * __builtin_fabs(vz) * vz) / mh;
+  const double ath = (Mth - P.M.cDth * wth - P.M.qth * __builtin_fabs(wth) * wth) / Ith;
+  const double aph = (Mph - P.M.cDph * wph - P.M.qph * __builtin_fabs(wph) * wph) / Iph;
+  vz += az * dt; wth += ath * dt; wph += aph * dt;
+  zO += vz * dt; pitch += wth * dt; roll += wph * dt;
+  if (pitch > kPitchClamp) { pitch = kPitchClamp; wth = 0; }
+  if (pitch < -kPitchClamp) { pitch = -kPitchClamp; wth = 0; }
+  if (roll > kRollClamp) { roll = kRollClamp; wph = 0; }
+  if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; }
+  const double tn = t0 + dt;
   if (lane == 0) {
-    B.scal[O_INFLOW] = seaIn / P.M.dt;
-    const double Mth = tauY, Mph = tauX * fr[F_CTH];
-    // integrate (body.hpp:42-57)
-    double vz = B.scal[S_VZ], wth = B.scal[S_WTH], wph = B.scal[S_WPH];
-    double zO = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
-    const double dt = P.M.dt;
-    const double mh = P.M.mh0 + mw;
-    const double Ith = P.M.Ith0 + Iw_p, Iph = P.M.Iph0 + Iw_r;
-    const double az = (Fz - P.M.cDz * vz - P.M.qz * __builtin_fabs(vz) * vz) / mh;
-    const double ath = (Mth - P.M.cDth * wth - P.M.qth * __builtin_fabs(wth) * wth) / Ith;
-    const double aph = (Mph - P.M.cDph * wph - P.M.qph * __builtin_fabs(wph) * wph) / Iph;
-    vz += az * dt; wth += ath * dt; wph += aph * dt;
-    zO += vz * dt; pitch += wth * dt; roll += wph * dt;
-    if (pitch > kPitchClamp) { pitch = kPitchClamp; wth = 0; }
-    if (pitch < -kPitchClamp) { pitch = -kPitchClamp; wth = 0; }
-    if (roll > kRollClamp) { roll = kRollClamp; wph = 0; }
-    if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; }
-    const double tn = B.scal[S_T] + dt;
+    B.scal[O_INFLOW] = seaIn / dt;
     B.scal[S_T] = tn; B.scal[S_ZO] = zO; B.scal[S_PITCH] = pitch; B.scal[S_ROLL] = roll;
     B.scal[S_VZ] = vz; B.scal[S_WTH] = wth; B.scal[S_WPH] = wph;
     B.scal[O_VB] = hV; B.scal[O_TOP] = htop; B.scal[O_MW] = mw; B.scal[O_BX] = hx; B.scal[O_BY] = hy; B.scal[O_BZ] = hz;
-    // events (events.hpp:11-38), with the pose of the step and the time after it
-    for (int m = 0; m < P.M.nmon; ++m) {
-      if (B.over[m] > P.M.monThreshold && B.overT[m] < 0) {
-        B.overT[m] = tn;
-        push_event(P, B, tn, m, 0);
+  }
+  // events (events.hpp:11-38), with the pose of the step and the time after it: monitors in order, then marks
+  const int nmon = P.M.nmon, nmark = P.M.nmark;
+  for (int m0 = 0; m0 < nmon; m0 += 64) {
+    const int m = m0 + lane;
+    const bool fire = m < nmon && B.over[m] > P.M.monThreshold && B.overT[m] < 0;
+    uint64_t mask = __ballot(fire);
+    if (lane == 0) {
+      while (mask) {
+        const int k = m0 + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        B.overT[k] = tn;
+        push_event(P, B, tn, k, 0);
       }
-    }
-    for (int i = 0; i < P.M.nmark; ++i) {
-      if (B.markT[i] >= 0) continue;
-      const double wz = world_z(fr, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]);
-      const bool under = P.M.mkWhen[i] == 0;
-      if ((under && wz < 0) || (!under && wz > 0)) {
-        B.markT[i] = tn;
-        push_event(P, B, tn, i, 1);
-      }
-    }
-    if (B.scal[S_FOUNDERED] == 0 &&
-        (htop < P.M.hullTopBelow || __builtin_fabs(pitch) > P.M.pitchAbsAbove || (hV < P.M.buoyancyBelow && tn > P.M.minT))) {
-      B.scal[S_FOUNDERED] = 1;
-      B.scal[S_FOUNDERT] = tn;
-      push_event(P, B, tn, -1, 2);
     }
   }
+  for (int i0 = 0; i0 < nmark; i0 += 64) {
+    const int i = i0 + lane;
+    bool fire = false;
+    if (i < nmark && !(B.markT[i] >= 0)) {
+      const double wz = world_z(fr, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]);
+      const bool under = P.M.mkWhen[i] == 0;
+      fire = (under && wz < 0) || (!under && wz > 0);
+    }
+    uint64_t mask = __ballot(fire);
+    if (lane == 0) {
+      while (mask) {
+        const int k = i0 + __builtin_ctzll(mask);
+        mask &= mask - 1;
+        B.markT[k] = tn;
+        push_event(P, B, tn, k, 1);
+      }
+    }
+  }
+  if (lane == 0 && !wasFoundered &&
+      (htop < P.M.hullTopBelow || __builtin_fabs(pitch) > P.M.pitchAbsAbove || (hV < P.M.buoyancyBelow && tn > P.M.minT))) {
+    B.scal[S_FOUNDERED] = 1;
+    B.scal[S_FOUNDERT] = tn;
+    push_event(P, B, tn, -1, 2);
+  }
   // the pose of the next step (frame.hpp:9-21); the frame slots are free: every reader of this step is done
-  pose_wave0(P, B, lane);
+  pose_wave0(P, B, lane, zO, pitch, roll);
 }
 
 // One step. On entry the frame slots hold the pose of the current state (written by the previous step's tail or

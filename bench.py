@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--numerics", default="portable", choices=["portable", "portable32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48, help="instances in the CPU-baseline sample")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first fill and copy MIB MiB with the probe library's plain kernels (ssp_fill, ssp_copy): the "
+                         "known byte counts that calibrate a profiler's FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
@@ -118,6 +121,15 @@ def main():
         b.close()
         return secs, steps, kms, launches, steps0
 
+    if args.copy_probe > 0:
+        import ctypes
+        from sinksim_amd.build import PROBE
+        probe = ctypes.CDLL(PROBE)
+        probe.ssp_copy_probe.argtypes = [ctypes.c_longlong]
+        probe.ssp_last_error.restype = ctypes.c_char_p
+        if probe.ssp_copy_probe(args.copy_probe << 20) != 0:
+            raise RuntimeError(probe.ssp_last_error().decode())
+        print(f"copy probe: {args.copy_probe} MiB filled and copied", file=sys.stderr, flush=True)
     for w in range(args.warmup):
         secs, steps, _, _, _ = one_run()
         print(f"warmup {w}: {steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)

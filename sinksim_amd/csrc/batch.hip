@@ -82,7 +82,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     B.cen[c] = P.Bt.en[iC + c]; B.carea[c] = P.Bt.area[iC + c]; B.ctOn[c] = P.Bt.tOn[iC + c];
     B.dV[c] = 0; B.q[c] = 0;
   }
-  for (int g = tid; g < P.M.ng; g += kThreads) B.merged[g] = 0;
+  for (int g = tid; g < P.M.ng; g += kThreads) {
+    B.merged[g] = 0;
+    B.gStart[g] = P.M.gStart[g]; B.gCount[g] = P.M.gCount[g]; B.gMode[g] = P.M.gMode[g];
+    B.gpx[g] = P.M.gpx[g]; B.gpy[g] = P.M.gpy[g]; B.gpz[g] = P.M.gpz[g];
+  }
+  for (int k = tid; k < P.M.nMembers; k += kThreads) B.gNodes[k] = P.M.gNodes[k];
+  for (int n = tid; n < NN; n += kThreads) {
+    B.nSegStart[n] = P.S.segStart[n]; B.nSegCount[n] = P.S.segCount[n];
+    B.nMu[n] = P.S.mu[n]; B.nVmax[n] = P.S.vmax[n]; B.nAov[n] = P.S.aov[n]; B.nAmin[n] = P.S.amin[n];
+  }
   for (int e = tid; e < evCap; e += kThreads) {
     const size_t k = static_cast<size_t>(inst) * evCap + e;
     B.evT[e] = P.Bt.evT[k]; B.evId[e] = P.Bt.evId[k]; B.evKind[e] = P.Bt.evKind[k];
@@ -283,8 +292,8 @@ struct ss_batch {
   int launches = 0;
 
   // ship
-  Dev<double> x, y, dxdy, zlo, zhi, sx, sy, sA, sa, se, mu, vmax, amin, aov;
-  Dev<float> xf, yf, dxdyf, zlof, zhif, sxf, syf, sAf, saf, sef;
+  Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record
+  Dev<float> colF, segF;                      // 8 floats per record
   Dev<int32_t> segStart, segCount;
   // scenario
   Dev<int32_t> gStart, gCount, gNodes, incStart, incCount, sea, monStart, monCount, monConn;
@@ -312,11 +321,9 @@ struct ss_batch {
   DevShip shipView() const {
     DevShip s{};
     s.ncol = ncol; s.nseg = nseg; s.nn = NN;
-    s.x = G(x.p); s.y = G(y.p); s.dxdy = G(dxdy.p); s.zlo = G(zlo.p); s.zhi = G(zhi.p);
-    s.xf = G(xf.p); s.yf = G(yf.p); s.dxdyf = G(dxdyf.p); s.zlof = G(zlof.p); s.zhif = G(zhif.p);
+    s.col = (const SSG d2v*)col.p; s.colF = (const SSG f4v*)colF.p;
+    s.seg = (const SSG d2v*)seg.p; s.segF = (const SSG f4v*)segF.p;
     s.segStart = G(segStart.p); s.segCount = G(segCount.p);
-    s.sx = G(sx.p); s.sy = G(sy.p); s.sA = G(sA.p); s.sa = G(sa.p); s.se = G(se.p);
-    s.sxf = G(sxf.p); s.syf = G(syf.p); s.sAf = G(sAf.p); s.saf = G(saf.p); s.sef = G(sef.p);
     s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
     return s;
   }
@@ -342,7 +349,7 @@ struct ss_batch {
 namespace {
 
 // LDS layout of one simulation, in doubles; byte and int arrays are packed into whole doubles.
-LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t nmark, int32_t maxGroup, int32_t evCap) {
+LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t nmark, int32_t maxGroup, int32_t evCap, int32_t nMembers) {
   LdsLayout L{};
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
@@ -357,8 +364,10 @@ LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t n
   L.frame = take(F_COUNT);
   L.scal = take(S_COUNT);
   L.over = take(nmon); L.overT = take(nmon); L.markT = take(nmark); L.evT = take(evCap);
-  L.deg = take(ints(NN)); L.evId = take(ints(evCap));
-  L.merged = take(bytes(NG)); L.cen = take(bytes(NC)); L.evKind = take(bytes(evCap));
+  L.nMu = take(NN); L.nVmax = take(NN); L.nAov = take(NN); L.nAmin = take(NN); L.gpx = take(NG); L.gpy = take(NG); L.gpz = take(NG);
+  L.deg = take(ints(NN)); L.evId = take(ints(evCap)); L.nSegStart = take(ints(NN)); L.nSegCount = take(ints(NN));
+  L.gStart = take(ints(NG)); L.gCount = take(ints(NG)); L.gNodes = take(ints(nMembers));
+  L.merged = take(bytes(NG)); L.cen = take(bytes(NC)); L.evKind = take(bytes(evCap)); L.gMode = take(bytes(NG));
   L.total = off;
   return L;
 }
@@ -511,7 +520,9 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     b.NN = S.nnode; b.NC = M.nconn; b.NG = M.ngroup; b.nmon = M.nmonitor; b.nmark = M.nmark; b.ncol = S.ncol; b.nseg = S.nseg;
     b.evCap = b.nmon + b.nmark + 1;
     for (int32_t g = 0; g < b.NG; ++g) b.maxGroup = std::max(b.maxGroup, M.gNodeCount[g]);
-    b.layout = lds_layout(b.NN, b.NC, b.NG, b.nmon, b.nmark, b.maxGroup, b.evCap);
+    size_t members = 0;  // length of the group member list the groups index into
+    for (int32_t g = 0; g < b.NG; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
+    b.layout = lds_layout(b.NN, b.NC, b.NG, b.nmon, b.nmark, b.maxGroup, b.evCap, static_cast<int32_t>(members));
     b.ldsBytes = static_cast<size_t>(b.layout.total) * sizeof(double);
     hipDeviceProp_t p{};
     hip_check(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties");
@@ -523,22 +534,31 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
 
     // ship: columns, and per segment the streams of its column
     const size_t nc = static_cast<size_t>(S.ncol), ns = static_cast<size_t>(S.nseg), nn = static_cast<size_t>(S.nnode);
-    b.x.upload(S.x, nc); b.y.upload(S.y, nc); b.dxdy.upload(S.dxdy, nc); b.zlo.upload(S.zlo, nc); b.zhi.upload(S.zhi, nc);
-    auto to_f = [](const double* v, size_t n) { std::vector<float> o(n); for (size_t i = 0; i < n; ++i) o[i] = static_cast<float>(v[i]); return o; };
-    b.xf.upload(to_f(S.x, nc)); b.yf.upload(to_f(S.y, nc)); b.dxdyf.upload(to_f(S.dxdy, nc)); b.zlof.upload(to_f(S.zlo, nc));
-    b.zhif.upload(to_f(S.zhi, nc));
-    std::vector<double> gx(ns), gy(ns), gA(ns);
-    for (size_t j = 0; j < ns; ++j) { const int32_t c = S.segCol[j]; gx[j] = S.x[c]; gy[j] = S.y[c]; gA[j] = S.dxdy[c]; }
-    b.sx.upload(gx); b.sy.upload(gy); b.sA.upload(gA); b.sa.upload(S.segA, ns); b.se.upload(S.segE, ns);
-    b.sxf.upload(to_f(gx.data(), ns)); b.syf.upload(to_f(gy.data(), ns)); b.sAf.upload(to_f(gA.data(), ns));
-    b.saf.upload(to_f(S.segA, ns)); b.sef.upload(to_f(S.segE, ns));
+    // records (x, y, bottom, top, footprint): 6 doubles (the last one 0) and 8 floats (the last three 0)
+    auto records = [](size_t n, auto&& value, std::vector<double>& d, std::vector<float>& f) {
+      d.assign(std::max<size_t>(n, 1) * 6, 0.0);
+      f.assign(std::max<size_t>(n, 1) * 8, 0.0f);
+      for (size_t i = 0; i < n; ++i)
+        for (int k = 0; k < 5; ++k) {
+          const double v = value(i, k);
+          d[6 * i + k] = v;
+          f[8 * i + k] = static_cast<float>(v);
+        }
+    };
+    std::vector<double> recD;
+    std::vector<float> recF;
+    records(nc, [&](size_t i, int k) { const double* a[5] = {S.x, S.y, S.zlo, S.zhi, S.dxdy}; return a[k][i]; }, recD, recF);
+    b.col.upload(recD); b.colF.upload(recF);
+    records(ns, [&](size_t j, int k) {
+      const size_t c = static_cast<size_t>(S.segCol[j]);
+      return k == 0 ? S.x[c] : k == 1 ? S.y[c] : k == 2 ? S.segA[j] : k == 3 ? S.segE[j] : S.dxdy[c];
+    }, recD, recF);
+    b.seg.upload(recD); b.segF.upload(recF);
     b.segStart.upload(S.segStart, nn); b.segCount.upload(S.segCount, nn);
     b.mu.upload(S.mu, nn); b.vmax.upload(S.vmax, nn); b.amin.upload(S.amin, nn); b.aov.upload(S.aov, nn);
 
     // scenario: groups, connections, incidence (model.cpp:137-167), monitor lists, marks
     const size_t ncn = static_cast<size_t>(M.nconn), ng = static_cast<size_t>(M.ngroup);
-    size_t members = 0;
-    for (size_t g = 0; g < ng; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
     b.gStart.upload(M.gNodeStart, ng); b.gCount.upload(M.gNodeCount, ng); b.gNodes.upload(M.gNodes, members); b.gMode.upload(M.gMode, ng);
     b.gpx.upload(M.gpx, ng); b.gpy.upload(M.gpy, ng); b.gpz.upload(M.gpz, ng);
     std::vector<int4> conI(std::max<size_t>(ncn, 1));
@@ -580,7 +600,7 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
 
     DevNet& d = b.net;
     d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
-    d.ng = M.ngroup; d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
+    d.ng = M.ngroup; d.nMembers = static_cast<int32_t>(members); d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
     d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
     d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p;
     d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.inc = (const SSG i4v*)b.inc.p;

@@ -1,7 +1,8 @@
 // libsinksim_probe.so: the device arithmetic the step relies on, evaluated over arrays, so that tests can hold it bit
 // for bit against the CPU: the fdlibm routines of fdlibm_dev.hpp as the step calls them, f64 division and square
 // root (correctly rounded in IEEE 754, which the CPU's SSE2 instructions are), f32 division, and the f64 -> f32
-// conversion. Not part of the batch engine's C ABI; a test hook of the same build flags.
+// conversion. Not part of the batch engine's C ABI; a test hook of the same build flags. It also holds a plain
+// device copy (ssp_copy_probe) whose known byte counts calibrate the hardware byte counters of a profiler run.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -33,6 +34,16 @@ __global__ void probe_kernel(int op, const double* __restrict__ a, const double*
   out[i] = r;
 }
 
+// dst[i] = v over n doubles, and dst[i] = src[i]: 16-byte accesses, grid-stride
+__global__ void ssp_fill(double2* __restrict__ dst, double v, long long n2) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += static_cast<long long>(gridDim.x) * blockDim.x)
+    dst[i] = make_double2(v, v);
+}
+__global__ void ssp_copy(const double2* __restrict__ src, double2* __restrict__ dst, long long n2) {
+  for (long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += static_cast<long long>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+}
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -61,6 +72,35 @@ int ssp_eval(int op, const double* a, const double* b, double* out, long long n)
   if (da) (void)hipFree(da);
   if (db) (void)hipFree(db);
   if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) {
+    g_err = hipGetErrorString(e);
+    return 1;
+  }
+  return 0;
+}
+
+// Allocates two device buffers of `bytes` (a multiple of 16), fills the first (ssp_fill: writes `bytes`), copies
+// it to the second (ssp_copy: reads and writes `bytes`), frees both. Returns 0 on success.
+int ssp_copy_probe(long long bytes) {
+  if (bytes <= 0 || bytes % 16 != 0) {
+    g_err = "ssp_copy_probe: bytes must be a positive multiple of 16";
+    return 1;
+  }
+  double2 *a = nullptr, *b = nullptr;
+  const long long n2 = bytes / 16;
+  hipError_t e = hipMalloc(&a, static_cast<size_t>(bytes));
+  if (e == hipSuccess) e = hipMalloc(&b, static_cast<size_t>(bytes));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(ssp_fill, dim3(4096), dim3(256), 0, nullptr, a, 1.0, n2);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(ssp_copy, dim3(4096), dim3(256), 0, nullptr, a, b, n2);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
   if (e != hipSuccess) {
     g_err = hipGetErrorString(e);
     return 1;

@@ -60,6 +60,8 @@ __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b 
 // four 32-bit integers or four doubles loaded as one vector (the packed per-connection and incidence records)
 typedef int32_t i4v __attribute__((ext_vector_type(4)));
 typedef double d4v __attribute__((ext_vector_type(4)));
+typedef double d2v __attribute__((ext_vector_type(2)));
+typedef float f4v __attribute__((ext_vector_type(4)));
 
 using gcd = const SSG double;
 using gcf = const SSG float;
@@ -68,23 +70,24 @@ using gcb = const SSG uint8_t;
 
 // ------------------------------------------------------------------------------------------- device views
 
-// The hull, shared by every simulation. Segment arrays are pre-gathered per segment (x, y and footprint of the
-// segment's column) so that a node pass reads five contiguous streams; the f32 copies are the same values
-// rounded once on the host, which is what the portable32 arithmetic converts them to anyway.
+// The hull, shared by every simulation, as records: per column (x, y, zlo, zhi, dxdy) and per segment the same
+// five values with the segment's own bottom and top in place of the column's (x, y and footprint pre-gathered from
+// the segment's column), as 48-byte double records and 32-byte float records. The float records hold the same
+// values rounded once on the host, which is what the portable32 arithmetic converts them to anyway.
 struct DevShip {
   int32_t ncol, nseg, nn;
-  gcd *x, *y, *dxdy, *zlo, *zhi;
-  gcf *xf, *yf, *dxdyf, *zlof, *zhif;
+  const SSG d2v* col;
+  const SSG f4v* colF;
+  const SSG d2v* seg;
+  const SSG f4v* segF;
   gci *segStart, *segCount;
-  gcd *sx, *sy, *sA, *sa, *se;
-  gcf *sxf, *syf, *sAf, *saf, *sef;
   gcd *mu, *vmax, *amin, *aov;
 };
 
 // The scenario's immutable network and constants.
 struct DevNet {
   double rho, g, sq2g, dt;
-  int32_t ng;
+  int32_t ng, nMembers;
   gci *gStart, *gCount, *gNodes;
   gcb* gMode;
   gcd *gpx, *gpy, *gpz;
@@ -142,8 +145,9 @@ struct LdsLayout {
   int32_t vol, level, cx, cy, cz, area, zmin, zmax, hEff, aEff, excess;
   int32_t tX, tY, tF, tM, tIp, tIr;
   int32_t dV, q, carea, ctOn, passes, warpHyd, frame, scal, over, overT, markT, evT;
-  int32_t deg, evId;            // int32 arrays, offsets in doubles
-  int32_t merged, cen, evKind;  // byte arrays, offsets in doubles
+  int32_t nMu, nVmax, nAov, nAmin, gpx, gpy, gpz;  // copies of the node and group tables
+  int32_t deg, evId, nSegStart, nSegCount, gStart, gCount, gNodes;  // int32 arrays, offsets in doubles
+  int32_t merged, cen, evKind, gMode;                               // byte arrays, offsets in doubles
   int32_t total;                // doubles
 };
 
@@ -167,8 +171,9 @@ struct Blk {
   SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
   SSL double *tX, *tY, *tF, *tM, *tIp, *tIr;
   SSL double *dV, *q, *carea, *ctOn, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
-  SSL int32_t *deg, *evId;
-  SSL uint8_t *merged, *cen, *evKind;
+  SSL double *nMu, *nVmax, *nAov, *nAmin, *gpx, *gpy, *gpz;
+  SSL int32_t *deg, *evId, *nSegStart, *nSegCount, *gStart, *gCount, *gNodes;
+  SSL uint8_t *merged, *cen, *evKind, *gMode;
 };
 
 __device__ inline Blk carve(SSL double* base, PC& P) {
@@ -181,6 +186,14 @@ __device__ inline Blk carve(SSL double* base, PC& P) {
   b.dV = base + L.dV; b.q = base + L.q; b.carea = base + L.carea; b.ctOn = base + L.ctOn; b.passes = base + L.passes;
   b.warpHyd = base + L.warpHyd; b.frame = base + L.frame; b.scal = base + L.scal; b.over = base + L.over; b.overT = base + L.overT;
   b.markT = base + L.markT; b.evT = base + L.evT;
+  b.nMu = base + L.nMu; b.nVmax = base + L.nVmax; b.nAov = base + L.nAov; b.nAmin = base + L.nAmin;
+  b.gpx = base + L.gpx; b.gpy = base + L.gpy; b.gpz = base + L.gpz;
+  b.nSegStart = reinterpret_cast<SSL int32_t*>(base + L.nSegStart);
+  b.nSegCount = reinterpret_cast<SSL int32_t*>(base + L.nSegCount);
+  b.gStart = reinterpret_cast<SSL int32_t*>(base + L.gStart);
+  b.gCount = reinterpret_cast<SSL int32_t*>(base + L.gCount);
+  b.gNodes = reinterpret_cast<SSL int32_t*>(base + L.gNodes);
+  b.gMode = reinterpret_cast<SSL uint8_t*>(base + L.gMode);
   b.deg = reinterpret_cast<SSL int32_t*>(base + L.deg);
   b.evId = reinterpret_cast<SSL int32_t*>(base + L.evId);
   b.merged = reinterpret_cast<SSL uint8_t*>(base + L.merged);
@@ -253,18 +266,14 @@ __device__ __forceinline__ float down(float v) {
   return __builtin_bit_cast(float, down32<OFF>(__builtin_bit_cast(uint32_t, v)));
 }
 
-// lane 0 of the logical group (wave lane 0 or 32) to every lane of the group
-__device__ __forceinline__ uint32_t lead32(uint32_t v, bool upper) {
-  const uint32_t a = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
-  const uint32_t b = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
-  return upper ? b : a;
-}
-__device__ __forceinline__ double lead(double v, bool upper) {
+// Lane 0 of each logical 32-lane group (wave lanes 0 and 32) to every lane of its group: an LDS-unit swizzle whose
+// source lane is ((l & 0) | 0) ^ 0 = 0 within each group of 32.
+__device__ __forceinline__ uint32_t lead32(uint32_t v) { return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x0000)); }
+__device__ __forceinline__ double lead(double v) {
   const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  return __builtin_bit_cast(double, (static_cast<uint64_t>(lead32(static_cast<uint32_t>(b >> 32), upper)) << 32) |
-                                        lead32(static_cast<uint32_t>(b), upper));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(lead32(static_cast<uint32_t>(b >> 32))) << 32) | lead32(static_cast<uint32_t>(b)));
 }
-__device__ __forceinline__ float lead(float v, bool upper) { return __builtin_bit_cast(float, lead32(__builtin_bit_cast(uint32_t, v), upper)); }
+__device__ __forceinline__ float lead(float v) { return __builtin_bit_cast(float, lead32(__builtin_bit_cast(uint32_t, v))); }
 
 // ------------------------------------------------------------------------------------------- partials
 
@@ -277,20 +286,20 @@ struct PP {
   T vol, dv, mx, my, mz, zmin, zmax;
 };
 
+// One level of the halving tree. Lanes l < OFF absorb lane l + OFF; the other lanes are never read again, so they
+// add too (their sums are discarded) and no lane mask is needed for the sums.
 template <int OFF, class T>
-__device__ __forceinline__ void tree_level(HP<T>& p, int lane) {
+__device__ __forceinline__ void tree_level(HP<T>& p) {
   const T V = down<OFF>(p.V), Mx = down<OFF>(p.Mx), My = down<OFF>(p.My), Mz = down<OFF>(p.Mz), top = down<OFF>(p.top);
-  if (lane < OFF) {
-    p.V += V; p.Mx += Mx; p.My += My; p.Mz += Mz;
-    if (top > p.top) p.top = top;
-  }
+  p.V += V; p.Mx += Mx; p.My += My; p.Mz += Mz;
+  if (top > p.top) p.top = top;
 }
-template <int OFF, class T>
-__device__ __forceinline__ void tree_level(PP<T>& p, int lane) {
+template <int OFF, bool Ext, class T>
+__device__ __forceinline__ void tree_level(PP<T>& p) {
   const T vol = down<OFF>(p.vol), dv = down<OFF>(p.dv), mx = down<OFF>(p.mx), my = down<OFF>(p.my), mz = down<OFF>(p.mz);
-  const T zmin = down<OFF>(p.zmin), zmax = down<OFF>(p.zmax);
-  if (lane < OFF) {
-    p.vol += vol; p.dv += dv; p.mx += mx; p.my += my; p.mz += mz;
+  p.vol += vol; p.dv += dv; p.mx += mx; p.my += my; p.mz += mz;
+  if constexpr (Ext) {
+    const T zmin = down<OFF>(p.zmin), zmax = down<OFF>(p.zmax);
     if (zmin < p.zmin) p.zmin = zmin;
     if (zmax > p.zmax) p.zmax = zmax;
   }
@@ -298,16 +307,15 @@ __device__ __forceinline__ void tree_level(PP<T>& p, int lane) {
 
 // The halving tree of a logical 32-lane group (reduce.hpp:102-107): l absorbs l + 16, 8, 4, 2, 1; the total is in
 // lane 0 of the group.
-template <class P_>
-__device__ __forceinline__ void tree32(P_& p, int lane) {
-  tree_level<16>(p, lane);
-  tree_level<8>(p, lane);
-  tree_level<4>(p, lane);
-  tree_level<2>(p, lane);
-  tree_level<1>(p, lane);
+template <class T>
+__device__ __forceinline__ void tree32(HP<T>& p) {
+  tree_level<16>(p); tree_level<8>(p); tree_level<4>(p); tree_level<2>(p); tree_level<1>(p);
+}
+template <bool Ext, class T>
+__device__ __forceinline__ void tree32(PP<T>& p) {
+  tree_level<16, Ext>(p); tree_level<8, Ext>(p); tree_level<4, Ext>(p); tree_level<2, Ext>(p); tree_level<1, Ext>(p);
 }
 
-// one column into a buoyancy partial (reduce.hpp:67-77 / 159-170)
 struct Fr64 {
   double R20, R21, R22, tz, iR;
 };
@@ -321,19 +329,25 @@ __device__ __forceinline__ Fr32 fr32(const SSL double* fr) {
   return Fr32{f[0], f[1], f[2], f[3], f[4]};
 }
 
-// A column's or a segment's geometry, fetched ahead of its evaluation so that a lane's loads of several
-// columns or segments are in flight together.
+// A column's or a segment's geometry: x, y, bottom, top and footprint (for a segment: those of its column, with
+// the segment's own bottom and top). Records are 48 bytes (double) or 32 bytes (float), addressed by a 32-bit
+// byte offset from the table's base.
 template <class T>
 struct Geo {
   T x, y, lo, hi, A;
 };
-__device__ __forceinline__ Geo<double> column_geo(PC& P, const double*, int i) { return Geo<double>{P.S.x[i], P.S.y[i], P.S.zlo[i], P.S.zhi[i], P.S.dxdy[i]}; }
-__device__ __forceinline__ Geo<float> column_geo(PC& P, const float*, int i) { return Geo<float>{P.S.xf[i], P.S.yf[i], P.S.zlof[i], P.S.zhif[i], P.S.dxdyf[i]}; }
-__device__ __forceinline__ Geo<double> segment_geo(PC& P, const double*, int j) { return Geo<double>{P.S.sx[j], P.S.sy[j], P.S.sa[j], P.S.se[j], P.S.sA[j]}; }
-__device__ __forceinline__ Geo<float> segment_geo(PC& P, const float*, int j) { return Geo<float>{P.S.sxf[j], P.S.syf[j], P.S.saf[j], P.S.sef[j], P.S.sAf[j]}; }
+__device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) {
+  const SSG d2v* r = (const SSG d2v*)((const SSG char*)table + static_cast<uint32_t>(i) * 48u);
+  const d2v a = r[0], b = r[1], c = r[2];
+  return Geo<double>{a.x, a.y, b.x, b.y, c.x};
+}
+__device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) {
+  const SSG f4v* r = (const SSG f4v*)((const SSG char*)table + static_cast<uint32_t>(i) * 32u);
+  const f4v a = r[0], b = r[1];
+  return Geo<float>{a.x, a.y, a.z, a.w, b.x};
+}
 
-constexpr int kFetch = 4;  // columns or segments a lane fetches before evaluating them in order
-
+// one column into a buoyancy partial (reduce.hpp:67-77 / 159-170)
 __device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, HP<double>& p) {
   const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
@@ -357,13 +371,18 @@ __device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, HP<fl
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
 }
 
-// one segment into a node-pass partial (reduce.hpp:80-99 / 172-192)
+// One segment into a node-pass partial (reduce.hpp:80-99 / 172-192). The world extent of the node (zmin, zmax)
+// depends on the pose only, not on the level: it is accumulated (Ext) in the first pass of a node in a step and
+// reused by the later ones, which the reference recomputes to the same values.
+template <bool Ext>
 __device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<double>& g, PP<double>& p) {
   const double x = g.x, y = g.y, a = g.lo, b = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
-  const double wa = base + F.R22 * a, wb = base + F.R22 * b;
-  if (wa < p.zmin) p.zmin = wa;
-  if (wb > p.zmax) p.zmax = wb;
+  if constexpr (Ext) {
+    const double wa = base + F.R22 * a, wb = base + F.R22 * b;
+    if (wa < p.zmin) p.zmin = wa;
+    if (wb > p.zmax) p.zmax = wb;
+  }
   const double zp = (h - base) * F.iR;
   if (zp <= a) return;
   const double A = g.A;
@@ -377,12 +396,15 @@ __device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<doubl
   const double v = A * L;
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
 }
+template <bool Ext>
 __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>& g, PP<float>& p) {
   const float x = g.x, y = g.y, a = g.lo, b = g.hi;
   const float base = f.R20 * x + f.R21 * y + f.tz;
-  const float wa = base + f.R22 * a, wb = base + f.R22 * b;
-  if (wa < p.zmin) p.zmin = wa;
-  if (wb > p.zmax) p.zmax = wb;
+  if constexpr (Ext) {
+    const float wa = base + f.R22 * a, wb = base + f.R22 * b;
+    if (wa < p.zmin) p.zmin = wa;
+    if (wb > p.zmax) p.zmax = wb;
+  }
   const float zp = (h - base) * f.iR;
   if (zp <= a) return;
   const float A = g.A;
@@ -397,62 +419,67 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5f * L);
 }
 
+// A lane's partial over the records s0 + lane, s0 + lane + step, ... below s1, in order, each record's load
+// issued while the previous record is evaluated.
+template <class T, class Tab, class Fn>
+__device__ __forceinline__ void stream(const Tab* table, int first, int end, int step, Fn&& eval) {
+  int j = first;
+  if (j >= end) return;
+  Geo<T> g = geo(table, j);
+  while (true) {
+    const int jn = j + step;
+    const bool more = jn < end;
+    const Geo<T> gn = geo(table, more ? jn : j);
+    eval(g);
+    if (!more) break;
+    g = gn;
+    j = jn;
+  }
+}
+
 struct Pass {
   double vol, dv, mx, my, mz, zmin, zmax;
 };
 
-// The node pass done by one logical 32-lane group; the result is in every lane of the group.
-// (hydro.hpp:306-316 / 338-349; finish_pass hydro.hpp:295-303)
-template <bool Mixed>
-struct UnitPass {
-  PC& P;
-  const SSL double* fr;
-  int lane;
-  bool upper;
-  __device__ __forceinline__ Pass operator()(int n, double h) const {
-    const int s0 = P.S.segStart[n], s1 = s0 + P.S.segCount[n];
-    double vol, dv, mx, my, mz, zmin, zmax;
-    if constexpr (Mixed) {
-      const Fr32 f = fr32(fr);
-      const float hf = static_cast<float>(h);
-      PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
-      for (int j0 = s0 + lane; j0 < s1; j0 += kFetch * kLanes) {
-        Geo<float> g[kFetch];
-#pragma unroll
-        for (int u = 0; u < kFetch; ++u) g[u] = segment_geo(P, static_cast<const float*>(nullptr), j0 + u * kLanes < s1 ? j0 + u * kLanes : j0);
-#pragma unroll
-        for (int u = 0; u < kFetch; ++u)
-          if (j0 + u * kLanes < s1) segment(f, hf, g[u], p);
-      }
-      tree32(p, lane);
-      vol = static_cast<double>(lead(p.vol, upper)); dv = static_cast<double>(lead(p.dv, upper));
-      mx = static_cast<double>(lead(p.mx, upper)); my = static_cast<double>(lead(p.my, upper)); mz = static_cast<double>(lead(p.mz, upper));
-      zmin = static_cast<double>(lead(p.zmin, upper)); zmax = static_cast<double>(lead(p.zmax, upper));
-    } else {
-      const Fr64 F = fr64(fr);
-      PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
-      for (int j0 = s0 + lane; j0 < s1; j0 += kFetch * kLanes) {
-        Geo<double> g[kFetch];
-#pragma unroll
-        for (int u = 0; u < kFetch; ++u) g[u] = segment_geo(P, static_cast<const double*>(nullptr), j0 + u * kLanes < s1 ? j0 + u * kLanes : j0);
-#pragma unroll
-        for (int u = 0; u < kFetch; ++u)
-          if (j0 + u * kLanes < s1) segment(F, h, g[u], p);
-      }
-      tree32(p, lane);
-      vol = lead(p.vol, upper); dv = lead(p.dv, upper); mx = lead(p.mx, upper); my = lead(p.my, upper); mz = lead(p.mz, upper);
-      zmin = lead(p.zmin, upper); zmax = lead(p.zmax, upper);
-    }
-    const double mu = P.S.mu[n];
-    const double iR = fr[F_IR];
-    Pass r;
-    r.vol = vol * mu;
-    r.dv = dv * mu * iR;
-    r.mx = mx * mu; r.my = my * mu; r.mz = mz * mu;
-    r.zmin = zmin; r.zmax = zmax;
-    return r;
+// The node pass of node n at level h by one logical 32-lane group (hydro.hpp:306-316 / 338-349; finish_pass
+// hydro.hpp:295-303); the result is in every lane of the group. Both groups of a wave call it together, each with
+// its own request; n < 0 is no request (an empty pass whose result is not used). zmin and zmax are computed only
+// with Ext.
+template <bool Mixed, bool Ext>
+__device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, double h) {
+  const SSL double* fr = B.frame;
+  int s0 = 0, s1 = 0;
+  if (n >= 0) {
+    s0 = B.nSegStart[n];
+    s1 = s0 + B.nSegCount[n];
   }
-};
+  double vol, dv, mx, my, mz, zmin = 0, zmax = 0;
+  if constexpr (Mixed) {
+    const Fr32 f = fr32(fr);
+    const float hf = static_cast<float>(h);
+    PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
+    stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g) { segment<Ext>(f, hf, g, p); });
+    tree32<Ext>(p);
+    vol = static_cast<double>(lead(p.vol)); dv = static_cast<double>(lead(p.dv));
+    mx = static_cast<double>(lead(p.mx)); my = static_cast<double>(lead(p.my)); mz = static_cast<double>(lead(p.mz));
+    if constexpr (Ext) { zmin = static_cast<double>(lead(p.zmin)); zmax = static_cast<double>(lead(p.zmax)); }
+  } else {
+    const Fr64 F = fr64(fr);
+    PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
+    stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g) { segment<Ext>(F, h, g, p); });
+    tree32<Ext>(p);
+    vol = lead(p.vol); dv = lead(p.dv); mx = lead(p.mx); my = lead(p.my); mz = lead(p.mz);
+    if constexpr (Ext) { zmin = lead(p.zmin); zmax = lead(p.zmax); }
+  }
+  const double mu = B.nMu[n >= 0 ? n : 0];
+  const double iR = fr[F_IR];
+  Pass r;
+  r.vol = vol * mu;
+  r.dv = dv * mu * iR;
+  r.mx = mx * mu; r.my = my * mu; r.mz = mz * mu;
+  r.zmin = zmin; r.zmax = zmax;
+  return r;
+}
 
 // ------------------------------------------------------------------------------------------- (b) levels
 
@@ -460,6 +487,8 @@ __device__ __forceinline__ void put_centroid(const Blk& B, int n, const Pass& p)
   if (p.vol > kCentroidMinVolume) { B.cx[n] = p.mx / p.vol; B.cy[n] = p.my / p.vol; B.cz[n] = p.mz / p.vol; }
 }
 
+// Pp (a unit's pass scratch for the members of a merged group) is written by every lane of the unit (each lane
+// holds the same pass result) and read back by each lane after its own write.
 __device__ __forceinline__ void put_pass(SSL double* Pp, int k, const Pass& p) {
   SSL double* q = Pp + 7 * k;
   q[0] = p.vol; q[1] = p.dv; q[2] = p.mx; q[3] = p.my; q[4] = p.mz; q[5] = p.zmin; q[6] = p.zmax;
@@ -469,194 +498,308 @@ __device__ __forceinline__ Pass get_pass(const SSL double* Pp, int k) {
   return Pass{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
 }
 
-// levels.hpp:13-49 with full = false (the step never asks for the 40-iteration solve)
-template <class PassFn>
-__device__ void solve_single(PC& P, const PassFn& pass, int n, const Blk& B, bool lead_) {
-  const double V = B.vol[n], Vmax = P.S.vmax[n];
-  double h = B.level[n];
-  Pass p = pass(n, h);
-  const double lo = p.zmin, hi = p.zmax;
-  if (lead_) { B.zmin[n] = lo; B.zmax[n] = hi; }
-  if (V <= kEmptyVolume) {
-    if (lead_) { B.level[n] = lo; B.area[n] = 0; }
-    return;
-  }
-  if (V >= Vmax * kFullFraction) {
-    p = pass(n, hi);
-    if (lead_) {
-      B.level[n] = hi + jmax(0.0, V - Vmax) / P.S.aov[n];
-      B.area[n] = P.S.aov[n];
-      put_centroid(B, n, p);
-    }
-    return;
-  }
-  if (h < lo || h > hi) {
-    h = jmin(hi, jmax(lo, h));
-    p = pass(n, h);
-  }
-  const double tol = jmax(1e-3 * Vmax, 0.5);
-  double a = lo, b = hi;
-  for (int it = 0; it < kNewtonFast; ++it) {
-    const double err = p.vol - V;
-    if (__builtin_fabs(err) < tol) break;
-    if (err > 0) b = h; else a = h;
-    double hn = p.dv > kSlopeMin ? h - err / p.dv : 0.5 * (a + b);
-    if (!(hn > a && hn < b)) hn = 0.5 * (a + b);
-    h = hn;
-    p = pass(n, h);
-  }
-  if (lead_) {
-    B.level[n] = h; B.area[n] = p.dv;
-    put_centroid(B, n, p);
-  }
-}
+// The level solve of one free-surface group as a sequence of node passes (levels.hpp:13-194 with full = false),
+// restated as a state machine so that the two logical groups of a wave issue each pass together: a state names
+// the pass it waits for, and advance() takes the pass result to the next state. Every lane of a unit holds the
+// same state.
+enum : int {
+  L_START = 0,  // begin the next group of this unit
+  L_DONE,
+  L_S_FIRST,    // single node: the pass at the node's level (with its extent)
+  L_S_FULL,     // single node: the pass at the top of a full node
+  L_S_CLAMP,    // single node: the pass at the level clamped into the extent
+  L_S_NEWTON,   // single node: a Newton pass
+  L_G_FIRST,    // merged group: member k's pass at the common level (with its extent)
+  L_G_FULL,     // merged group: member k's pass at the top of a full group
+  L_G_CLAMP,    // merged group: member k's pass at the clamped level
+  L_G_NEWTON,   // merged group: member k's Newton pass
+};
 
-// levels.hpp:54-162 with full = false. Pp is this unit's pass scratch in LDS (maxGroup entries).
-template <class PassFn>
-__device__ void solve_group(PC& P, const PassFn& pass, gci* members, int N, const Blk& B, SSL double* Pp, bool lead_) {
-  double V = 0, Vmax = 0;
-  for (int k = 0; k < N; ++k) { V += B.vol[members[k]]; Vmax += P.S.vmax[members[k]]; }
-  bool allWet = true;
-  int firstWet = -1;
-  for (int k = 0; k < N; ++k) {
-    if (B.vol[members[k]] > 0) { if (firstWet < 0) firstWet = k; }
-    else allWet = false;
+struct LevelSolve {
+  int st, g, N, base, k, it;
+  double V, Vmax, h, a, b, tol, lo, hi, dh;
+};
+
+struct Solver {
+  PC& P;
+  const Blk& B;
+  SSL double* Pp;
+  bool lead;
+
+  __device__ __forceinline__ int member(const LevelSolve& S, int k) const { return B.gNodes[S.base + k]; }
+
+  __device__ __forceinline__ void start_single(LevelSolve& S) const {
+    const int n = member(S, S.k);
+    S.V = B.vol[n]; S.Vmax = B.nVmax[n]; S.h = B.level[n];
+    S.st = L_S_FIRST;
   }
-  double h;
-  if (allWet) {
-    double s = 0;
-    for (int k = 0; k < N; ++k) s += B.level[members[k]];
-    h = s / static_cast<double>(N);
-  } else {
-    h = B.level[members[firstWet >= 0 ? firstWet : N - 1]];
+  __device__ __forceinline__ void next_group(LevelSolve& S) const { S.g += kUnits; S.st = L_START; }
+  __device__ __forceinline__ void next_single(LevelSolve& S) const {
+    if (++S.k < S.N) start_single(S);
+    else next_group(S);
   }
-  double lo = 0, hi = 0;
-  // Pp is written by every lane of the unit (each lane holds the same pass result) and read back by each lane
-  // after its own write, so no lane depends on another lane's LDS store between barriers.
-  for (int k = 0; k < N; ++k) {
-    const Pass p = pass(members[k], h);
-    put_pass(Pp, k, p);
-    if (lead_) { B.zmin[members[k]] = p.zmin; B.zmax[members[k]] = p.zmax; }
-    if (k == 0) { lo = p.zmin; hi = p.zmax; }
-    else { if (p.zmin < lo) lo = p.zmin; if (p.zmax > hi) hi = p.zmax; }
-  }
-  if (V <= kEmptyVolume) {
-    if (lead_) {
-      for (int k = 0; k < N; ++k) { const int n = members[k]; B.vol[n] = 0; B.level[n] = Pp[7 * k + 5]; B.area[n] = 0; }
+
+  // levels.hpp:166-188: the merge decision of group g, then the first state of its solve
+  __device__ void begin(LevelSolve& S) const {
+    const int g = S.g;
+    S.N = B.gCount[g];
+    S.base = B.gStart[g];
+    const int N = S.N;
+    const uint8_t mode = B.gMode[g];
+    bool merged = false;
+    if (N >= 2) {
+      if (mode == 1) {
+        merged = true;
+      } else if (mode == 2) {
+        const double zMw = world_z(B.frame, B.gpx[g], B.gpy[g], B.gpz[g]);
+        merged = true;
+        for (int k = 0; k < N; ++k) {
+          const int n = member(S, k);
+          if (!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + P.M.gMargin)) { merged = false; break; }
+        }
+      }
     }
-    return;
-  }
-  if (V >= Vmax * kFullFraction) {
-    const double over = jmax(0.0, V - Vmax);
-    double aSum = 0;
-    for (int k = 0; k < N; ++k) aSum += P.S.aov[members[k]];
-    const double dh = over / aSum;
-    double assigned = 0;
-    for (int k = 0; k < N - 1; ++k) {
-      const double v = P.S.vmax[members[k]] + dh * P.S.aov[members[k]];
-      if (lead_) B.vol[members[k]] = v;
-      assigned += v;
+    if (lead) B.merged[g] = merged ? 1 : 0;
+    S.k = 0;
+    if (!merged) {
+      if (N > 0) start_single(S);
+      else next_group(S);
+      return;
     }
-    if (lead_) B.vol[members[N - 1]] = V - assigned;
+    // levels.hpp:54-80: the common starting level
+    double V = 0, Vmax = 0;
+    for (int k = 0; k < N; ++k) { V += B.vol[member(S, k)]; Vmax += B.nVmax[member(S, k)]; }
+    bool allWet = true;
+    int firstWet = -1;
     for (int k = 0; k < N; ++k) {
-      const int n = members[k];
-      const Pass p = pass(n, hi);
-      if (lead_) {
-        B.level[n] = hi + dh; B.area[n] = P.S.aov[n];
+      if (B.vol[member(S, k)] > 0) { if (firstWet < 0) firstWet = k; }
+      else allWet = false;
+    }
+    double h;
+    if (allWet) {
+      double s = 0;
+      for (int k = 0; k < N; ++k) s += B.level[member(S, k)];
+      h = s / static_cast<double>(N);
+    } else {
+      h = B.level[member(S, firstWet >= 0 ? firstWet : N - 1)];
+    }
+    S.V = V; S.Vmax = Vmax; S.h = h;
+    S.st = L_G_FIRST;
+  }
+
+  // Moves L_START on to the next state that waits for a pass, or to L_DONE.
+  __device__ __forceinline__ void resolve(LevelSolve& S, int NG) const {
+    while (S.st == L_START) {
+      if (S.g >= NG) { S.st = L_DONE; break; }
+      begin(S);
+    }
+  }
+
+  // levels.hpp:33-48: the Newton iteration of a single node after a pass
+  __device__ void newton_single(LevelSolve& S, const Pass& p) const {
+    const int n = member(S, S.k);
+    const double err = p.vol - S.V;
+    if (S.it >= kNewtonFast || __builtin_fabs(err) < S.tol) {
+      if (lead) { B.level[n] = S.h; B.area[n] = p.dv; put_centroid(B, n, p); }
+      next_single(S);
+      return;
+    }
+    if (err > 0) S.b = S.h; else S.a = S.h;
+    double hn = p.dv > kSlopeMin ? S.h - err / p.dv : 0.5 * (S.a + S.b);
+    if (!(hn > S.a && hn < S.b)) hn = 0.5 * (S.a + S.b);
+    S.h = hn;
+    ++S.it;
+    S.st = L_S_NEWTON;
+  }
+
+  // levels.hpp:125-161: the water of a merged group shared in proportion to the geometry at the final level
+  __device__ void share(LevelSolve& S) const {
+    const int N = S.N;
+    const double V = S.V;
+    if (lead) {
+      double tot = 0;
+      for (int k = 0; k < N; ++k) tot += Pp[7 * k + 0];
+      double assigned = 0;
+      for (int k = 0; k < N - 1; ++k) {
+        double v = tot > 0 ? V * Pp[7 * k + 0] / tot : V / static_cast<double>(N);
+        if (v > B.nVmax[member(S, k)]) v = B.nVmax[member(S, k)];
+        B.vol[member(S, k)] = v;
+        assigned += v;
+      }
+      const double vmaxLast = B.nVmax[member(S, N - 1)];
+      if (V - assigned > vmaxLast) {
+        const double need = V - vmaxLast;
+        double capped = 0;
+        for (int k = N - 2; k >= 0; --k) {
+          double prior = 0;
+          for (int j = 0; j < k; ++j) prior += B.vol[member(S, j)];
+          const double v = (need - capped) - prior;
+          if (v > B.nVmax[member(S, k)] && k > 0) {
+            B.vol[member(S, k)] = B.nVmax[member(S, k)];
+            capped += B.nVmax[member(S, k)];
+            continue;
+          }
+          B.vol[member(S, k)] = v;
+          break;
+        }
+        assigned = 0;
+        for (int k = 0; k < N - 1; ++k) assigned += B.vol[member(S, k)];
+      }
+      B.vol[member(S, N - 1)] = V - assigned;
+      for (int k = 0; k < N; ++k) {
+        const int n = member(S, k);
+        const Pass p = get_pass(Pp, k);
+        B.level[n] = S.h; B.area[n] = p.dv;
         put_centroid(B, n, p);
       }
     }
-    return;
+    next_group(S);
   }
-  if (!(h >= lo && h <= hi)) {
-    h = jmin(hi, jmax(lo, h));
-    for (int k = 0; k < N; ++k) {
-      const Pass p = pass(members[k], h);
-      put_pass(Pp, k, p);
-    }
-  }
-  const double tol = jmax(1e-3 * Vmax, 0.5);
-  double a = lo, b = hi;
-  for (int it = 0; it < kNewtonFast; ++it) {
+
+  // levels.hpp:102-123: the Newton iteration of a merged group after a round of member passes
+  __device__ void newton_group(LevelSolve& S) const {
+    const int N = S.N;
     double sum = 0;
     for (int k = 0; k < N; ++k) sum += Pp[7 * k + 0];
-    const double err = sum - V;
-    if (__builtin_fabs(err) < tol) break;
-    if (err > 0) b = h; else a = h;
+    const double err = sum - S.V;
+    if (S.it >= kNewtonFast || __builtin_fabs(err) < S.tol) { share(S); return; }
+    if (err > 0) S.b = S.h; else S.a = S.h;
     double dv = 0;
     for (int k = 0; k < N; ++k) dv += Pp[7 * k + 1];
-    double hn = dv > kSlopeMin ? h - err / dv : 0.5 * (a + b);
-    if (!(hn > a && hn < b)) hn = 0.5 * (a + b);
-    h = hn;
-    for (int k = 0; k < N; ++k) {
-      const Pass p = pass(members[k], h);
-      put_pass(Pp, k, p);
-    }
+    double hn = dv > kSlopeMin ? S.h - err / dv : 0.5 * (S.a + S.b);
+    if (!(hn > S.a && hn < S.b)) hn = 0.5 * (S.a + S.b);
+    S.h = hn;
+    ++S.it;
+    S.k = 0;
+    S.st = L_G_NEWTON;
   }
-  // share the water in proportion to the geometry at this level (levels.hpp:125-161)
-  if (!lead_) return;
-  double tot = 0;
-  for (int k = 0; k < N; ++k) tot += Pp[7 * k + 0];
-  double assigned = 0;
-  for (int k = 0; k < N - 1; ++k) {
-    double v = tot > 0 ? V * Pp[7 * k + 0] / tot : V / static_cast<double>(N);
-    if (v > P.S.vmax[members[k]]) v = P.S.vmax[members[k]];
-    B.vol[members[k]] = v;
-    assigned += v;
-  }
-  const double vmaxLast = P.S.vmax[members[N - 1]];
-  if (V - assigned > vmaxLast) {
-    const double need = V - vmaxLast;
-    double capped = 0;
-    for (int k = N - 2; k >= 0; --k) {
-      double prior = 0;
-      for (int j = 0; j < k; ++j) prior += B.vol[members[j]];
-      const double v = (need - capped) - prior;
-      if (v > P.S.vmax[members[k]] && k > 0) {
-        B.vol[members[k]] = P.S.vmax[members[k]];
-        capped += P.S.vmax[members[k]];
-        continue;
-      }
-      B.vol[members[k]] = v;
-      break;
-    }
-    assigned = 0;
-    for (int k = 0; k < N - 1; ++k) assigned += B.vol[members[k]];
-  }
-  B.vol[members[N - 1]] = V - assigned;
-  for (int k = 0; k < N; ++k) {
-    const int n = members[k];
-    const Pass p = get_pass(Pp, k);
-    B.level[n] = h; B.area[n] = p.dv;
-    put_centroid(B, n, p);
-  }
-}
 
-// levels.hpp:166-188: the merge decision, then the solve, for group g
-template <class PassFn>
-__device__ void solve_group_entry(PC& P, const PassFn& pass, int g, const Blk& B, SSL double* Pp, bool lead_) {
-  const int N = P.M.gCount[g];
-  gci* members = P.M.gNodes + P.M.gStart[g];
-  const uint8_t mode = P.M.gMode[g];
-  bool merged = false;
-  if (N >= 2) {
-    if (mode == 1) {
-      merged = true;
-    } else if (mode == 2) {
-      const double zMw = world_z(B.frame, P.M.gpx[g], P.M.gpy[g], P.M.gpz[g]);
-      merged = true;
-      for (int k = 0; k < N; ++k) {
-        const int n = members[k];
-        if (!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + P.M.gMargin)) { merged = false; break; }
+  // The request of the current state: the node and level of the pass it waits for, and whether the pass must
+  // also give the node's extent.
+  __device__ __forceinline__ void request(const LevelSolve& S, int& n, double& h, bool& ext) const {
+    n = -1; h = 0; ext = false;
+    if (S.st == L_DONE) return;
+    n = member(S, S.k);
+    h = (S.st == L_S_FULL || S.st == L_G_FULL) ? S.hi : S.h;
+    ext = S.st == L_S_FIRST || S.st == L_G_FIRST;
+  }
+
+  __device__ void advance(LevelSolve& S, const Pass& p) const {
+    const int n = member(S, S.k);
+    switch (S.st) {
+      case L_S_FIRST: {  // levels.hpp:13-32
+        S.lo = p.zmin; S.hi = p.zmax;
+        if (lead) { B.zmin[n] = S.lo; B.zmax[n] = S.hi; }
+        if (S.V <= kEmptyVolume) {
+          if (lead) { B.level[n] = S.lo; B.area[n] = 0; }
+          next_single(S);
+          return;
+        }
+        if (S.V >= S.Vmax * kFullFraction) { S.st = L_S_FULL; return; }
+        S.tol = jmax(1e-3 * S.Vmax, 0.5);
+        S.a = S.lo; S.b = S.hi; S.it = 0;
+        if (S.h < S.lo || S.h > S.hi) {
+          S.h = jmin(S.hi, jmax(S.lo, S.h));
+          S.st = L_S_CLAMP;
+          return;
+        }
+        newton_single(S, p);
+        return;
       }
+      case L_S_FULL:
+        if (lead) {
+          B.level[n] = S.hi + jmax(0.0, S.V - S.Vmax) / B.nAov[n];
+          B.area[n] = B.nAov[n];
+          put_centroid(B, n, p);
+        }
+        next_single(S);
+        return;
+      case L_S_CLAMP:
+      case L_S_NEWTON:
+        newton_single(S, p);
+        return;
+      case L_G_FIRST: {  // levels.hpp:81-99
+        put_pass(Pp, S.k, p);
+        if (lead) { B.zmin[n] = p.zmin; B.zmax[n] = p.zmax; }
+        if (S.k == 0) { S.lo = p.zmin; S.hi = p.zmax; }
+        else { if (p.zmin < S.lo) S.lo = p.zmin; if (p.zmax > S.hi) S.hi = p.zmax; }
+        if (++S.k < S.N) return;
+        const int N = S.N;
+        if (S.V <= kEmptyVolume) {
+          if (lead) {
+            for (int k = 0; k < N; ++k) { const int m = member(S, k); B.vol[m] = 0; B.level[m] = Pp[7 * k + 5]; B.area[m] = 0; }
+          }
+          next_group(S);
+          return;
+        }
+        if (S.V >= S.Vmax * kFullFraction) {
+          const double over = jmax(0.0, S.V - S.Vmax);
+          double aSum = 0;
+          for (int k = 0; k < N; ++k) aSum += B.nAov[member(S, k)];
+          S.dh = over / aSum;
+          double assigned = 0;
+          for (int k = 0; k < N - 1; ++k) {
+            const double v = B.nVmax[member(S, k)] + S.dh * B.nAov[member(S, k)];
+            if (lead) B.vol[member(S, k)] = v;
+            assigned += v;
+          }
+          if (lead) B.vol[member(S, N - 1)] = S.V - assigned;
+          S.k = 0;
+          S.st = L_G_FULL;
+          return;
+        }
+        S.tol = jmax(1e-3 * S.Vmax, 0.5);
+        S.a = S.lo; S.b = S.hi; S.it = 0;
+        if (!(S.h >= S.lo && S.h <= S.hi)) {
+          S.h = jmin(S.hi, jmax(S.lo, S.h));
+          S.k = 0;
+          S.st = L_G_CLAMP;
+          return;
+        }
+        newton_group(S);
+        return;
+      }
+      case L_G_FULL:
+        if (lead) {
+          B.level[n] = S.hi + S.dh; B.area[n] = B.nAov[n];
+          put_centroid(B, n, p);
+        }
+        if (++S.k == S.N) next_group(S);
+        return;
+      default:  // L_G_CLAMP, L_G_NEWTON
+        put_pass(Pp, S.k, p);
+        if (++S.k < S.N) return;
+        newton_group(S);
+        return;
     }
   }
-  if (lead_) B.merged[g] = merged ? 1 : 0;
-  if (merged) solve_group(P, pass, members, N, B, Pp, lead_);
-  else for (int k = 0; k < N; ++k) solve_single(P, pass, members[k], B, lead_);
+};
+
+// The level solve of every group: logical group `unit` takes groups unit, unit + 8, ...; the two groups of a wave
+// issue their passes together, round after round, until both are done.
+template <bool Mixed>
+__device__ void solve_levels(PC& P, const Blk& B, int tid) {
+  const int lane = tid & (kLanes - 1), unit = tid / kLanes;
+  const int NG = P.M.ng;
+  const Solver sv{P, B, B.passes + 7 * P.M.maxGroup * unit, lane == 0};
+  LevelSolve S{};
+  S.st = L_START;
+  S.g = unit;
+  while (true) {
+    sv.resolve(S, NG);
+    int n;
+    double h;
+    bool ext;
+    sv.request(S, n, h, ext);
+    if (__ballot(n >= 0) == 0) break;
+    const Pass p = __ballot(ext) != 0 ? node_pass<Mixed, true>(P, B, lane, n, h) : node_pass<Mixed, false>(P, B, lane, n, h);
+    if (n >= 0) sv.advance(S, p);
+  }
 }
 
 // ------------------------------------------------------------------------------------------- (c) network
+
+constexpr int kFetch = 4;  // incidence records a lane fetches before using them in order
 
 __device__ __forceinline__ double head_of(const Blk& B, int n) { return B.vol[n] > kEmptyVolume ? B.level[n] : kNoHead; }
 
@@ -666,7 +809,7 @@ __device__ __forceinline__ double head_of(const Blk& B, int n) { return B.vol[n]
 __device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
   const double v = B.vol[n];
   B.hEff[n] = v > kEmptyVolume ? B.level[n] : kNoHead;
-  B.aEff[n] = v >= P.S.vmax[n] ? P.S.aov[n] : jmax(B.area[n], P.S.amin[n]);
+  B.aEff[n] = v >= B.nVmax[n] ? B.nAov[n] : jmax(B.area[n], B.nAmin[n]);
   const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   int d = 0;
   for (int k0 = s; k0 < e; k0 += kFetch) {
@@ -745,16 +888,20 @@ __device__ inline void node_update(PC& P, const Blk& B, int n) {
   }
   double v = B.vol[n] + acc;
   if (v < 0) v = 0;
-  const double vm = P.S.vmax[n];
+  const double vm = B.nVmax[n];
   double excess = 0;
   if (v > vm) {
     const double head = jmax(0.0, -B.zmax[n]);
-    const double cap = vm + P.S.aov[n] * head;
+    const double cap = vm + B.nAov[n] * head;
     if (v > cap) { excess = v - cap; v = cap; }
   }
-  B.excess[n] = excess;
+  // The excess is subtracted from the sea inflow in node order (flows.hpp:310-314) and the loads take every node
+  // with !(vol <= 0) (body.hpp:21-34). The tail adds these terms in node order without tests: a skipped node
+  // contributes -0.0 (x + (-0.0) == x for every x) and a subtracted term is stored negated (x - y == x + (-y)).
+  B.excess[n] = -excess;
   B.vol[n] = v;
-  if (v > 0) {
+  double tx = -0.0, ty = -0.0, tf = -0.0, tm = -0.0, tip = -0.0, tir = -0.0;
+  if (!(v <= 0)) {
     const SSL double* fr = B.frame;
     const double W = P.M.rho * P.M.g;
     const double x = B.cx[n], y = B.cy[n], z = B.cz[n];
@@ -763,13 +910,14 @@ __device__ inline void node_update(PC& P, const Blk& B, int n) {
     const double wz = fr[F_R20] * x + fr[F_R21] * y + fr[F_R22] * z + fr[F_TZ] - B.scal[S_ZO];
     const double f = -W * v;
     const double m = P.M.rho * v;
-    B.tX[n] = wy * f;
-    B.tY[n] = wx * f;
-    B.tF[n] = f;
-    B.tM[n] = m;
-    B.tIp[n] = m * (wx * wx + wz * wz);
-    B.tIr[n] = m * (wy * wy + wz * wz);
+    tx = wy * f;
+    ty = -(wx * f);
+    tf = f;
+    tm = m;
+    tip = m * (wx * wx + wz * wz);
+    tir = m * (wy * wy + wz * wz);
   }
+  B.tX[n] = tx; B.tY[n] = ty; B.tF[n] = tf; B.tM[n] = tm; B.tIp[n] = tip; B.tIr[n] = tir;
 }
 
 // ------------------------------------------------------------------------------------------- readouts
@@ -809,28 +957,14 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   if constexpr (Mixed) {
     const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
-    for (int i0 = tid; i0 < P.S.ncol; i0 += kFetch * kThreads) {
-      Geo<float> g[kFetch];
-#pragma unroll
-      for (int u = 0; u < kFetch; ++u) g[u] = column_geo(P, static_cast<const float*>(nullptr), i0 + u * kThreads < P.S.ncol ? i0 + u * kThreads : i0);
-#pragma unroll
-      for (int u = 0; u < kFetch; ++u)
-        if (i0 + u * kThreads < P.S.ncol) column(f, g[u], p);
-    }
-    tree32(p, lane);
+    stream<float>(P.S.colF, tid, P.S.ncol, kThreads, [&](const Geo<float>& g) { column(f, g, p); });
+    tree32(p);
     if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   } else {
     const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
-    for (int i0 = tid; i0 < P.S.ncol; i0 += kFetch * kThreads) {
-      Geo<double> g[kFetch];
-#pragma unroll
-      for (int u = 0; u < kFetch; ++u) g[u] = column_geo(P, static_cast<const double*>(nullptr), i0 + u * kThreads < P.S.ncol ? i0 + u * kThreads : i0);
-#pragma unroll
-      for (int u = 0; u < kFetch; ++u)
-        if (i0 + u * kThreads < P.S.ncol) column(F, g[u], p);
-    }
-    tree32(p, lane);
+    stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g) { column(F, g, p); });
+    tree32(p);
     if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
   }
 }
@@ -858,8 +992,8 @@ __device__ __forceinline__ double bcast(double v, int from) {
 }
 
 // The tail of a step on wave 0 (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp). Lanes 0..6 run the
-// seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess), eight
-// nodes' values fetched at a time; every lane then integrates the same uniform values, the lanes test the monitors
+// seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess) over terms
+// node_update stored gated and signed, eight nodes' values fetched at a time; every lane then integrates the same uniform values, the lanes test the monitors
 // and marks side by side and lane 0 records the crossings in order; lanes 0..3 evaluate the pose of the next step.
 // Values flow between lanes through registers only: LDS written here is read after the step's closing barrier.
 __device__ void step_tail(PC& P, const Blk& B, int lane) {
@@ -877,23 +1011,16 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
   const int NN = P.S.nn;
   double acc = 0;
   if (lane < 7) {
+    // node_update stored each chain's terms gated and signed: the chains only add
     const SSL double* src = lane == 0 ? B.tX : lane == 1 ? B.tY : lane == 2 ? B.tF : lane == 3 ? B.tM : lane == 4 ? B.tIp : lane == 5 ? B.tIr : B.excess;
-    const SSL double* gate = lane < 6 ? B.vol : B.excess;
-    const bool subtract = lane == 1 || lane == 6;
-    const bool loads = lane < 6;
     acc = lane == 0 ? (byw - P.M.YO) * Fb : lane == 1 ? -(bxw - P.M.XO) * Fb : lane == 2 ? Fb - P.M.g * P.M.ms : lane == 6 ? B.scal[S_SEA] : 0.0;
     for (int n0 = 0; n0 < NN; n0 += 8) {
-      double gv[8], sv[8];
+      double sv[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int n = n0 + k < NN ? n0 + k : NN - 1;
-        gv[k] = gate[n];
-        sv[k] = src[n];
-      }
+      for (int k = 0; k < 8; ++k) sv[k] = src[n0 + k < NN ? n0 + k : NN - 1];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        if (n0 + k < NN && (loads ? !(gv[k] <= 0) : gv[k] > 0)) acc = subtract ? acc - sv[k] : acc + sv[k];
-      }
+      for (int k = 0; k < 8; ++k)
+        if (n0 + k < NN) acc += sv[k];
     }
   }
   const double tauX = bcast(acc, 0), tauY = bcast(acc, 1), Fz = bcast(acc, 2), mw = bcast(acc, 3);
@@ -966,16 +1093,11 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
 // at load time); every thread may read them. Ends with a barrier.
 template <bool Mixed>
 __device__ void device_step(PC& P, const Blk& B, int tid) {
-  const int lane = tid & (kLanes - 1), unit = tid / kLanes;
-  const int NN = P.S.nn, NC = P.M.nc, NG = P.M.ng;
+  const int NN = P.S.nn, NC = P.M.nc;
 
   // (a) buoyancy partials, then (b) the free surfaces: logical group `unit` solves groups unit, unit + 8, ...
   buoyancy_partials<Mixed>(P, B, tid);
-  {
-    const UnitPass<Mixed> pass{P, B.frame, lane, (tid & 32) != 0};
-    SSL double* Pp = B.passes + 7 * P.M.maxGroup * unit;
-    for (int g = unit; g < NG; g += kUnits) solve_group_entry(P, pass, g, B, Pp, lane == 0);
-  }
+  solve_levels<Mixed>(P, B, tid);
   __syncthreads();
 
   // (c) heads and degrees per node

@@ -32,6 +32,9 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two o
 # and 5,135 dry segment evaluations, 72.8 node passes, 51.6 active connections, 34.5 pow calls) and priced by
 # oracle/pyoracle.py:flops_per_step: 2.44e5. tests/test_bench_constants.py re-counts a sample and pins this figure.
 FLOPS_PER_STEP = 2.44e5
+# HBM bytes per launch of run_chunk<false> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a plain
+# copy kernel: profiles/run_chunk_fp64_titanic64_batch1024.md).
+TRAFFIC_BYTES_PER_LAUNCH = 90.5e6
 
 
 def parse():
@@ -165,7 +168,8 @@ def main():
                    # full sinkings per minute at instance 0's length, as apps/sinksim_cuda_run.cpp:138-140 reports it
                    "sinkings_per_min": round(value / max(steps0, 1) * 60.0, 1), "device": _device()},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": None,
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH if (args.count == 1024 and not mixed) else None,
                      "kernel": "run_chunk<false>" if not mixed else "run_chunk<true>", "kernel_ms": round(per_launch_ms, 3)},
     }
     if not args.no_cpu:

@@ -137,6 +137,15 @@ ss_status ss_device_info(ss_device_properties* out);
  * kernel's LDS budget. */
 ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch_options* opt, ss_batch** out);
 
+/* No counterpart in the reference (whose batch runs one hull): a batch of opt->count instances over nmodels
+ * models (ships[k], sims[k]); instance i runs model instModel[i] from that model's initial state and connection
+ * table. One persistent launch hands the instances out (largest models first) and refills each workgroup as its
+ * simulation founders or reaches tMax. Every per-instance entry point below works on a fleet as on a batch;
+ * ss_batch_set_connections takes each instance's row of its own model's connection count, concatenated in
+ * instance order. Traces need a single model (SS_INVALID_ARGUMENT otherwise). */
+ss_status ss_batch_create_fleet(const ss_ship* ships, const ss_sim* sims, int32_t nmodels, const int32_t* instModel,
+                                const ss_batch_options* opt, ss_batch** out);
+
 /* Batch::~Batch: frees every device buffer. Null is ignored. */
 void ss_batch_destroy(ss_batch* b);
 
@@ -155,6 +164,11 @@ ss_status ss_batch_set_connection_activation(ss_batch* b, int32_t instance, int3
  * Any pointer may be null to keep that field. No reference counterpart: it is the bulk form a driver that
  * applies a whole sweep uses instead of count * nconn single calls. */
 ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double* area, const double* tOn);
+
+/* No counterpart in the reference: the number of workgroups of the persistent launch (0, the default: as many as
+ * are resident on the device at once, from the kernel's occupancy at this batch's LDS size). Fewer workgroups refill
+ * more often; the autotuner sets this. SS_LOGIC after run. */
+ss_status ss_batch_set_workgroups(ss_batch* b, int32_t workgroups);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the
  * wall time of the device work (uploads excluded, as in the reference). SS_LOGIC when called twice. */

@@ -68,12 +68,15 @@ SIGNATURES = {
     "ss_available": (C.c_int, [C.c_char_p, C.c_size_t]),
     "ss_device_info": (C.c_int, [C.POINTER(SsDeviceProperties)]),
     "ss_batch_create": (C.c_int, [C.POINTER(SsShip), C.POINTER(SsSim), C.POINTER(SsBatchOptions), C.POINTER(C.c_void_p)]),
+    "ss_batch_create_fleet": (C.c_int, [C.POINTER(SsShip), C.POINTER(SsSim), C.c_int32, _ip, C.POINTER(SsBatchOptions),
+                                        C.POINTER(C.c_void_p)]),
     "ss_batch_destroy": (None, [C.c_void_p]),
     "ss_batch_restore": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp]),
     "ss_batch_set_connection_enabled": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32]),
     "ss_batch_set_connection_area": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "ss_batch_set_connection_activation": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "ss_batch_set_connections": (C.c_int, [C.c_void_p, _bp, _dp, _dp]),
+    "ss_batch_set_workgroups": (C.c_int, [C.c_void_p, C.c_int32]),
     "ss_batch_run": (C.c_int, [C.c_void_p, _dp]),
     "ss_batch_total_steps": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "ss_batch_shared_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),

@@ -69,19 +69,42 @@ def device_info() -> dict:
 class Batch:
     """Many simulations of one ship and scenario on the GPU, computing the portable numerics bit for bit."""
 
-    def __init__(self, ship, sim, opt: BatchOptions):
+    def __init__(self, ship, sim, opt: BatchOptions, _fleet=None):
         self._lib = A.load_library()
         self.ship, self.sim, self.opt = ship, sim, opt
-        self._shipS = A.ShipStruct(ship)
-        self._simS = A.SimStruct(sim)
         o = A.SsBatchOptions(int(opt.count), A.SS_NUMERICS_PORTABLE32 if opt.mixed_precision else A.SS_NUMERICS_PORTABLE,
                              int(opt.traced_instances), int(opt.dense_steps), float(opt.sample_every),
                              float(opt.curve_every), float(opt.t_max), int(opt.steps_per_launch))
         h = C.c_void_p()
-        A.check(self._lib.ss_batch_create(C.byref(self._shipS.s), C.byref(self._simS.s), C.byref(o), C.byref(h)))
+        if _fleet is None:
+            self.models = [(ship, sim)]
+            self.inst_model = np.zeros(int(opt.count), np.int32)
+            self._structs = [(A.ShipStruct(ship), A.SimStruct(sim))]
+            A.check(self._lib.ss_batch_create(C.byref(self._structs[0][0].s), C.byref(self._structs[0][1].s), C.byref(o), C.byref(h)))
+        else:
+            self.models, self.inst_model = _fleet
+            self._structs = [(A.ShipStruct(sh), A.SimStruct(sm)) for sh, sm in self.models]
+            ships = (A.SsShip * len(self.models))(*[a.s for a, _ in self._structs])
+            sims = (A.SsSim * len(self.models))(*[b.s for _, b in self._structs])
+            A.check(self._lib.ss_batch_create_fleet(ships, sims, len(self.models), A.ptr(self.inst_model, C.c_int32), C.byref(o),
+                                                    C.byref(h)))
         self._h = h
         self.count = int(opt.count)
         self.traced = min(int(opt.traced_instances), self.count)
+
+    @classmethod
+    def fleet(cls, models, inst_model, opt: BatchOptions) -> "Batch":
+        """An uneven batch: `models` a list of (ship, sim), instance i running models[inst_model[i]] (no counterpart
+        in the reference, whose batch runs one hull). One persistent launch refills its workgroups as simulations
+        end."""
+        im = np.ascontiguousarray(inst_model, dtype=np.int32)
+        if im.shape != (int(opt.count),):
+            raise ValueError("inst_model must hold one model index per instance")
+        return cls(models[0][0], models[0][1], opt, _fleet=(list(models), im))
+
+    def model_of(self, instance: int):
+        """(ship, sim) of an instance."""
+        return self.models[int(self.inst_model[instance])]
 
     def close(self) -> None:
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -116,18 +139,23 @@ class Batch:
         A.check(self._lib.ss_batch_set_connection_activation(self._h, instance, c, float(tOn)))
 
     def set_connections(self, en=None, area=None, tOn=None) -> None:
-        """Every instance's connection table at once: [count, nconn] arrays (None keeps a field)."""
-        n = self.count * self.sim.connections
+        """Every instance's connection table at once: [count, nconn] arrays, or for a fleet each instance's row of
+        its own model's connection count concatenated (None keeps a field)."""
+        n = int(sum(self.models[m][1].connections for m in self.inst_model))
         e = A.u8(en).reshape(-1) if en is not None else None
         a = A.f64(area).reshape(-1) if area is not None else None
         t = A.f64(tOn).reshape(-1) if tOn is not None else None
         for arr in (e, a, t):
             if arr is not None and arr.size != n:
-                raise ValueError(f"connection tables must hold count * nconn = {n} values")
+                raise ValueError(f"connection tables must hold {n} values (every instance's connections)")
         A.check(self._lib.ss_batch_set_connections(
             self._h, A.ptr(e, C.c_uint8) if e is not None else None, A.ptr(a, C.c_double) if a is not None else None,
             A.ptr(t, C.c_double) if t is not None else None))
         self._keep = (e, a, t)
+
+    def set_workgroups(self, n: int) -> None:
+        """Workgroups of the persistent launch (0: as many as are resident at once)."""
+        A.check(self._lib.ss_batch_set_workgroups(self._h, int(n)))
 
     # ------------------------------------------------------------------ run and results
 
@@ -160,10 +188,11 @@ class Batch:
         return {"steps": steps, "foundered": f.astype(bool), "founderT": ft, "tEnd": te}
 
     def result(self, instance: int) -> InstanceResult:
-        nn = self.ship.nodes
+        ship, sim = self.model_of(instance)
+        nn = ship.nodes
         r = A.SsInstanceResult()
         arrs = [np.zeros(nn) for _ in range(5)]
-        cap = self.sim.monitors + self.sim.marks + 1
+        cap = sim.monitors + sim.marks + 1
         ev = (A.SsEvent * cap)()
         A.check(self._lib.ss_batch_result(self._h, instance, C.byref(r), *[A.ptr(a, C.c_double) for a in arrs], ev, cap))
         events = [(ev[k].t, ev[k].id, ev[k].kind) for k in range(min(r.events, cap))]
@@ -174,7 +203,7 @@ class Batch:
         """Dense and sampled records of a traced instance, [n, 10 + 5 * nodes] each."""
         nd, ns = C.c_int32(), C.c_int32()
         A.check(self._lib.ss_batch_trace(self._h, instance, None, 0, C.byref(nd), None, 0, C.byref(ns)))
-        rs = 10 + 5 * self.ship.nodes
+        rs = 10 + 5 * self.model_of(instance)[0].nodes
         d, s = np.zeros((nd.value, rs)), np.zeros((ns.value, rs))
         A.check(self._lib.ss_batch_trace(self._h, instance, A.ptr(d, C.c_double), nd.value, C.byref(nd),
                                          A.ptr(s, C.c_double), ns.value, C.byref(ns)))

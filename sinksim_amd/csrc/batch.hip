@@ -27,59 +27,60 @@ constexpr int kCurveValues = 7;
 
 // ------------------------------------------------------------------------------------------- the kernel
 
-// Writes the complete state record (the trace layout, 10 + 5 * nodes values), one thread.
-__device__ __noinline__ void write_record(SSG double* rec, const Blk& B, int nn) {
-  for (int k = 0; k < 7; ++k) rec[k] = B.scal[S_T + k];
-  rec[7] = B.scal[O_INFLOW]; rec[8] = B.scal[O_VB]; rec[9] = B.scal[O_TOP];
+// Writes the complete state record (the trace layout, 10 + 5 * nodes values), one thread; the out-of-line part takes
+// the LDS arrays it reads rather than the block's view, which a call would have to copy to scratch.
+__device__ __noinline__ void write_record_arrays(SSG double* rec, const SSL double* scal, const SSL double* vol, const SSL double* level,
+                                                const SSL double* cx, const SSL double* cy, const SSL double* cz, int nn) {
+  for (int k = 0; k < 7; ++k) rec[k] = scal[S_T + k];
+  rec[7] = scal[O_INFLOW]; rec[8] = scal[O_VB]; rec[9] = scal[O_TOP];
   SSG double* p = rec + 10;
-  for (int n = 0; n < nn; ++n) p[n] = B.vol[n];
+  for (int n = 0; n < nn; ++n) p[n] = vol[n];
   p += nn;
-  for (int n = 0; n < nn; ++n) p[n] = B.level[n];
+  for (int n = 0; n < nn; ++n) p[n] = level[n];
   p += nn;
-  for (int n = 0; n < nn; ++n) p[n] = B.cx[n];
+  for (int n = 0; n < nn; ++n) p[n] = cx[n];
   p += nn;
-  for (int n = 0; n < nn; ++n) p[n] = B.cy[n];
+  for (int n = 0; n < nn; ++n) p[n] = cy[n];
   p += nn;
-  for (int n = 0; n < nn; ++n) p[n] = B.cz[n];
+  for (int n = 0; n < nn; ++n) p[n] = cz[n];
+}
+__device__ __forceinline__ void write_record(SSG double* rec, const Blk& B, int nn) {
+  write_record_arrays(rec, B.scal, B.vol, B.level, B.cx, B.cy, B.cz, nn);
 }
 
-// One workgroup per instance: load the instance into LDS, step it until it founders, reaches tMax or has taken
-// maxSteps steps in this launch, store it back. Traces and curves are recorded as in the reference
+// One instance on one workgroup: load it into LDS, step it until it founders, reaches tMax or has taken maxSteps
+// steps in this launch, store it back. Traces and curves are recorded as in the reference
 // (device_step.cuh:379-416); the readouts of the state at the end of the launch go to final7.
 template <bool Mixed>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_chunk(const Params* __restrict__ params, long long maxSteps, double tMax, int firstLaunch) {
-  PC& P = *(PC*)params;
-  extern __shared__ double lds_raw[];
-  SSL double* lds = (SSL double*)lds_raw;
-  const int inst = blockIdx.x;
-  const int tid = threadIdx.x;
+__device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double tMax, SSL double* lds, int tid) {
   const int NN = P.S.nn, NC = P.M.nc, evCap = P.M.evCap, nmon = P.M.nmon, nmark = P.M.nmark;
-  const size_t iN = static_cast<size_t>(inst) * NN, iC = static_cast<size_t>(inst) * NC;
+  const size_t iN = static_cast<size_t>(Bt.nodeOff[inst]), iC = static_cast<size_t>(Bt.connOff[inst]);
+  const size_t iE = static_cast<size_t>(Bt.evOff[inst]), iMon = static_cast<size_t>(Bt.monOff[inst]), iMark = static_cast<size_t>(Bt.markOff[inst]);
   const Blk B = carve(lds, P);
 
-  // load the instance
+  // load the instance, and the model's node and group tables
   if (tid < 7) {
-    B.scal[S_T + tid] = P.Bt.st[inst * 7 + tid];
-    B.scal[O_INFLOW + tid] = P.Bt.out[inst * 7 + tid];
+    B.scal[S_T + tid] = Bt.st[inst * 7 + tid];
+    B.scal[O_INFLOW + tid] = Bt.out[inst * 7 + tid];
   }
   if (tid == 0) {
-    B.scal[S_EVCOUNT] = P.Bt.evCount[inst];
-    B.scal[S_FOUNDERED] = P.Bt.foundered[inst];
-    B.scal[S_FOUNDERT] = P.Bt.founderT[inst];
+    B.scal[S_EVCOUNT] = Bt.evCount[inst];
+    B.scal[S_FOUNDERED] = Bt.foundered[inst];
+    B.scal[S_FOUNDERT] = Bt.founderT[inst];
     B.scal[S_SEA] = 0;
-    if (P.Bt.curveEvery > 0) {
-      B.scal[S_NEXTCURVE] = P.Bt.nextCurve[inst];
-      B.scal[S_CURVECOUNT] = P.Bt.curveCount[inst];
+    if (Bt.curveEvery > 0) {
+      B.scal[S_NEXTCURVE] = Bt.nextCurve[inst];
+      B.scal[S_CURVECOUNT] = Bt.curveCount[inst];
     }
   }
   for (int n = tid; n < NN; n += kThreads) {
-    B.vol[n] = P.Bt.vol[iN + n]; B.level[n] = P.Bt.level[iN + n];
-    B.cx[n] = P.Bt.cx[iN + n]; B.cy[n] = P.Bt.cy[iN + n]; B.cz[n] = P.Bt.cz[iN + n];
+    B.vol[n] = Bt.vol[iN + n]; B.level[n] = Bt.level[iN + n];
+    B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n];
     B.area[n] = 0; B.zmin[n] = 0; B.zmax[n] = 0; B.hEff[n] = 0; B.aEff[n] = 0; B.excess[n] = 0; B.deg[n] = 0;
     B.tX[n] = 0; B.tY[n] = 0; B.tF[n] = 0; B.tM[n] = 0; B.tIp[n] = 0; B.tIr[n] = 0;
   }
   for (int c = tid; c < NC; c += kThreads) {
-    B.cen[c] = P.Bt.en[iC + c]; B.carea[c] = P.Bt.area[iC + c]; B.ctOn[c] = P.Bt.tOn[iC + c];
+    B.cen[c] = Bt.en[iC + c]; B.carea[c] = Bt.area[iC + c]; B.ctOn[c] = Bt.tOn[iC + c];
     B.dV[c] = 0; B.q[c] = 0;
   }
   for (int g = tid; g < P.M.ng; g += kThreads) {
@@ -92,19 +93,16 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     B.nSegStart[n] = P.S.segStart[n]; B.nSegCount[n] = P.S.segCount[n];
     B.nMu[n] = P.S.mu[n]; B.nVmax[n] = P.S.vmax[n]; B.nAov[n] = P.S.aov[n]; B.nAmin[n] = P.S.amin[n];
   }
-  for (int e = tid; e < evCap; e += kThreads) {
-    const size_t k = static_cast<size_t>(inst) * evCap + e;
-    B.evT[e] = P.Bt.evT[k]; B.evId[e] = P.Bt.evId[k]; B.evKind[e] = P.Bt.evKind[k];
-  }
-  for (int m = tid; m < nmon; m += kThreads) { B.overT[m] = P.Bt.overT[static_cast<size_t>(inst) * nmon + m]; B.over[m] = 0; }
-  for (int m = tid; m < nmark; m += kThreads) B.markT[m] = P.Bt.markT[static_cast<size_t>(inst) * nmark + m];
+  for (int e = tid; e < evCap; e += kThreads) { B.evT[e] = Bt.evT[iE + e]; B.evId[e] = Bt.evId[iE + e]; B.evKind[e] = Bt.evKind[iE + e]; }
+  for (int m = tid; m < nmon; m += kThreads) { B.overT[m] = Bt.overT[iMon + m]; B.over[m] = 0; }
+  for (int m = tid; m < nmark; m += kThreads) B.markT[m] = Bt.markT[iMark + m];
   __syncthreads();
   if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);  // the loaded state's pose
   __syncthreads();
 
-  const bool traced = inst < P.Bt.traced;
-  long long done = P.Bt.steps[inst];
-  if (firstLaunch && traced) {
+  const bool traced = inst < Bt.traced;
+  long long done = Bt.steps[inst];
+  if (done == 0 && traced) {
     // the initial record: buoyancy at the initial pose, no inflow yet (simulation.cpp:137-145, 204-206)
     buoyancy_partials<Mixed>(P, B, tid);
     __syncthreads();
@@ -112,45 +110,48 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
       double V, bx, by, bz, top;
       finish_buoyancy(B, V, bx, by, bz, top);
       B.scal[O_INFLOW] = 0; B.scal[O_VB] = V; B.scal[O_TOP] = top; B.scal[O_BX] = bx; B.scal[O_BY] = by; B.scal[O_BZ] = bz;
-      const size_t rs = static_cast<size_t>(P.Bt.recSize);
-      write_record(P.Bt.denseRec + static_cast<size_t>(inst) * (P.Bt.denseSteps + 1) * rs, B, NN);
-      write_record(P.Bt.sampleRec + static_cast<size_t>(inst) * P.Bt.maxSamples * rs, B, NN);
-      P.Bt.denseCount[inst] = 1; P.Bt.sampleCount[inst] = 1; P.Bt.nextSample[inst] = P.Bt.sampleEvery;
+      const size_t rs = static_cast<size_t>(Bt.recSize);
+      write_record(Bt.denseRec + static_cast<size_t>(inst) * (Bt.denseSteps + 1) * rs, B, NN);
+      write_record(Bt.sampleRec + static_cast<size_t>(inst) * Bt.maxSamples * rs, B, NN);
+      Bt.denseCount[inst] = 1; Bt.sampleCount[inst] = 1; Bt.nextSample[inst] = Bt.sampleEvery;
     }
     __syncthreads();
   }
 
-  const bool curves = P.Bt.curveEvery > 0;
+  const bool curves = Bt.curveEvery > 0;
+  uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P));  // the same in every lane, held in scalar registers
   for (long long s = 0; s < maxSteps; ++s) {
-    if (B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= tMax) break;  // uniform: LDS values after a barrier
+    PC& Q = model_at(pa);
+    const Blk Bq = carve(lds, Q);
+    if (Bq.scal[S_FOUNDERED] != 0 || Bq.scal[S_T] >= tMax) break;  // uniform: LDS values after a barrier
     // a readout-curve record when due (simulation.cpp:175-176). Every wave evaluates the condition on the same
     // LDS values; the barrier inside orders all of those reads (and the record's reads of the volumes) before
     // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
-    if (curves && B.scal[S_T] >= B.scal[S_NEXTCURVE] && B.scal[S_CURVECOUNT] < P.Bt.curveMax) {
+    if (curves && Bq.scal[S_T] >= Bq.scal[S_NEXTCURVE] && Bq.scal[S_CURVECOUNT] < Bt.curveMax) {
       if (tid == 0) {
-        SSG double* rec = P.Bt.curveRec + (static_cast<size_t>(inst) * P.Bt.curveMax + static_cast<size_t>(B.scal[S_CURVECOUNT])) * kCurveValues;
-        readouts7(P, B, rec);
+        SSG double* rec = Bt.curveRec + (static_cast<size_t>(inst) * Bt.curveMax + static_cast<size_t>(Bq.scal[S_CURVECOUNT])) * kCurveValues;
+        readouts7(Q, Bq.scal, Bq.vol, rec);
       }
       __syncthreads();
       if (tid == 0) {
-        B.scal[S_CURVECOUNT] += 1;
-        B.scal[S_NEXTCURVE] += P.Bt.curveEvery;
+        Bq.scal[S_CURVECOUNT] += 1;
+        Bq.scal[S_NEXTCURVE] += Bt.curveEvery;
       }
       // read again only at the next iteration, after the step's barriers
     }
-    device_step<Mixed>(P, B, tid);
+    device_step<Mixed>(pa, lds, tid);
     ++done;
     if (traced) {
       if (tid == 0) {
-        const size_t rs = static_cast<size_t>(P.Bt.recSize);
-        if (done <= P.Bt.denseSteps) {
-          const int k = P.Bt.denseCount[inst]++;
-          write_record(P.Bt.denseRec + (static_cast<size_t>(inst) * (P.Bt.denseSteps + 1) + k) * rs, B, NN);
+        const size_t rs = static_cast<size_t>(Bt.recSize);
+        if (done <= Bt.denseSteps) {
+          const int k = Bt.denseCount[inst]++;
+          write_record(Bt.denseRec + (static_cast<size_t>(inst) * (Bt.denseSteps + 1) + k) * rs, Bq, NN);
         }
-        if (B.scal[S_T] >= P.Bt.nextSample[inst] - 1e-9 && P.Bt.sampleCount[inst] < P.Bt.maxSamples) {
-          const int k = P.Bt.sampleCount[inst]++;
-          write_record(P.Bt.sampleRec + (static_cast<size_t>(inst) * P.Bt.maxSamples + k) * rs, B, NN);
-          P.Bt.nextSample[inst] += P.Bt.sampleEvery;
+        if (Bq.scal[S_T] >= Bt.nextSample[inst] - 1e-9 && Bt.sampleCount[inst] < Bt.maxSamples) {
+          const int k = Bt.sampleCount[inst]++;
+          write_record(Bt.sampleRec + (static_cast<size_t>(inst) * Bt.maxSamples + k) * rs, Bq, NN);
+          Bt.nextSample[inst] += Bt.sampleEvery;
         }
       }
       __syncthreads();
@@ -159,31 +160,65 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
 
   // store the instance
   if (tid < 7) {
-    P.Bt.st[inst * 7 + tid] = B.scal[S_T + tid];
-    P.Bt.out[inst * 7 + tid] = B.scal[O_INFLOW + tid];
+    Bt.st[inst * 7 + tid] = B.scal[S_T + tid];
+    Bt.out[inst * 7 + tid] = B.scal[O_INFLOW + tid];
   }
   if (tid == 0) {
-    P.Bt.evCount[inst] = static_cast<int32_t>(B.scal[S_EVCOUNT]);
-    P.Bt.foundered[inst] = static_cast<uint8_t>(B.scal[S_FOUNDERED]);
-    P.Bt.founderT[inst] = B.scal[S_FOUNDERT];
-    P.Bt.steps[inst] = done;
-    if (P.Bt.curveEvery > 0) {
-      P.Bt.nextCurve[inst] = B.scal[S_NEXTCURVE];
-      P.Bt.curveCount[inst] = static_cast<int32_t>(B.scal[S_CURVECOUNT]);
+    Bt.evCount[inst] = static_cast<int32_t>(B.scal[S_EVCOUNT]);
+    Bt.foundered[inst] = static_cast<uint8_t>(B.scal[S_FOUNDERED]);
+    Bt.founderT[inst] = B.scal[S_FOUNDERT];
+    Bt.steps[inst] = done;
+    if (Bt.curveEvery > 0) {
+      Bt.nextCurve[inst] = B.scal[S_NEXTCURVE];
+      Bt.curveCount[inst] = static_cast<int32_t>(B.scal[S_CURVECOUNT]);
     }
-    readouts7(P, B, P.Bt.final7 + static_cast<size_t>(inst) * 7);
+    readouts7(P, B.scal, B.vol, Bt.final7 + static_cast<size_t>(inst) * 7);
   }
   for (int n = tid; n < NN; n += kThreads) {
-    P.Bt.vol[iN + n] = B.vol[n]; P.Bt.level[iN + n] = B.level[n];
-    P.Bt.cx[iN + n] = B.cx[n]; P.Bt.cy[iN + n] = B.cy[n]; P.Bt.cz[iN + n] = B.cz[n];
+    Bt.vol[iN + n] = B.vol[n]; Bt.level[iN + n] = B.level[n];
+    Bt.cx[iN + n] = B.cx[n]; Bt.cy[iN + n] = B.cy[n]; Bt.cz[iN + n] = B.cz[n];
   }
-  for (int e = tid; e < evCap; e += kThreads) {
-    const size_t k = static_cast<size_t>(inst) * evCap + e;
-    P.Bt.evT[k] = B.evT[e]; P.Bt.evId[k] = B.evId[e]; P.Bt.evKind[k] = B.evKind[e];
-  }
-  for (int m = tid; m < nmon; m += kThreads) P.Bt.overT[static_cast<size_t>(inst) * nmon + m] = B.overT[m];
-  for (int m = tid; m < nmark; m += kThreads) P.Bt.markT[static_cast<size_t>(inst) * nmark + m] = B.markT[m];
+  for (int e = tid; e < evCap; e += kThreads) { Bt.evT[iE + e] = B.evT[e]; Bt.evId[iE + e] = B.evId[e]; Bt.evKind[iE + e] = B.evKind[e]; }
+  for (int m = tid; m < nmon; m += kThreads) Bt.overT[iMon + m] = B.overT[m];
+  for (int m = tid; m < nmark; m += kThreads) Bt.markT[iMark + m] = B.markT[m];
 }
+
+// The batch kernel. Persistent: each workgroup takes the next work item from the batch's counter, runs that
+// instance (of its own model) to the end of the launch, and takes another, until the queue is empty; a workgroup
+// whose instance founders or reaches tMax early refills itself with the next one, and every workgroup leaves the
+// loop once the counter has passed the item count. Otherwise (when every instance has a resident workgroup of its
+// own, so there is nothing to refill) workgroup k runs work item k; that form schedules a little better.
+template <bool Mixed, bool Persistent>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_batch(const ModelParams* __restrict__ models,
+                                                                                                  const DevBatch* __restrict__ batch,
+                                                                                                  long long maxSteps, double tMax) {
+  BC& Bt = *(BC*)batch;
+  extern __shared__ double lds_raw[];
+  __shared__ int32_t item;
+  SSL double* lds = (SSL double*)lds_raw;
+  const int tid = threadIdx.x;
+  if constexpr (!Persistent) {
+    const int inst = __builtin_amdgcn_readfirstlane(Bt.order[blockIdx.x]);
+    PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(Bt.instModel[inst]) * sizeof(ModelParams));
+    run_instance<Mixed>(P, Bt, inst, maxSteps, tMax, lds, tid);
+    return;
+  }
+  for (;;) {
+    if (tid == 0) item = atomicAdd((int32_t*)Bt.counter, 1);
+    __syncthreads();
+    const int k = __builtin_amdgcn_readfirstlane(item);  // uniform: the model's address stays in scalar registers
+    __syncthreads();  // every wave has read the item before thread 0 takes the next one
+    if (k >= Bt.count) break;
+    const int inst = __builtin_amdgcn_readfirstlane(Bt.order[k]);
+    PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(Bt.instModel[inst]) * sizeof(ModelParams));
+    run_instance<Mixed>(P, Bt, inst, maxSteps, tMax, lds, tid);
+    __syncthreads();  // the instance's LDS is free for the next one
+  }
+}
+
+// Resets the batch's work counter before a launch (a kernel rather than a memset, so that the pair can be captured
+// in a graph and replayed).
+__global__ void reset_counter(int32_t* counter) { *counter = 0; }
 
 // ------------------------------------------------------------------------------------------- host side
 
@@ -279,18 +314,13 @@ int device_count(std::string* why) {
 
 using namespace ssa;
 
-struct ss_batch {
-  ss_batch_options opt{};
-  int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0;
-  bool mixed = true;
+// One model (a hull and a flooding scenario) on the device: the geometry records, the network tables and their
+// views, with the scenario's per-instance defaults kept on the host.
+struct Model {
+  int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0, members = 0;
   LdsLayout layout{};
   size_t ldsBytes = 0;
-  int32_t recSize = 0, maxSamples = 0, curveMax = 0;
-  int64_t totalSteps = 0;
-  bool ran = false;
-  double kernelMs = 0;
-  int launches = 0;
-
+  double cost = 0;  // work per step, relative: the order in which a batch hands its instances out
   // ship
   Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record
   Dev<float> colF, segF;                      // 8 floats per record
@@ -302,13 +332,49 @@ struct ss_batch {
   Dev<uint8_t> gMode, mkWhen;
   Dev<double> gpx, gpy, gpz, mkx, mky, mkz;
   DevNet net{};
+  // the scenario's defaults of the per-instance tables and state
+  std::vector<uint8_t> en0;
+  std::vector<double> area0, tOn0, state0, vol0, level0, cx0, cy0, cz0;
+
+  ModelParams params() const {
+    ModelParams m{};
+    DevShip& s = m.S;
+    s.ncol = ncol; s.nseg = nseg; s.nn = NN;
+    s.col = (const SSG d2v*)col.p; s.colF = (const SSG f4v*)colF.p;
+    s.seg = (const SSG d2v*)seg.p; s.segF = (const SSG f4v*)segF.p;
+    s.segStart = G(segStart.p); s.segCount = G(segCount.p);
+    s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
+    m.M = net;
+    m.L = layout;
+    return m;
+  }
+};
+
+struct ss_batch {
+  ss_batch_options opt{};
+  bool mixed = true;
+  std::vector<std::unique_ptr<Model>> models;
+  // per instance: its model and the offsets of its rows; the work order
+  std::vector<int32_t> instModel, nodeOff, connOff, evOff, monOff, markOff, order;
+  size_t nodeTotal = 0, connTotal = 0, evTotal = 0, monTotal = 0, markTotal = 0;
+  size_t ldsBytes = 0;  // the largest model's
+  int32_t recSize = 0, maxSamples = 0, curveMax = 0;
+  int32_t blocks = 0;   // workgroups of the persistent launch (0: as many as fit on the device at once)
+  bool refill = false;  // whether the last run used the refilling (persistent) kernel
+  int64_t totalSteps = 0;
+  bool ran = false;
+  double kernelMs = 0;
+  int launches = 0;
+
+  Dev<ModelParams> dModels;
+  Dev<DevBatch> dBatch;
+  Dev<int32_t> dInstModel, dOrder, dNodeOff, dConnOff, dEvOff, dMonOff, dMarkOff, counter;
   // per instance
   Dev<uint8_t> en, foundered, evKind;
   Dev<double> area, tOn, st, out, vol, level, ccx, ccy, ccz, evT, overT, markT, founderT, final7, denseRec, sampleRec, nextSample, curveRec,
       nextCurve;
   Dev<int32_t> evId, evCount, denseCount, sampleCount, curveCount;
   Dev<long long> steps;
-  Dev<Params> params;
   // host staging of the per-instance tables until run()
   std::vector<uint8_t> hEn;
   std::vector<double> hArea, hTOn, hSt, hVol, hLevel, hCx, hCy, hCz;
@@ -318,19 +384,14 @@ struct ss_batch {
   std::vector<double> rFounderT, rSt, rOut, rFinal;
   std::vector<int32_t> rEvCount, rCurveCount;
 
-  DevShip shipView() const {
-    DevShip s{};
-    s.ncol = ncol; s.nseg = nseg; s.nn = NN;
-    s.col = (const SSG d2v*)col.p; s.colF = (const SSG f4v*)colF.p;
-    s.seg = (const SSG d2v*)seg.p; s.segF = (const SSG f4v*)segF.p;
-    s.segStart = G(segStart.p); s.segCount = G(segCount.p);
-    s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
-    return s;
-  }
+  const Model& model(int32_t i) const { return *models[static_cast<size_t>(instModel[static_cast<size_t>(i)])]; }
 
   DevBatch batchView() const {
     DevBatch b{};
     b.count = opt.count;
+    b.instModel = G(dInstModel.p); b.order = G(dOrder.p);
+    b.nodeOff = G(dNodeOff.p); b.connOff = G(dConnOff.p); b.evOff = G(dEvOff.p); b.monOff = G(dMonOff.p); b.markOff = G(dMarkOff.p);
+    b.counter = W(counter.p);
     b.en = G(en.p); b.area = G(area.p); b.tOn = G(tOn.p);
     b.st = W(st.p); b.out = W(out.p);
     b.vol = W(vol.p); b.level = W(level.p); b.cx = W(ccx.p); b.cy = W(ccy.p); b.cz = W(ccz.p);
@@ -492,135 +553,211 @@ ss_status ss_device_info(ss_device_properties* out) {
   });
 }
 
+}  // extern "C"
+
+namespace {
+
+void check_options(const ss_batch_options* opt, const char* who) {
+  const std::string w(who);
+  need(opt != nullptr, w + ": null options");
+  need(opt->count >= 1, w + ": count must be at least 1");
+  need(opt->numerics == SS_NUMERICS_PORTABLE || opt->numerics == SS_NUMERICS_PORTABLE32,
+       w + ": numerics must be portable (1) or portable32 (2); the oracle and std numerics are CPU-only");
+  need(opt->traced >= 0 && opt->denseSteps >= 0, w + ": negative trace option");
+  need(opt->traced == 0 || opt->sampleEvery > 0, w + ": sampleEvery must be positive when tracing");
+  need(opt->tMax > 0 && std::isfinite(opt->tMax), w + ": tMax must be positive and finite");
+  need(opt->curveEvery >= 0, w + ": curveEvery must not be negative");
+}
+
+// Uploads one model: geometry records, network tables (model.cpp:137-167 for the incidence), LDS layout.
+std::unique_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t ldsLimit) {
+  std::unique_ptr<Model> mp(new Model);
+  Model& b = *mp;
+  b.NN = S.nnode; b.NC = M.nconn; b.NG = M.ngroup; b.nmon = M.nmonitor; b.nmark = M.nmark; b.ncol = S.ncol; b.nseg = S.nseg;
+  b.evCap = b.nmon + b.nmark + 1;
+  for (int32_t g = 0; g < b.NG; ++g) b.maxGroup = std::max(b.maxGroup, M.gNodeCount[g]);
+  size_t members = 0;  // length of the group member list the groups index into
+  for (int32_t g = 0; g < b.NG; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
+  b.members = static_cast<int32_t>(members);
+  b.layout = lds_layout(b.NN, b.NC, b.NG, b.nmon, b.nmark, b.maxGroup, b.evCap, b.members);
+  b.ldsBytes = static_cast<size_t>(b.layout.total) * sizeof(double);
+  if (b.ldsBytes > ldsLimit)
+    fail(SS_RUNTIME, "ss_batch_create: this model needs " + std::to_string(b.ldsBytes) + " bytes of LDS per simulation, the device allows " +
+                         std::to_string(ldsLimit));
+  b.cost = static_cast<double>(S.ncol) + 2.0 * static_cast<double>(S.nseg) + 50.0 * static_cast<double>(S.nnode);
+
+  const size_t nc = static_cast<size_t>(S.ncol), ns = static_cast<size_t>(S.nseg), nn = static_cast<size_t>(S.nnode);
+  // records (x, y, bottom, top, footprint): 6 doubles (the last one 0) and 8 floats (the last three 0)
+  auto records = [](size_t n, auto&& value, std::vector<double>& d, std::vector<float>& f) {
+    d.assign(std::max<size_t>(n, 1) * 6, 0.0);
+    f.assign(std::max<size_t>(n, 1) * 8, 0.0f);
+    for (size_t i = 0; i < n; ++i)
+      for (int k = 0; k < 5; ++k) {
+        const double v = value(i, k);
+        d[6 * i + k] = v;
+        f[8 * i + k] = static_cast<float>(v);
+      }
+  };
+  std::vector<double> recD;
+  std::vector<float> recF;
+  records(nc, [&](size_t i, int k) { const double* a[5] = {S.x, S.y, S.zlo, S.zhi, S.dxdy}; return a[k][i]; }, recD, recF);
+  b.col.upload(recD); b.colF.upload(recF);
+  records(ns, [&](size_t j, int k) {
+    const size_t c = static_cast<size_t>(S.segCol[j]);
+    return k == 0 ? S.x[c] : k == 1 ? S.y[c] : k == 2 ? S.segA[j] : k == 3 ? S.segE[j] : S.dxdy[c];
+  }, recD, recF);
+  b.seg.upload(recD); b.segF.upload(recF);
+  b.segStart.upload(S.segStart, nn); b.segCount.upload(S.segCount, nn);
+  b.mu.upload(S.mu, nn); b.vmax.upload(S.vmax, nn); b.amin.upload(S.amin, nn); b.aov.upload(S.aov, nn);
+
+  const size_t ncn = static_cast<size_t>(M.nconn), ng = static_cast<size_t>(M.ngroup);
+  b.gStart.upload(M.gNodeStart, ng); b.gCount.upload(M.gNodeCount, ng); b.gNodes.upload(M.gNodes, members); b.gMode.upload(M.gMode, ng);
+  b.gpx.upload(M.gpx, ng); b.gpy.upload(M.gpy, ng); b.gpz.upload(M.gpz, ng);
+  std::vector<int4> conI(std::max<size_t>(ncn, 1));
+  std::vector<double4> conD(std::max<size_t>(ncn, 1));
+  for (size_t c = 0; c < ncn; ++c) {
+    conI[c] = make_int4(M.ca[c], M.cb[c], M.skipGroup[c], M.law[c]);
+    conD[c] = make_double4(M.cx[c], M.cy[c], M.cz[c], M.coef[c]);
+  }
+  b.conI.upload(conI); b.conD.upload(conD);
+  std::vector<int32_t> incCount(nn, 0), incStart(nn, 0), fill(nn, 0), sea;
+  for (size_t c = 0; c < ncn; ++c) {
+    if (M.ca[c] >= 0) incCount[M.ca[c]]++;
+    incCount[M.cb[c]]++;
+  }
+  int32_t total = 0;
+  for (size_t n = 0; n < nn; ++n) { incStart[n] = total; total += incCount[n]; }
+  std::vector<int4> inc(std::max<int32_t>(total, 1), make_int4(0, 0, 0, 0));
+  for (size_t c = 0; c < ncn; ++c) {
+    const int32_t ci = static_cast<int32_t>(c);
+    if (M.ca[c] >= 0) {
+      const int32_t n = M.ca[c];
+      inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 0);
+    } else {
+      sea.push_back(ci);
+    }
+    const int32_t n = M.cb[c];
+    inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 1);
+  }
+  b.incStart.upload(incStart); b.incCount.upload(incCount); b.inc.upload(inc); b.sea.upload(sea);
+  std::vector<int32_t> monCount(std::max<int32_t>(b.nmon, 0), 0), monStart(std::max<int32_t>(b.nmon, 0), 0), monConn;
+  for (int32_t m = 0; m < b.nmon; ++m) {
+    monStart[m] = static_cast<int32_t>(monConn.size());
+    for (size_t c = 0; c < ncn; ++c)
+      if (M.monitor[c] == m) monConn.push_back(static_cast<int32_t>(c));
+    monCount[m] = static_cast<int32_t>(monConn.size()) - monStart[m];
+  }
+  b.monStart.upload(monStart); b.monCount.upload(monCount); b.monConn.upload(monConn);
+  b.mkx.upload(M.mkx, M.nmark); b.mky.upload(M.mky, M.nmark); b.mkz.upload(M.mkz, M.nmark); b.mkWhen.upload(M.mkWhen, M.nmark);
+
+  DevNet& d = b.net;
+  d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
+  d.ng = M.ngroup; d.nMembers = b.members; d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
+  d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
+  d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p;
+  d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.inc = (const SSG i4v*)b.inc.p;
+  d.seaCount = static_cast<int32_t>(sea.size()); d.sea = G(b.sea.p);
+  d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = G(b.monStart.p); d.monCount = G(b.monCount.p); d.monConn = G(b.monConn.p);
+  d.nmark = M.nmark; d.mkx = G(b.mkx.p); d.mky = G(b.mky.p); d.mkz = G(b.mkz.p); d.mkWhen = G(b.mkWhen.p);
+  d.hullTopBelow = M.founder[0]; d.pitchAbsAbove = M.founder[1]; d.buoyancyBelow = M.founder[2]; d.minT = M.founder[3];
+  const double* bd = M.body;
+  d.ms = bd[0]; d.g0x = bd[2]; d.g0y = bd[3]; d.g0z = bd[4]; d.XO = bd[5]; d.YO = bd[6]; d.mh0 = bd[7]; d.Ith0 = bd[8]; d.Iph0 = bd[9];
+  d.cDz = bd[10]; d.cDth = bd[11]; d.cDph = bd[12]; d.qz = bd[13]; d.qth = bd[14]; d.qph = bd[15];
+  d.xFP = M.readout[0]; d.xAP = M.readout[1];
+  d.maxGroup = b.maxGroup; d.evCap = b.evCap;
+
+  b.en0.assign(M.en, M.en + ncn); b.area0.assign(M.area, M.area + ncn); b.tOn0.assign(M.tOn, M.tOn + ncn);
+  b.state0.assign(M.state0, M.state0 + 7);
+  b.vol0.assign(M.vol0, M.vol0 + nn); b.level0.assign(M.level0, M.level0 + nn);
+  b.cx0.assign(M.cx0, M.cx0 + nn); b.cy0.assign(M.cy0, M.cy0 + nn); b.cz0.assign(M.cz0, M.cz0 + nn);
+  return mp;
+}
+
+// A batch over uploaded models: per-instance offsets, the work order (largest models first, so that the long
+// simulations start early and the small ones fill in at the end), and the host staging of every instance's tables
+// and state from its model's defaults.
+std::unique_ptr<ss_batch> make_batch(const ss_batch_options& opt, std::vector<std::unique_ptr<Model>> models, std::vector<int32_t> instModel) {
+  std::unique_ptr<ss_batch> B(new ss_batch);
+  ss_batch& b = *B;
+  b.opt = opt;
+  if (b.opt.traced > b.opt.count) b.opt.traced = b.opt.count;
+  b.mixed = opt.numerics == SS_NUMERICS_PORTABLE32;
+  b.models = std::move(models);
+  b.instModel = std::move(instModel);
+  const size_t count = static_cast<size_t>(b.opt.count);
+  b.nodeOff.resize(count); b.connOff.resize(count); b.evOff.resize(count); b.monOff.resize(count); b.markOff.resize(count);
+  for (size_t i = 0; i < count; ++i) {
+    const Model& m = b.model(static_cast<int32_t>(i));
+    b.nodeOff[i] = static_cast<int32_t>(b.nodeTotal); b.nodeTotal += static_cast<size_t>(m.NN);
+    b.connOff[i] = static_cast<int32_t>(b.connTotal); b.connTotal += static_cast<size_t>(m.NC);
+    b.evOff[i] = static_cast<int32_t>(b.evTotal); b.evTotal += static_cast<size_t>(m.evCap);
+    b.monOff[i] = static_cast<int32_t>(b.monTotal); b.monTotal += static_cast<size_t>(m.nmon);
+    b.markOff[i] = static_cast<int32_t>(b.markTotal); b.markTotal += static_cast<size_t>(m.nmark);
+    b.hEn.insert(b.hEn.end(), m.en0.begin(), m.en0.end());
+    b.hArea.insert(b.hArea.end(), m.area0.begin(), m.area0.end());
+    b.hTOn.insert(b.hTOn.end(), m.tOn0.begin(), m.tOn0.end());
+    b.hSt.insert(b.hSt.end(), m.state0.begin(), m.state0.end());
+    b.hVol.insert(b.hVol.end(), m.vol0.begin(), m.vol0.end());
+    b.hLevel.insert(b.hLevel.end(), m.level0.begin(), m.level0.end());
+    b.hCx.insert(b.hCx.end(), m.cx0.begin(), m.cx0.end());
+    b.hCy.insert(b.hCy.end(), m.cy0.begin(), m.cy0.end());
+    b.hCz.insert(b.hCz.end(), m.cz0.begin(), m.cz0.end());
+  }
+  if (b.nodeTotal > static_cast<size_t>(INT32_MAX) || b.connTotal > static_cast<size_t>(INT32_MAX))
+    fail(SS_INVALID_ARGUMENT, "ss_batch_create: the batch's node or connection rows exceed 2^31 entries");
+  for (const auto& m : b.models) b.ldsBytes = std::max(b.ldsBytes, m->ldsBytes);
+  b.order.resize(count);
+  for (size_t i = 0; i < count; ++i) b.order[i] = static_cast<int32_t>(i);
+  std::stable_sort(b.order.begin(), b.order.end(), [&](int32_t x, int32_t y) { return b.model(x).cost > b.model(y).cost; });
+  b.recSize = 10 + 5 * b.models[0]->NN;
+  b.maxSamples = b.opt.traced > 0 ? static_cast<int32_t>(std::floor(b.opt.tMax / b.opt.sampleEvery)) + 2 : 0;
+  return B;
+}
+
+size_t lds_limit() {
+  hipDeviceProp_t p{};
+  hip_check(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties");
+  return p.sharedMemPerBlock;
+}
+
+}  // namespace
+
+extern "C" {
+
 ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch_options* opt, ss_batch** out) {
   return guarded([&] {
     need(out != nullptr, "ss_batch_create: null output handle");
     *out = nullptr;
     need(ship != nullptr, "ss_batch_create: null ship");
     need(sim != nullptr, "ss_batch_create: null simulation");
-    need(opt != nullptr, "ss_batch_create: null options");
-    need(opt->count >= 1, "ss_batch_create: count must be at least 1");
-    need(opt->numerics == SS_NUMERICS_PORTABLE || opt->numerics == SS_NUMERICS_PORTABLE32,
-         "ss_batch_create: numerics must be portable (1) or portable32 (2); the oracle and std numerics are CPU-only");
-    need(opt->traced >= 0 && opt->denseSteps >= 0, "ss_batch_create: negative trace option");
-    need(opt->traced == 0 || opt->sampleEvery > 0, "ss_batch_create: sampleEvery must be positive when tracing");
-    need(opt->tMax > 0 && std::isfinite(opt->tMax), "ss_batch_create: tMax must be positive and finite");
-    need(opt->curveEvery >= 0, "ss_batch_create: curveEvery must not be negative");
+    check_options(opt, "ss_batch_create");
     validate(*ship, *sim);
     std::string why;
     if (device_count(&why) == 0) fail(SS_RUNTIME, "ss_batch_create: " + why);
+    std::vector<std::unique_ptr<Model>> models;
+    models.push_back(build_model(*ship, *sim, lds_limit()));
+    *out = make_batch(*opt, std::move(models), std::vector<int32_t>(static_cast<size_t>(opt->count), 0)).release();
+  });
+}
 
-    std::unique_ptr<ss_batch> B(new ss_batch);
-    ss_batch& b = *B;
-    b.opt = *opt;
-    if (b.opt.traced > b.opt.count) b.opt.traced = b.opt.count;
-    b.mixed = opt->numerics == SS_NUMERICS_PORTABLE32;
-    const ss_ship& S = *ship;
-    const ss_sim& M = *sim;
-    b.NN = S.nnode; b.NC = M.nconn; b.NG = M.ngroup; b.nmon = M.nmonitor; b.nmark = M.nmark; b.ncol = S.ncol; b.nseg = S.nseg;
-    b.evCap = b.nmon + b.nmark + 1;
-    for (int32_t g = 0; g < b.NG; ++g) b.maxGroup = std::max(b.maxGroup, M.gNodeCount[g]);
-    size_t members = 0;  // length of the group member list the groups index into
-    for (int32_t g = 0; g < b.NG; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
-    b.layout = lds_layout(b.NN, b.NC, b.NG, b.nmon, b.nmark, b.maxGroup, b.evCap, static_cast<int32_t>(members));
-    b.ldsBytes = static_cast<size_t>(b.layout.total) * sizeof(double);
-    hipDeviceProp_t p{};
-    hip_check(hipGetDeviceProperties(&p, 0), "hipGetDeviceProperties");
-    if (b.ldsBytes > p.sharedMemPerBlock)
-      fail(SS_RUNTIME, "ss_batch_create: this model needs " + std::to_string(b.ldsBytes) + " bytes of LDS per simulation, the device allows " +
-                           std::to_string(p.sharedMemPerBlock));
-    b.recSize = 10 + 5 * b.NN;
-    b.maxSamples = b.opt.traced > 0 ? static_cast<int32_t>(std::floor(b.opt.tMax / b.opt.sampleEvery)) + 2 : 0;
-
-    // ship: columns, and per segment the streams of its column
-    const size_t nc = static_cast<size_t>(S.ncol), ns = static_cast<size_t>(S.nseg), nn = static_cast<size_t>(S.nnode);
-    // records (x, y, bottom, top, footprint): 6 doubles (the last one 0) and 8 floats (the last three 0)
-    auto records = [](size_t n, auto&& value, std::vector<double>& d, std::vector<float>& f) {
-      d.assign(std::max<size_t>(n, 1) * 6, 0.0);
-      f.assign(std::max<size_t>(n, 1) * 8, 0.0f);
-      for (size_t i = 0; i < n; ++i)
-        for (int k = 0; k < 5; ++k) {
-          const double v = value(i, k);
-          d[6 * i + k] = v;
-          f[8 * i + k] = static_cast<float>(v);
-        }
-    };
-    std::vector<double> recD;
-    std::vector<float> recF;
-    records(nc, [&](size_t i, int k) { const double* a[5] = {S.x, S.y, S.zlo, S.zhi, S.dxdy}; return a[k][i]; }, recD, recF);
-    b.col.upload(recD); b.colF.upload(recF);
-    records(ns, [&](size_t j, int k) {
-      const size_t c = static_cast<size_t>(S.segCol[j]);
-      return k == 0 ? S.x[c] : k == 1 ? S.y[c] : k == 2 ? S.segA[j] : k == 3 ? S.segE[j] : S.dxdy[c];
-    }, recD, recF);
-    b.seg.upload(recD); b.segF.upload(recF);
-    b.segStart.upload(S.segStart, nn); b.segCount.upload(S.segCount, nn);
-    b.mu.upload(S.mu, nn); b.vmax.upload(S.vmax, nn); b.amin.upload(S.amin, nn); b.aov.upload(S.aov, nn);
-
-    // scenario: groups, connections, incidence (model.cpp:137-167), monitor lists, marks
-    const size_t ncn = static_cast<size_t>(M.nconn), ng = static_cast<size_t>(M.ngroup);
-    b.gStart.upload(M.gNodeStart, ng); b.gCount.upload(M.gNodeCount, ng); b.gNodes.upload(M.gNodes, members); b.gMode.upload(M.gMode, ng);
-    b.gpx.upload(M.gpx, ng); b.gpy.upload(M.gpy, ng); b.gpz.upload(M.gpz, ng);
-    std::vector<int4> conI(std::max<size_t>(ncn, 1));
-    std::vector<double4> conD(std::max<size_t>(ncn, 1));
-    for (size_t c = 0; c < ncn; ++c) {
-      conI[c] = make_int4(M.ca[c], M.cb[c], M.skipGroup[c], M.law[c]);
-      conD[c] = make_double4(M.cx[c], M.cy[c], M.cz[c], M.coef[c]);
-    }
-    b.conI.upload(conI); b.conD.upload(conD);
-    std::vector<int32_t> incCount(nn, 0), incStart(nn, 0), fill(nn, 0), sea;
-    for (size_t c = 0; c < ncn; ++c) {
-      if (M.ca[c] >= 0) incCount[M.ca[c]]++;
-      incCount[M.cb[c]]++;
-    }
-    int32_t total = 0;
-    for (size_t n = 0; n < nn; ++n) { incStart[n] = total; total += incCount[n]; }
-    std::vector<int4> inc(std::max<int32_t>(total, 1), make_int4(0, 0, 0, 0));
-    for (size_t c = 0; c < ncn; ++c) {
-      const int32_t ci = static_cast<int32_t>(c);
-      if (M.ca[c] >= 0) {
-        const int32_t n = M.ca[c];
-        inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 0);
-      } else {
-        sea.push_back(ci);
-      }
-      const int32_t n = M.cb[c];
-      inc[incStart[n] + fill[n]++] = make_int4(ci, M.ca[c], M.cb[c], 1);
-    }
-    b.incStart.upload(incStart); b.incCount.upload(incCount); b.inc.upload(inc); b.sea.upload(sea);
-    std::vector<int32_t> monCount(std::max<int32_t>(b.nmon, 0), 0), monStart(std::max<int32_t>(b.nmon, 0), 0), monConn;
-    for (int32_t m = 0; m < b.nmon; ++m) {
-      monStart[m] = static_cast<int32_t>(monConn.size());
-      for (size_t c = 0; c < ncn; ++c)
-        if (M.monitor[c] == m) monConn.push_back(static_cast<int32_t>(c));
-      monCount[m] = static_cast<int32_t>(monConn.size()) - monStart[m];
-    }
-    b.monStart.upload(monStart); b.monCount.upload(monCount); b.monConn.upload(monConn);
-    b.mkx.upload(M.mkx, M.nmark); b.mky.upload(M.mky, M.nmark); b.mkz.upload(M.mkz, M.nmark); b.mkWhen.upload(M.mkWhen, M.nmark);
-
-    DevNet& d = b.net;
-    d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
-    d.ng = M.ngroup; d.nMembers = static_cast<int32_t>(members); d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
-    d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
-    d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p;
-    d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.inc = (const SSG i4v*)b.inc.p;
-    d.seaCount = static_cast<int32_t>(sea.size()); d.sea = G(b.sea.p);
-    d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = G(b.monStart.p); d.monCount = G(b.monCount.p); d.monConn = G(b.monConn.p);
-    d.nmark = M.nmark; d.mkx = G(b.mkx.p); d.mky = G(b.mky.p); d.mkz = G(b.mkz.p); d.mkWhen = G(b.mkWhen.p);
-    d.hullTopBelow = M.founder[0]; d.pitchAbsAbove = M.founder[1]; d.buoyancyBelow = M.founder[2]; d.minT = M.founder[3];
-    const double* bd = M.body;
-    d.ms = bd[0]; d.g0x = bd[2]; d.g0y = bd[3]; d.g0z = bd[4]; d.XO = bd[5]; d.YO = bd[6]; d.mh0 = bd[7]; d.Ith0 = bd[8]; d.Iph0 = bd[9];
-    d.cDz = bd[10]; d.cDth = bd[11]; d.cDph = bd[12]; d.qz = bd[13]; d.qth = bd[14]; d.qph = bd[15];
-    d.xFP = M.readout[0]; d.xAP = M.readout[1];
-    d.maxGroup = b.maxGroup; d.evCap = b.evCap;
-
-    // per-instance tables, staged on the host until run()
-    const size_t count = static_cast<size_t>(b.opt.count);
-    b.hEn = repeat(M.en, ncn, count); b.hArea = repeat(M.area, ncn, count); b.hTOn = repeat(M.tOn, ncn, count);
-    b.hSt = repeat(M.state0, 7, count);
-    b.hVol = repeat(M.vol0, nn, count); b.hLevel = repeat(M.level0, nn, count);
-    b.hCx = repeat(M.cx0, nn, count); b.hCy = repeat(M.cy0, nn, count); b.hCz = repeat(M.cz0, nn, count);
-    *out = B.release();
+ss_status ss_batch_create_fleet(const ss_ship* ships, const ss_sim* sims, int32_t nmodels, const int32_t* instModel,
+                                const ss_batch_options* opt, ss_batch** out) {
+  return guarded([&] {
+    need(out != nullptr, "ss_batch_create_fleet: null output handle");
+    *out = nullptr;
+    need(ships != nullptr && sims != nullptr && instModel != nullptr, "ss_batch_create_fleet: null argument");
+    need(nmodels >= 1, "ss_batch_create_fleet: at least one model");
+    check_options(opt, "ss_batch_create_fleet");
+    need(opt->traced == 0 || nmodels == 1, "ss_batch_create_fleet: traces are recorded for single-model batches only");
+    for (int32_t k = 0; k < nmodels; ++k) validate(ships[k], sims[k]);
+    std::vector<int32_t> im(instModel, instModel + opt->count);
+    for (int32_t m : im) need(m >= 0 && m < nmodels, "ss_batch_create_fleet: instance model out of range");
+    std::string why;
+    if (device_count(&why) == 0) fail(SS_RUNTIME, "ss_batch_create_fleet: " + why);
+    const size_t limit = lds_limit();
+    std::vector<std::unique_ptr<Model>> models;
+    for (int32_t k = 0; k < nmodels; ++k) models.push_back(build_model(ships[k], sims[k], limit));
+    *out = make_batch(*opt, std::move(models), std::move(im)).release();
   });
 }
 
@@ -633,13 +770,13 @@ ss_status ss_batch_restore(ss_batch* b, int32_t instance, const double* state7, 
     if (b->ran) fail(SS_LOGIC, "ss_batch_restore: restore after run");
     check_instance(b, instance);
     need(state7 && vol && level && cx && cy && cz, "ss_batch_restore: null array");
-    const size_t i = static_cast<size_t>(instance), nn = static_cast<size_t>(b->NN);
+    const size_t i = static_cast<size_t>(instance), nn = static_cast<size_t>(b->model(instance).NN), o = static_cast<size_t>(b->nodeOff[i]);
     std::copy(state7, state7 + 7, b->hSt.begin() + i * 7);
-    std::copy(vol, vol + nn, b->hVol.begin() + i * nn);
-    std::copy(level, level + nn, b->hLevel.begin() + i * nn);
-    std::copy(cx, cx + nn, b->hCx.begin() + i * nn);
-    std::copy(cy, cy + nn, b->hCy.begin() + i * nn);
-    std::copy(cz, cz + nn, b->hCz.begin() + i * nn);
+    std::copy(vol, vol + nn, b->hVol.begin() + o);
+    std::copy(level, level + nn, b->hLevel.begin() + o);
+    std::copy(cx, cx + nn, b->hCx.begin() + o);
+    std::copy(cy, cy + nn, b->hCy.begin() + o);
+    std::copy(cz, cz + nn, b->hCz.begin() + o);
   });
 }
 
@@ -666,21 +803,28 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
   });
 }
 
+ss_status ss_batch_set_workgroups(ss_batch* b, int32_t workgroups) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_set_workgroups: null batch");
+    if (b->ran) fail(SS_LOGIC, "ss_batch_set_workgroups: set after run");
+    need(workgroups >= 0, "ss_batch_set_workgroups: negative count");
+    b->blocks = workgroups;
+  });
+}
+
 ss_status ss_batch_run(ss_batch* b, double* seconds) {
   return guarded([&] {
     need(b != nullptr, "ss_batch_run: null batch");
     if (b->ran) fail(SS_LOGIC, "ss_batch_run: run twice");
     b->ran = true;
     const size_t count = static_cast<size_t>(b->opt.count);
-    const size_t nn = static_cast<size_t>(b->NN);
     b->en.upload(b->hEn); b->area.upload(b->hArea); b->tOn.upload(b->hTOn);
     b->st.upload(b->hSt);
     b->out.fill(0.0, count * 7);
     b->vol.upload(b->hVol); b->level.upload(b->hLevel); b->ccx.upload(b->hCx); b->ccy.upload(b->hCy); b->ccz.upload(b->hCz);
-    const size_t ev = count * static_cast<size_t>(b->evCap);
-    b->evT.fill(0.0, ev); b->evId.fill(0, ev); b->evKind.fill(0, ev); b->evCount.fill(0, count);
-    b->overT.fill(-1.0, count * static_cast<size_t>(b->nmon));
-    b->markT.fill(-1.0, count * static_cast<size_t>(b->nmark));
+    b->evT.fill(0.0, b->evTotal); b->evId.fill(0, b->evTotal); b->evKind.fill(0, b->evTotal); b->evCount.fill(0, count);
+    b->overT.fill(-1.0, b->monTotal);
+    b->markT.fill(-1.0, b->markTotal);
     b->foundered.fill(0, count); b->founderT.fill(-1.0, count); b->steps.fill(0, count); b->final7.fill(0.0, count * 7);
     const size_t traced = static_cast<size_t>(b->opt.traced), rs = static_cast<size_t>(b->recSize);
     b->denseRec.alloc(traced * static_cast<size_t>(b->opt.denseSteps + 1) * rs);
@@ -692,32 +836,48 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
       b->curveCount.fill(0, count);
       b->nextCurve.fill(0.0, count);  // the first record is the initial state, as in the oracle's run
     }
-    (void)nn;
-    Params prm{};
-    prm.S = b->shipView();
-    prm.M = b->net;
-    prm.Bt = b->batchView();
-    prm.L = b->layout;
-    b->params.upload(&prm, 1);
+    b->dInstModel.upload(b->instModel); b->dOrder.upload(b->order);
+    b->dNodeOff.upload(b->nodeOff); b->dConnOff.upload(b->connOff); b->dEvOff.upload(b->evOff);
+    b->dMonOff.upload(b->monOff); b->dMarkOff.upload(b->markOff);
+    b->counter.fill(0, 1);
+    std::vector<ModelParams> mp;
+    for (const auto& m : b->models) mp.push_back(m->params());
+    b->dModels.upload(mp);
+    const DevBatch bv = b->batchView();
+    b->dBatch.upload(&bv, 1);
+
     const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
-    auto kernel = b->mixed ? run_chunk<true> : run_chunk<false>;
-    if (b->ldsBytes > 64 * 1024)
-      hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(b->ldsBytes)),
-                "hipFuncSetAttribute");
+    auto persistent = b->mixed ? run_batch<true, true> : run_batch<false, true>;
+    // the persistent grid: as many workgroups as are resident on the device at once (or the tuned count)
+    int perCU = 0, cus = 0;
+    for (auto k : {run_batch<true, true>, run_batch<false, true>, run_batch<true, false>, run_batch<false, false>})
+      if (b->ldsBytes > 64 * 1024)
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(b->ldsBytes)),
+                  "hipFuncSetAttribute");
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(persistent), kThreads, b->ldsBytes), "occupancy");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+    const int resident = std::max(1, perCU) * std::max(1, cus);
+    const int grid = static_cast<int>(std::min<long long>(static_cast<long long>(count), b->blocks > 0 ? b->blocks : resident));
+    // one workgroup per instance when all of them are resident at once; the refilling loop otherwise
+    const bool refill = grid < static_cast<int>(count);
+    auto kernel = refill ? persistent : (b->mixed ? run_batch<true, false> : run_batch<false, false>);
+    b->refill = refill;
+
     hipEvent_t e0, e1;
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     const auto t0 = std::chrono::steady_clock::now();
-    int first = 1;
     b->kernelMs = 0;
     b->launches = 0;
     std::vector<uint8_t> f(count);
     std::vector<double> st(count * 7);
     for (;;) {
+      hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, nullptr, b->counter.p);
+      hip_check(hipGetLastError(), "counter reset");
       hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
-      hipLaunchKernelGGL(kernel, dim3(b->opt.count), dim3(kThreads), b->ldsBytes, nullptr, static_cast<const Params*>(b->params.p), maxSteps,
-                         b->opt.tMax, first);
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), b->ldsBytes, nullptr, static_cast<const ModelParams*>(b->dModels.p),
+                         static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
       hip_check(hipGetLastError(), "kernel launch");
       hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
       hip_check(hipEventSynchronize(e1), "kernel execution");
@@ -725,7 +885,6 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
       hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
       b->kernelMs += ms;
       b->launches++;
-      first = 0;
       if (maxSteps == LLONG_MAX) break;
       b->foundered.download(f.data(), count);
       b->st.download(st.data(), count * 7);
@@ -762,7 +921,7 @@ ss_status ss_batch_result(const ss_batch* b, int32_t instance, ss_instance_resul
     need(b != nullptr, "ss_batch_result: null batch");
     if (!b->ran) fail(SS_LOGIC, "ss_batch_result: result before run");
     check_instance(b, instance);
-    const size_t i = static_cast<size_t>(instance), nn = static_cast<size_t>(b->NN);
+    const size_t i = static_cast<size_t>(instance), nn = static_cast<size_t>(b->model(instance).NN), o = static_cast<size_t>(b->nodeOff[i]);
     if (res) {
       res->foundered = b->rFoundered[i];
       res->founderT = b->rFounderT[i];
@@ -771,17 +930,17 @@ ss_status ss_batch_result(const ss_batch* b, int32_t instance, ss_instance_resul
       for (int k = 0; k < 7; ++k) { res->state[k] = b->rSt[i * 7 + k]; res->outputs[k] = b->rOut[i * 7 + k]; }
       res->events = b->rEvCount[i];
     }
-    if (vol) b->vol.download(vol, nn, i * nn);
-    if (level) b->level.download(level, nn, i * nn);
-    if (cx) b->ccx.download(cx, nn, i * nn);
-    if (cy) b->ccy.download(cy, nn, i * nn);
-    if (cz) b->ccz.download(cz, nn, i * nn);
+    if (vol) b->vol.download(vol, nn, o);
+    if (level) b->level.download(level, nn, o);
+    if (cx) b->ccx.download(cx, nn, o);
+    if (cy) b->ccy.download(cy, nn, o);
+    if (cz) b->ccz.download(cz, nn, o);
     if (events && evCap > 0) {
       const int32_t n = std::min(b->rEvCount[i], evCap);
       std::vector<double> t(n);
       std::vector<int32_t> id(n);
       std::vector<uint8_t> kind(n);
-      const size_t off = i * static_cast<size_t>(b->evCap);
+      const size_t off = static_cast<size_t>(b->evOff[i]);
       b->evT.download(t.data(), n, off);
       b->evId.download(id.data(), n, off);
       b->evKind.download(kind.data(), n, off);
@@ -862,7 +1021,8 @@ static ss_status set_one(ss_batch* b, int32_t instance, int32_t conn, const char
     need(b != nullptr, std::string(what) + ": null batch");
     if (b->ran) fail(SS_LOGIC, std::string(what) + ": set after run");
     check_instance(b, instance);
-    if (conn < 0 || conn >= b->NC) fail(SS_OUT_OF_RANGE, std::string(what) + ": connection " + std::to_string(conn) + " out of range");
-    f(static_cast<size_t>(instance) * b->NC + static_cast<size_t>(conn));
+    if (conn < 0 || conn >= b->model(instance).NC)
+      fail(SS_OUT_OF_RANGE, std::string(what) + ": connection " + std::to_string(conn) + " out of range");
+    f(static_cast<size_t>(b->connOff[static_cast<size_t>(instance)]) + static_cast<size_t>(conn));
   });
 }

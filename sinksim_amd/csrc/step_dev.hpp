@@ -111,14 +111,19 @@ struct DevNet {
   int32_t maxGroup, evCap;
 };
 
-// Per-instance global arrays of the batch.
+// Per-instance global arrays of the batch. The instances of one batch may run different models (hulls and
+// scenarios), so an instance's rows start at its own offsets; work items are handed out in `order` through the
+// counter, which the host resets before each launch.
 struct DevBatch {
   int32_t count;
+  const SSG int32_t *instModel, *order;                   // per instance its model; per work item its instance
+  const SSG int32_t *nodeOff, *connOff, *evOff, *monOff, *markOff;  // per instance, in elements
+  SSG int32_t* counter;
   gcb* en;
-  gcd *area, *tOn;                      // [count * nc]
+  gcd *area, *tOn;                      // rows of the instance's connection count, at connOff
   SSG double* st;                       // [count * 7]: t zO pitch roll vz wth wph
   SSG double* out;                      // [count * 7]: inflow Vb hullTop mw bx by bz
-  SSG double *vol, *level, *cx, *cy, *cz;  // [count * nn]
+  SSG double *vol, *level, *cx, *cy, *cz;  // rows of the instance's node count, at nodeOff
   SSG double* evT;
   SSG int32_t* evId;
   SSG uint8_t* evKind;
@@ -151,14 +156,15 @@ struct LdsLayout {
   int32_t total;                // doubles
 };
 
-// Everything a launch needs, in constant memory; device code reads fields on demand with scalar loads.
-struct Params {
+// One model (hull, scenario and its LDS layout) in constant memory; device code reads fields on demand with scalar
+// loads. A batch holds one record per model.
+struct ModelParams {
   DevShip S;
   DevNet M;
-  DevBatch Bt;
   LdsLayout L;
 };
-using PC = const SSC Params;
+using PC = const SSC ModelParams;
+using BC = const SSC DevBatch;
 
 // frame slots: R00 R01 R02 R10 R11 R12 R20 R21 R22 tx ty tz cth iR, then the f32 frame R20 R21 R22 tz iR
 enum : int { F_R00 = 0, F_R01, F_R02, F_R10, F_R11, F_R12, F_R20, F_R21, F_R22, F_TX, F_TY, F_TZ, F_CTH, F_IR, F_F32, F_COUNT = F_F32 + 3 };
@@ -922,19 +928,20 @@ __device__ inline void node_update(PC& P, const Blk& B, int n) {
 
 // ------------------------------------------------------------------------------------------- readouts
 
-// readouts.hpp:175-191 (7 values: t trim list water(t) inflow(t/min) draftF draftA), one thread
-__device__ __noinline__ void readouts7(PC& P, const Blk& B, SSG double* r) {
-  const double t = B.scal[S_T], zO = B.scal[S_ZO], pitch = B.scal[S_PITCH], roll = B.scal[S_ROLL];
+// readouts.hpp:175-191 (7 values: t trim list water(t) inflow(t/min) draftF draftA), one thread; it takes the LDS
+// arrays it reads rather than the block's view, which a call would have to copy to scratch
+__device__ __noinline__ void readouts7(PC& P, const SSL double* scal, const SSL double* vol, SSG double* r) {
+  const double t = scal[S_T], zO = scal[S_ZO], pitch = scal[S_PITCH], roll = scal[S_ROLL];
   const double cth = trig(pitch, 1), sth = trig(pitch, 0), cph = trig(roll, 1), sph = trig(roll, 0);
   const double R20 = -sth, R21 = cth * sph, R22 = cth * cph;
   const double tz = zO - (R20 * P.M.g0x + R21 * P.M.g0y + R22 * P.M.g0z);
   double water = 0;
-  for (int n = 0; n < P.S.nn; ++n) water += B.vol[n];
+  for (int n = 0; n < P.S.nn; ++n) water += vol[n];
   r[0] = t;
   r[1] = pitch * 180 / kPi;
   r[2] = roll * 180 / kPi;
   r[3] = water * P.M.rho / 1000;
-  r[4] = B.scal[O_INFLOW] * P.M.rho / 1000 * 60;
+  r[4] = scal[O_INFLOW] * P.M.rho / 1000 * 60;
   r[5] = -(R20 * P.M.xFP + R21 * 0.0 + R22 * 0.0 + tz);
   r[6] = -(R20 * P.M.xAP + R21 * 0.0 + R22 * 0.0 + tz);
 }
@@ -1089,46 +1096,78 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
   pose_wave0(P, B, lane, zO, pitch, roll);
 }
 
-// One step. On entry the frame slots hold the pose of the current state (written by the previous step's tail or
-// at load time); every thread may read them. Ends with a barrier.
-template <bool Mixed>
-__device__ void device_step(PC& P, const Blk& B, int tid) {
-  const int NN = P.S.nn, NC = P.M.nc;
+// The model's address made opaque to the compiler: the fields a phase reads are then loaded (scalar loads, from the
+// constant cache) within that phase instead of being held in scalar registers across the whole step loop, where
+// all of them together exceed the register file and spill.
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v))));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(static_cast<int>(static_cast<uint32_t>(v >> 32))));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+__device__ __forceinline__ PC& model_at(uint64_t& pa) {
+  asm volatile("" : "+s"(pa));
+  return *(PC*)pa;
+}
 
+// One step of the model at address pa. On entry the frame slots hold the pose of the current state (written by the
+// previous step's tail or at load time); every thread may read them. Ends with a barrier.
+template <bool Mixed>
+__device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
   // (a) buoyancy partials, then (b) the free surfaces: logical group `unit` solves groups unit, unit + 8, ...
-  buoyancy_partials<Mixed>(P, B, tid);
-  solve_levels<Mixed>(P, B, tid);
+  {
+    PC& P = model_at(pa);
+    buoyancy_partials<Mixed>(P, carve(lds, P), tid);
+  }
+  {
+    PC& P = model_at(pa);
+    solve_levels<Mixed>(P, carve(lds, P), tid);
+  }
   __syncthreads();
 
   // (c) heads and degrees per node
-  for (int n = tid; n < NN; n += kThreads) node_heads(P, B, n);
+  {
+    PC& P = model_at(pa);
+    const Blk B = carve(lds, P);
+    for (int n = tid; n < P.S.nn; n += kThreads) node_heads(P, B, n);
+  }
   __syncthreads();
 
   // (c) flows per connection
-  const double t = B.scal[S_T];
-  for (int c = tid; c < NC; c += kThreads) B.dV[c] = connection_flow(P, B.frame, t, B, c);
+  {
+    PC& P = model_at(pa);
+    const Blk B = carve(lds, P);
+    const double t = B.scal[S_T];
+    for (int c = tid; c < P.M.nc; c += kThreads) B.dV[c] = connection_flow(P, B.frame, t, B, c);
+  }
   __syncthreads();
 
   // (c) gathers, volume updates and load terms per node; monitor sums (events.hpp:73-75, per monitor in increasing
   // connection order); the sea inflow (flows.hpp:288-292) on the last thread
-  for (int n = tid; n < NN; n += kThreads) node_update(P, B, n);
-  for (int m = tid; m < P.M.nmon; m += kThreads) {
-    double o = 0;
-    const int s = P.M.monStart[m], e = s + P.M.monCount[m];
-    for (int k = s; k < e; ++k) {
-      const double qc = B.q[P.M.monConn[k]];
-      if (qc > 0) o += qc;
+  {
+    PC& P = model_at(pa);
+    const Blk B = carve(lds, P);
+    for (int n = tid; n < P.S.nn; n += kThreads) node_update(P, B, n);
+    for (int m = tid; m < P.M.nmon; m += kThreads) {
+      double o = 0;
+      const int s = P.M.monStart[m], e = s + P.M.monCount[m];
+      for (int k = s; k < e; ++k) {
+        const double qc = B.q[P.M.monConn[k]];
+        if (qc > 0) o += qc;
+      }
+      B.over[m] = o;
     }
-    B.over[m] = o;
-  }
-  if (tid == kThreads - 1) {
-    double s = 0;
-    for (int k = 0; k < P.M.seaCount; ++k) s += B.dV[P.M.sea[k]];
-    B.scal[S_SEA] = s;
+    if (tid == kThreads - 1) {
+      double s = 0;
+      for (int k = 0; k < P.M.seaCount; ++k) s += B.dV[P.M.sea[k]];
+      B.scal[S_SEA] = s;
+    }
   }
   __syncthreads();
 
-  if (tid < 64) step_tail(P, B, tid);
+  {
+    PC& P = model_at(pa);  // outside the branch: the address stays uniform
+    if (tid < 64) step_tail(P, carve(lds, P), tid);
+  }
   __syncthreads();
 }
 

@@ -1,0 +1,90 @@
+"""The uneven persistent batch (a fleet): instances of different hulls, space and connection counts in one launch,
+handed out by a work counter and refilled as simulations end.
+
+- Every instance of a fleet of seeded synthetic hulls (1,000-4,000 columns, 8-48 spaces) and titanic64 equals the
+  oracle's CPU run of that instance, in both numerics, with the grid forced down to a few workgroups so that each
+  workgroup runs many instances in turn (the refill path), and at the default grid.
+- A fleet of one model equals the plain batch of that model.
+"""
+import zlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _models():
+    from sinksim_amd.data_paths import titanic
+    from sinksim_amd.synth import synthetic_hull
+    models = [synthetic_hull(seed, columns=cols, spaces=sp) for seed, cols, sp in
+              [(3, 1000, 8), (11, 2500, 24), (17, 4000, 48), (23, 1800, 12)]]
+    models.append(titanic("titanic"))
+    return models
+
+
+def _tables(models, inst_model, seed):
+    from sinksim_amd.workloads import random_tables
+    en, area, tOn = [], [], []
+    for i, m in enumerate(inst_model):
+        e, a, t = random_tables(models[m][1], 1, seed=seed + i, activation_share=0.1)
+        en.append(e[0]); area.append(a[0]); tOn.append(t[0])
+    return en, area, tOn
+
+
+def _check_fleet(models, inst_model, numerics, t_max, every, workgroups):
+    import pyoracle
+    from sinksim_amd.batch import Batch, BatchOptions
+    en, area, tOn = _tables(models, inst_model, seed=zlib.crc32(numerics.encode()))
+    b = Batch.fleet(models, inst_model, BatchOptions(count=len(inst_model), mixed_precision=numerics == "portable32",
+                                                     traced_instances=0, curve_every=every, t_max=t_max))
+    b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
+    b.set_workgroups(workgroups)
+    b.run()
+    for i, m in enumerate(inst_model):
+        ship, sim = models[m]
+        cpu = pyoracle.sweep(ship, sim, numerics, en[i][None], area[i][None], tOn[i][None], t_max, every, threads=1)[0]
+        r = b.result(i)
+        where = f"{numerics} instance {i} (model {m}, {ship.columns} columns, {ship.nodes} spaces)"
+        assert r.steps == cpu["steps"], where
+        assert r.foundered == cpu["foundered"] and r.founderT == cpu["founderT"] and r.tEnd == cpu["tEnd"], where
+        assert r.events == cpu["events"], where
+        g = b.curve(i)
+        assert g.shape == cpu["curve"].shape, where
+        bad = np.argwhere(g != cpu["curve"])
+        assert bad.size == 0, f"{where}: curve record {bad[0][0]} value {bad[0][1]}"
+    assert b.total_steps() == int(b.results()["steps"].sum())
+    b.close()
+
+
+@pytest.mark.parametrize("numerics", ["portable", "portable32"])
+def test_fleet_matches_oracle_with_refill(numerics):
+    models = _models()
+    inst_model = np.array([4, 0, 1, 2, 3, 0, 2, 1, 3, 4, 0, 2], np.int32)
+    _check_fleet(models, inst_model, numerics, t_max=1200.0, every=30.0, workgroups=3)
+
+
+def test_fleet_matches_oracle_full_grid():
+    models = _models()
+    inst_model = np.array([2, 1, 0, 3, 4, 2, 0], np.int32)
+    _check_fleet(models, inst_model, "portable", t_max=900.0, every=60.0, workgroups=0)
+
+
+def test_single_model_fleet_equals_batch():
+    from sinksim_amd.batch import Batch, BatchOptions
+    from sinksim_amd.data_paths import titanic
+    from sinksim_amd.workloads import random_tables
+    ship, sim = titanic("titanic")
+    en, area, tOn = random_tables(sim, 16, seed=5)
+    opt = BatchOptions(count=16, mixed_precision=False, traced_instances=0, curve_every=60.0, t_max=1800.0)
+    a = Batch(ship, sim, opt)
+    a.set_connections(en, area, tOn)
+    a.run()
+    f = Batch.fleet([(ship, sim)], np.zeros(16, np.int32), opt)
+    f.set_connections(en, area, tOn)
+    f.set_workgroups(5)
+    f.run()
+    for i in range(16):
+        assert np.array_equal(a.curve(i), f.curve(i)), i
+        ra, rf = a.result(i), f.result(i)
+        assert ra.steps == rf.steps and ra.events == rf.events and np.array_equal(ra.vol, rf.vol), i

@@ -42,7 +42,11 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--count", type=int, default=1024)
+    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven"],
+                    help="titanic: the 1,024-instance titanic64 batch (the headline); uneven: the persistent batch of "
+                         "seeded synthetic hulls of uneven size")
+    ap.add_argument("--count", type=int, default=None, help="instances (default 1024 titanic, 4096 uneven)")
+    ap.add_argument("--tmax", type=float, default=None, help="s (default 4 h titanic, 2 h uneven)")
     ap.add_argument("--numerics", default="portable", choices=["portable", "portable32"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=48, help="instances in the CPU-baseline sample")
@@ -97,19 +101,80 @@ def _device() -> str:
     return f"{d['name'] or 'AMD GPU'} ({d['arch']}, {d['computeUnits']} CUs)"
 
 
+def bench_uneven(args):
+    """The uneven persistent batch: `count` seeded synthetic hulls, one instance each, in one refilling launch.
+    Roofline: HBM, with 40 bytes per column (x, y, zlo, zhi, dxdy as doubles) and 20 per segment (bottom, top and
+    column index) per simulation step as the algorithmic geometry traffic; the hulls together are far larger than
+    the caches, so each step streams its hull."""
+    import numpy as np
+    from sinksim_amd.batch import Batch, BatchOptions
+    from sinksim_amd.workloads import uneven_fleet
+    count = args.count or 4096
+    t_max = args.tmax or 2 * 3600.0
+    mixed = args.numerics == "portable32"
+    t0 = time.perf_counter()
+    models, inst_model = uneven_fleet(count)
+    print(f"{count} hulls generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    geo = np.array([40.0 * m[0].columns + (20.0 if not mixed else 12.0) * m[0].segments for m in models])
+
+    def one_run():
+        b = Batch.fleet(models, inst_model, BatchOptions(count=count, mixed_precision=mixed, traced_instances=0,
+                                                         curve_every=30.0, t_max=t_max))
+        secs = b.run()
+        steps = b.results()["steps"]
+        kms, launches = b.kernel_time()
+        b.close()
+        return secs, steps, kms, launches
+
+    for w in range(args.warmup):
+        secs, steps, _, _ = one_run()
+        print(f"warmup {w}: {int(steps.sum())} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+    tot_s = tot_kms = 0.0
+    tot_steps = tot_bytes = 0.0
+    tot_launch = 0
+    for k in range(args.steps):
+        secs, steps, kms, launches = one_run()
+        tot_s += secs; tot_kms += kms; tot_launch += launches
+        tot_steps += float(steps.sum()); tot_bytes += float((steps * geo[inst_model]).sum())
+        print(f"step {k}: {int(steps.sum())} steps in {secs:.3f} s ({steps.sum() / secs:.0f} steps/s)", file=sys.stderr, flush=True)
+    value = tot_steps / tot_s
+    achieved = tot_bytes / (tot_kms * 1e-3) / 1e9
+    cols = np.array([m[0].columns for m in models]); sp = np.array([m[0].nodes for m in models])
+    nc = np.array([m[1].connections for m in models])
+    out = {
+        "metric": "aggregate sim-steps/s, uneven persistent batch", "value": round(value, 1), "unit": "sim-steps/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * tot_s / args.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64" if not mixed else "f32+f64",
+        "data": "synthetic (seeded synthetic hulls)",
+        "config": {"workload": f"{count} seeded synthetic hulls ({int(cols.min())}-{int(cols.max())} columns, {int(sp.min())}-{int(sp.max())} "
+                               f"spaces, {int(nc.min())}-{int(nc.max())} connections), one instance each, {args.numerics} "
+                               f"numerics, to founder or {t_max / 3600:g} h, curves every 30 s, refilled workgroups",
+                   "instances": count, "numerics": args.numerics, "sim_steps_per_bench_step": int(tot_steps // args.steps),
+                   "device": _device()},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+                     "frac": round(achieved / 8000.0, 5), "traffic": None,
+                     "kernel": "run_batch<%s, true>" % ("true" if mixed else "false"), "kernel_ms": round(tot_kms / max(tot_launch, 1), 3)},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); instances are independent, so N GPUs "
               "would each run count/N of them", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven":
+        bench_uneven(args)
+        return
+    args.count = args.count or 1024
     from sinksim_amd import sweep as S
     from sinksim_amd.batch import Batch, BatchOptions
     from sinksim_amd.data_paths import titanic
     from sinksim_amd.workloads import titanic_batch_sweep
 
     ship, sim = titanic("titanic")
-    sw = titanic_batch_sweep(sim, count=args.count)
+    sw = titanic_batch_sweep(sim, count=args.count, t_max=args.tmax or 4 * 3600.0)
     en, area, tOn = S.connection_tables(sw, sim)
     mixed = args.numerics == "portable32"
 

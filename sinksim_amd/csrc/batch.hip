@@ -863,9 +863,31 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
     auto kernel = refill ? persistent : (b->mixed ? run_batch<true, false> : run_batch<false, false>);
     b->refill = refill;
 
-    hipEvent_t e0, e1;
+    // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
+    // replayed for every chunk: identical launches, one submission each.
+    hipStream_t stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    struct Release {
+      hipStream_t& s; hipGraph_t& g; hipGraphExec_t& x; hipEvent_t& a; hipEvent_t& b;
+      ~Release() {
+        if (x) (void)hipGraphExecDestroy(x);
+        if (g) (void)hipGraphDestroy(g);
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        if (s) (void)hipStreamDestroy(s);
+      }
+    } release{stream, graph, exec, e0, e1};
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
     hip_check(hipEventCreate(&e0), "hipEventCreate");
     hip_check(hipEventCreate(&e1), "hipEventCreate");
+    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, b->counter.p);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
+                       static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
+    hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     const auto t0 = std::chrono::steady_clock::now();
     b->kernelMs = 0;
@@ -873,13 +895,9 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
     std::vector<uint8_t> f(count);
     std::vector<double> st(count * 7);
     for (;;) {
-      hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, nullptr, b->counter.p);
-      hip_check(hipGetLastError(), "counter reset");
-      hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
-      hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), b->ldsBytes, nullptr, static_cast<const ModelParams*>(b->dModels.p),
-                         static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
-      hip_check(hipGetLastError(), "kernel launch");
-      hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+      hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+      hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+      hip_check(hipEventRecord(e1, stream), "hipEventRecord");
       hip_check(hipEventSynchronize(e1), "kernel execution");
       float ms = 0;
       hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
@@ -894,8 +912,6 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
       if (!active) break;
     }
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     read_back(b);
     if (seconds) *seconds = secs;
   });

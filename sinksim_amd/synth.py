@@ -79,17 +79,21 @@ def box_barge(L=100.0, B=20.0, H=10.0, T=4.0, dx=5.0, dy=1.0, parts=2, shared_su
 KIND_NAMES = ["door", "over", "down", "xlow", "xmerge", "xup", "top", "breach", "open"]
 
 
-def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = None, rho: float = 1025.0, g: float = 9.81,
-                   dt: float = 0.25):
+def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = None, connections: int | None = None,
+                   rho: float = 1025.0, g: float = 9.81, dt: float = 0.25, hashed: bool = False):
     """A seeded ship and flooding scenario of the kind the uneven batch runs: length 100-300 m, parabolic sections,
-    `columns` (1,000-16,000) hull columns, `spaces` (8-128) nodes, a network of doors, bulkhead-top weirs, deck
-    openings, cross-flow ducts, deck-edge weirs to the sea and 1-6 breaches of 0.01-1 m^2 below the waterline.
+    `columns` (1,000-16,000) hull columns, `spaces` (8-128) nodes, `connections` (20-600; at least the structural
+    minimum) doors, bulkhead-top weirs, deck openings, cross-flow ducts, deck-edge weirs to the sea and 1-6 breaches
+    of 0.01-1 m^2 below the waterline. Unset sizes are drawn from the seed (connections log-uniform).
 
     Compartments are bounded by transverse bulkheads; each is split port/starboard and lower/upper at a flat deck, so
     a compartment holds four nodes. Lower pairs share a surface always, above a threshold point, never, or are
-    solved singly; some compartments join all four nodes in one group (a shared surface over N = 4 members). The
-    body floats level at its design draft: mass = displaced mass there, centre of gravity above the centre of
-    buoyancy with a positive metacentric height, added mass and damping as the reference's box-barge rig."""
+    solved singly; some compartments join all four nodes in one group (a shared surface over N = 4 members). Extra
+    connections beyond the structural ones are further doors between neighbouring compartments (closed with
+    probability 0.7), deck openings and upper cross-flow ducts. The body floats level at its design draft: mass =
+    displaced mass there, centre of gravity above the centre of buoyancy with a positive metacentric height, added
+    mass and damping as the reference's box-barge rig. `hashed` fills the FNV hashes of the in-memory objects
+    (write_compiled computes its own)."""
     rng = np.random.default_rng(seed)
     L = float(rng.uniform(100.0, 300.0))
     B = L / float(rng.uniform(7.5, 10.0))
@@ -97,6 +101,7 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
     T = D * float(rng.uniform(0.45, 0.62))
     ncol_target = int(columns if columns is not None else rng.integers(1000, 16001))
     nsp_target = int(spaces if spaces is not None else rng.integers(8, 129))
+    nconn_target = int(connections if connections is not None else round(math.exp(rng.uniform(math.log(20.0), math.log(600.0)))))
     ncomp = max(2, nsp_target // 4)
     # grid: the plan fills about 0.78 of the bounding box
     cells = ncol_target / 0.78
@@ -106,60 +111,55 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
     zdeck = D * float(rng.uniform(0.4, 0.55))
 
     def half_breadth(x):
-        return 0.5 * B * max(0.0, 1.0 - abs(2.0 * x / L) ** 2.4) ** 0.5
+        return 0.5 * B * np.sqrt(np.maximum(0.0, 1.0 - np.abs(2.0 * np.asarray(x) / L) ** 2.4))
 
-    def keel_rise(x):
-        return 0.35 * D * abs(2.0 * x / L) ** 5
-
-    cols = []  # (x, y, dxdy, zlo, zhi, compartment, side)
+    # columns, row by row along the length (x outer, y inner)
+    X, Y = np.meshgrid(-L / 2 + (np.arange(nx) + 0.5) * dx, -B / 2 + (np.arange(ny) + 0.5) * dy, indexing="ij")
+    X, Y = X.reshape(-1), Y.reshape(-1)
+    hb = half_breadth(X)
+    keep = (np.abs(Y) < hb) & (hb > 0)
+    rise = 0.35 * D * np.abs(2.0 * X / L) ** 5
+    with np.errstate(divide="ignore", invalid="ignore"):
+        zlo = rise + (D - rise) * (np.abs(Y) / np.where(hb > 0, hb, 1.0)) ** 2
+    zhi = D + 0.02 * L * (2.0 * X / L) ** 2
+    keep &= (zhi - zlo) >= 0.05
     bulkheads = np.linspace(-L / 2, L / 2, ncomp + 1)
-    for i in range(nx):
-        x = -L / 2 + (i + 0.5) * dx
-        hb = half_breadth(x)
-        comp = min(ncomp - 1, int(np.searchsorted(bulkheads, x, side="right") - 1))
-        for j in range(ny):
-            y = -B / 2 + (j + 0.5) * dy
-            if abs(y) >= hb or hb <= 0:
-                continue
-            rise = keel_rise(x)
-            zlo = rise + (D - rise) * (abs(y) / hb) ** 2
-            zhi = D + 0.02 * L * (2.0 * x / L) ** 2
-            if zhi - zlo < 0.05:
-                continue
-            cols.append((x, y, dx * dy, zlo, zhi, comp, 1 if y >= 0 else 0))
-    ncol = len(cols)
-    cx_ = np.array([c[0] for c in cols]); cy_ = np.array([c[1] for c in cols]); cA = np.array([c[2] for c in cols])
-    czlo = np.array([c[3] for c in cols]); czhi = np.array([c[4] for c in cols])
-    ccomp = np.array([c[5] for c in cols]); cside = np.array([c[6] for c in cols])
+    cx_, cy_, czlo, czhi = X[keep], Y[keep], zlo[keep], zhi[keep]
+    cA = np.full(cx_.shape, dx * dy)
+    ccomp = np.minimum(ncomp - 1, np.searchsorted(bulkheads, cx_, side="right") - 1)
+    cside = (cy_ >= 0).astype(np.int64)
+    ncol = cx_.size
 
     # nodes: compartment k -> (P lower, P upper, S lower, S upper); side 1 = port (y >= 0)
     node_of = {}
-    labels, mu, vmax, amin, aov, segStart, segCount, segCol, segA, segE, ncent = [], [], [], [], [], [], [], [], [], [], []
+    labels, mu, vmax, amin, aov, segStart, segCount, ncent = [], [], [], [], [], [], [], []
+    segColL, segAL, segEL = [], [], []
+    nseg = 0
     for k in range(ncomp):
         for side in (1, 0):
+            cols = np.flatnonzero((ccomp == k) & (cside == side))
             for upper in (False, True):
                 n = len(labels)
                 node_of[(k, side, upper)] = n
                 labels.append(f"C{k}/{'P' if side else 'S'}/{'upper' if upper else 'lower'}")
-                segStart.append(len(segCol))
                 perm = float(rng.uniform(0.85, 0.98))
-                vol = footprint = 0.0
-                mom = np.zeros(3)
-                for c in np.flatnonzero((ccomp == k) & (cside == side)):
-                    a, e = (max(czlo[c], zdeck), czhi[c]) if upper else (czlo[c], min(czhi[c], zdeck))
-                    if e - a <= 1e-3:
-                        continue
-                    segCol.append(int(c)); segA.append(float(a)); segE.append(float(e))
-                    v = cA[c] * (e - a)
-                    vol += v
-                    footprint += cA[c]
-                    mom += v * np.array([cx_[c], cy_[c], 0.5 * (a + e)])
-                segCount.append(len(segCol) - segStart[-1])
-                if segCount[-1] == 0:
+                if upper:
+                    a, e = np.maximum(czlo[cols], zdeck), czhi[cols]
+                else:
+                    a, e = czlo[cols], np.minimum(czhi[cols], zdeck)
+                ok = (e - a) > 1e-3
+                c, a, e = cols[ok], a[ok], e[ok]
+                if c.size == 0:
                     raise RuntimeError("synthetic hull produced an empty node; change the seed")
+                segStart.append(nseg); segCount.append(int(c.size)); nseg += int(c.size)
+                segColL.append(c); segAL.append(a); segEL.append(e)
+                v = cA[c] * (e - a)
+                vol, footprint = float(v.sum()), float(cA[c].sum())
                 mu.append(perm); vmax.append(perm * vol); amin.append(max(1.0, 0.01 * footprint))
-                aov.append(max(1.0, 0.03 * footprint)); ncent.append(mom / vol)
+                aov.append(max(1.0, 0.03 * footprint))
+                ncent.append(np.array([(v * cx_[c]).sum(), (v * cy_[c]).sum(), (v * 0.5 * (a + e)).sum()]) / vol)
     nn = len(labels)
+    segCol, segA, segE = np.concatenate(segColL), np.concatenate(segAL), np.concatenate(segEL)
 
     # groups
     gS, gC, gN, gM, gpx, gpy, gpz = [], [], [], [], [], [], []
@@ -169,13 +169,13 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
         gpx.append(p[0]); gpy.append(p[1]); gpz.append(p[2])
         return len(gC) - 1
 
-    lower_group = {}
+    lower_group, upper_group = {}, {}
     for k in range(ncomp):
         xc = 0.5 * (bulkheads[k] + bulkheads[k + 1])
         lp, up, ls, us = node_of[(k, 1, False)], node_of[(k, 1, True)], node_of[(k, 0, False)], node_of[(k, 0, True)]
         r = rng.uniform()
         if r < 0.1:
-            lower_group[k] = group([lp, ls, up, us], 1)
+            lower_group[k] = upper_group[k] = group([lp, ls, up, us], 1)
             continue
         if r < 0.55:
             lower_group[k] = group([lp, ls], 1)
@@ -185,16 +185,20 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
             lower_group[k] = group([lp, ls], 3)
         else:
             group([lp], 0); group([ls], 0)
-        group([up, us], 1)
+        upper_group[k] = group([up, us], 1)
 
     # connections: (a, b, law, x, y, z, area/width, coef, kind, idx, en, tOn, skipGroup, monitor)
     conns = []
     K = {name: i for i, name in enumerate(KIND_NAMES)}
+
+    def pair_skip(gr):  # a cross connection inside a group is skipped while that group is merged
+        return gr if gr is not None and gr >= 0 else -1
+
     for k in range(ncomp):
         xc = 0.5 * (bulkheads[k] + bulkheads[k + 1])
-        hb = half_breadth(xc)
+        hb = float(half_breadth(xc))
         lp, up, ls, us = node_of[(k, 1, False)], node_of[(k, 1, True)], node_of[(k, 0, False)], node_of[(k, 0, True)]
-        skip = lower_group.get(k, -1) if (lower_group.get(k, -1) >= 0 and gC[lower_group[k]] == 2) else -1
+        skip = pair_skip(lower_group.get(k))
         conns.append((lp, ls, 0, xc, 0.0, 0.6, float(rng.uniform(0.5, 3.0)), 0.6, K["xlow"], k, 1, 0.0, skip, -1))
         for side, lo_, up_ in ((1, lp, up), (0, ls, us)):
             ysgn = 1.0 if side else -1.0
@@ -203,7 +207,7 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
                           1, 0.0, -1, -1))
     for k in range(ncomp - 1):
         xb = float(bulkheads[k + 1])
-        hb = half_breadth(xb)
+        hb = float(half_breadth(xb))
         top = zdeck + (D - zdeck) * float(rng.uniform(0.3, 0.8))
         for side in (1, 0):
             ysgn = 1.0 if side else -1.0
@@ -213,13 +217,36 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
                           int(rng.uniform() < 0.3), 0.0, -1, -1))
             conns.append((a_up, b_up, 1, xb, ysgn * 0.5 * hb, top, float(rng.uniform(2.0, 10.0)), 0.6, K["over"], k, 1, 0.0, -1, k))
     nbreach = int(rng.integers(1, 7))
+    # extra connections up to the target count (the breaches come last)
+    extra = max(0, nconn_target - len(conns) - nbreach)
+    for m in range(extra):
+        kind = m % 3
+        if kind == 0 and ncomp > 1:  # a further door between neighbouring lower spaces
+            k = int(rng.integers(ncomp - 1))
+            side = int(rng.integers(2))
+            xb = float(bulkheads[k + 1])
+            hb = float(half_breadth(xb))
+            conns.append((node_of[(k, side, False)], node_of[(k + 1, side, False)], 0, xb, (1 if side else -1) * float(rng.uniform(0.1, 0.8)) * hb,
+                          float(rng.uniform(0.5, max(0.6, zdeck - 0.5))), float(rng.uniform(0.5, 4.0)), 0.6, K["door"], k + 2,
+                          int(rng.uniform() < 0.3), 0.0, -1, -1))
+        elif kind == 1:  # a further deck opening
+            k = int(rng.integers(ncomp))
+            side = int(rng.integers(2))
+            xc = float(rng.uniform(bulkheads[k], bulkheads[k + 1]))
+            conns.append((node_of[(k, side, True)], node_of[(k, side, False)], 0, xc, (1 if side else -1) * 0.3 * float(half_breadth(xc)), zdeck,
+                          float(rng.uniform(0.2, 2.0)), 0.6, K["down"], k, 1, 0.0, -1, -1))
+        else:  # an upper cross-flow duct
+            k = int(rng.integers(ncomp))
+            xc = float(rng.uniform(bulkheads[k], bulkheads[k + 1]))
+            conns.append((node_of[(k, 1, True)], node_of[(k, 0, True)], 0, xc, 0.0, zdeck + 0.3 * (D - zdeck), float(rng.uniform(0.5, 3.0)), 0.6,
+                          K["xup"], k, 1, 0.0, pair_skip(upper_group.get(k)), -1))
     for m in range(nbreach):
         k = int(rng.integers(max(1, ncomp // 3))) if m else 0
         side = int(rng.integers(2))
         xc = float(rng.uniform(bulkheads[k], bulkheads[k + 1]))
         z = float(max(0.3, T - rng.uniform(0.5, 0.8 * T)))
         tOn = 0.0 if m == 0 or rng.uniform() < 0.6 else float(rng.uniform(0, 1800))
-        conns.append((-1, node_of[(k, side, False)], 0, xc, (1 if side else -1) * 0.9 * half_breadth(xc), z, float(rng.uniform(0.01, 1.0)),
+        conns.append((-1, node_of[(k, side, False)], 0, xc, (1 if side else -1) * 0.9 * float(half_breadth(xc)), z, float(rng.uniform(0.01, 1.0)),
                       0.6, K["breach"], m, 1, tOn, -1, -1))
     nc = len(conns)
     col = lambda q, dtp: np.array([c[q] for c in conns], dtype=dtp)
@@ -242,7 +269,7 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
     body = np.array([ms, KG, xB, 0.0, KG, xB, 0.0, mh0, Ith0, Iph0, 2 * 0.6 * math.sqrt(Kz * mh0), 2 * 0.6 * math.sqrt(Kth * Ith0),
                      2 * 0.12 * math.sqrt(Kph * Iph0), 0.0, 0.0, 0.0])
     zO = KG - T
-    floor = np.array([min(segA[s] for s in range(segStart[n], segStart[n] + segCount[n])) for n in range(nn)]) - T
+    floor = np.minimum.reduceat(segA, np.asarray(segStart)) - T
     cen = np.array(ncent)
     f64 = lambda v: np.asarray(v, dtype=np.float64)
     i32 = lambda v: np.asarray(v, dtype=np.int32)
@@ -261,7 +288,8 @@ def synthetic_hull(seed: int, columns: int | None = None, spaces: int | None = N
         f64([m[1] for m in mk]), f64([m[2] for m in mk]), u8([m[3] for m in mk]), f64([-0.5, 1.2, 1.0, 10.0]), body,
         f64([L / 2, -L / 2]), f64([0.0, zO, 0.0, 0.0, 0.0, 0.0, 0.0]), np.zeros(nn), floor, cen[:, 0].copy(), cen[:, 1].copy(),
         cen[:, 2].copy())
-    _hash_in_place(ship, sim)
+    if hashed:
+        _hash_in_place(ship, sim)
     return ship, sim
 
 

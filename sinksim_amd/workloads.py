@@ -47,3 +47,25 @@ def random_tables(sim, count: int, seed: int, activation_share: float = 0.0, p_o
             late = rng.uniform(size=nc) < activation_share
             tOn[i, late] = rng.uniform(0, 1800, size=int(late.sum()))
     return en, area, tOn
+
+
+def uneven_fleet(count: int = 4096, seed: int = 20261008, processes: int = 0):
+    """The uneven batch: `count` seeded synthetic hulls (sinksim_amd.synth.synthetic_hull with seeds seed + i: length
+    100-300 m, 1,000-16,000 columns, 8-128 spaces, 20-600 connections, 1-6 breaches), one instance of each running its
+    own scenario from its initial state. Returns (models, inst_model). Hulls are built in `processes` worker
+    processes (0: one per available core, at most 16)."""
+    from .synth import synthetic_hull
+    seeds = [seed + i for i in range(count)]
+    if processes == 0:
+        try:
+            import os
+            processes = min(16, len(os.sched_getaffinity(0)))
+        except (AttributeError, OSError):
+            processes = 1
+    if processes > 1 and count > 64:
+        from multiprocessing import get_context
+        with get_context("fork").Pool(processes) as pool:
+            models = pool.map(synthetic_hull, seeds, chunksize=32)
+    else:
+        models = [synthetic_hull(s) for s in seeds]
+    return models, np.arange(count, dtype=np.int32)

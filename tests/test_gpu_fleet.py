@@ -88,3 +88,28 @@ def test_single_model_fleet_equals_batch():
         assert np.array_equal(a.curve(i), f.curve(i)), i
         ra, rf = a.result(i), f.result(i)
         assert ra.steps == rf.steps and ra.events == rf.events and np.array_equal(ra.vol, rf.vol), i
+
+
+@pytest.mark.parametrize("fleet", [False, True])
+def test_chunked_launches_equal_one_launch(fleet):
+    """stepsPerLaunch > 0 replays the captured launch graph once per chunk (counter reset + kernel); every replay
+    continues the instances where the last one stopped, and the results equal one launch to the end bit for bit."""
+    from sinksim_amd.batch import Batch, BatchOptions
+    models = _models()
+    inst_model = np.array([4, 0, 1, 2, 3, 1, 4, 0], np.int32) if fleet else np.full(8, 4, np.int32)
+    en, area, tOn = _tables(models, inst_model, seed=99)
+    runs = []
+    for spl in (0, 700):
+        opt = BatchOptions(count=len(inst_model), mixed_precision=False, traced_instances=0, curve_every=30.0, t_max=2400.0,
+                           steps_per_launch=spl)
+        b = Batch.fleet(models, inst_model, opt)
+        b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
+        b.set_workgroups(3)
+        b.run()
+        runs.append(([b.curve(i) for i in range(len(inst_model))], [b.result(i) for i in range(len(inst_model))], b.kernel_time()[1]))
+        b.close()
+    (c0, r0, l0), (c1, r1, l1) = runs
+    assert l0 == 1 and l1 > 3
+    for i in range(len(inst_model)):
+        assert np.array_equal(c0[i], c1[i]), i
+        assert r0[i].steps == r1[i].steps and r0[i].events == r1[i].events and np.array_equal(r0[i].vol, r1[i].vol), i

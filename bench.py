@@ -116,10 +116,16 @@ def bench_uneven(args):
     models, inst_model = uneven_fleet(count)
     print(f"{count} hulls generated in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     geo = np.array([40.0 * m[0].columns + (20.0 if not mixed else 12.0) * m[0].segments for m in models])
+    opts = BatchOptions(count=count, mixed_precision=mixed, traced_instances=0, curve_every=30.0, t_max=t_max)
+    # the launch shape, chosen once by the autotuner on the largest hulls (untimed)
+    b = Batch.fleet(models, inst_model, opts)
+    tune = b.autotune(sample=min(count, 1024), trial_steps=300)
+    b.close()
+    print(f"autotune: {tune}", file=sys.stderr, flush=True)
 
     def one_run():
-        b = Batch.fleet(models, inst_model, BatchOptions(count=count, mixed_precision=mixed, traced_instances=0,
-                                                         curve_every=30.0, t_max=t_max))
+        b = Batch.fleet(models, inst_model, opts)
+        b.set_launch(tune["threads_per_sim"], tune["workgroups"])
         secs = b.run()
         steps = b.results()["steps"]
         kms, launches = b.kernel_time()
@@ -150,6 +156,7 @@ def bench_uneven(args):
                                f"spaces, {int(nc.min())}-{int(nc.max())} connections), one instance each, {args.numerics} "
                                f"numerics, to founder or {t_max / 3600:g} h, curves every 30 s, refilled workgroups",
                    "instances": count, "numerics": args.numerics, "sim_steps_per_bench_step": int(tot_steps // args.steps),
+                   "threads_per_sim": tune["threads_per_sim"], "autotune_rates": [round(tune["rate_256"]), round(tune["rate_512"])],
                    "device": _device()},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 5), "traffic": None,

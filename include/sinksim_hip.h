@@ -165,10 +165,24 @@ ss_status ss_batch_set_connection_activation(ss_batch* b, int32_t instance, int3
  * applies a whole sweep uses instead of count * nconn single calls. */
 ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double* area, const double* tOn);
 
-/* No counterpart in the reference: the number of workgroups of the persistent launch (0, the default: as many as
- * are resident on the device at once, from the kernel's occupancy at this batch's LDS size). Fewer workgroups refill
- * more often; the autotuner sets this. SS_LOGIC after run. */
-ss_status ss_batch_set_workgroups(ss_batch* b, int32_t workgroups);
+/* No counterpart in the reference: the launch shape. threadsPerSim is the workgroup size of one simulation: 256
+ * (0 selects it; the default) or 512, whose second half starts the level solve while the first computes the
+ * buoyancy partials. workgroups is the number of workgroups of the launch (0, the default: as many as are resident
+ * on the device at once, from the kernel's occupancy at this batch's LDS size); fewer than the instance count makes
+ * the kernel refill them. SS_LOGIC after run. */
+ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgroups);
+
+/* No counterpart in the reference: what ss_batch_autotune measured and chose. rate[k]: simulation steps per second
+ * of the trial at 256 (k = 0) and 512 (k = 1) threads per simulation (0 if that shape does not fit). */
+typedef struct ss_tune_result {
+  int32_t threadsPerSim, workgroups;
+  double rate[2];
+} ss_tune_result;
+
+/* No counterpart in the reference: the autotuner. Runs the first `sample` instances of the work order (the largest
+ * models) for trialSteps steps at each launch shape on a trial copy (this batch's staged data is not touched),
+ * and sets the fastest shape on the batch. SS_LOGIC after run. */
+ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the
  * wall time of the device work (uploads excluded, as in the reference). SS_LOGIC when called twice. */

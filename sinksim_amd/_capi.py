@@ -58,6 +58,10 @@ class SsInstanceResult(C.Structure):
                 ("state", C.c_double * 7), ("outputs", C.c_double * 7), ("events", C.c_int32)]
 
 
+class SsTuneResult(C.Structure):
+    _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 2)]
+
+
 class SsEvent(C.Structure):
     _fields_ = [("t", C.c_double), ("id", C.c_int32), ("kind", C.c_int32)]
 
@@ -76,7 +80,8 @@ SIGNATURES = {
     "ss_batch_set_connection_area": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "ss_batch_set_connection_activation": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double]),
     "ss_batch_set_connections": (C.c_int, [C.c_void_p, _bp, _dp, _dp]),
-    "ss_batch_set_workgroups": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ss_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "ss_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(SsTuneResult)]),
     "ss_batch_run": (C.c_int, [C.c_void_p, _dp]),
     "ss_batch_total_steps": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "ss_batch_shared_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),

@@ -153,9 +153,20 @@ class Batch:
             A.ptr(t, C.c_double) if t is not None else None))
         self._keep = (e, a, t)
 
+    def set_launch(self, threads_per_sim: int = 0, workgroups: int = 0) -> None:
+        """The launch shape: 256 (0) or 512 threads per simulation, and the workgroup count (0: as many as are
+        resident at once; fewer makes the kernel refill them)."""
+        A.check(self._lib.ss_batch_set_launch(self._h, int(threads_per_sim), int(workgroups)))
+
     def set_workgroups(self, n: int) -> None:
-        """Workgroups of the persistent launch (0: as many as are resident at once)."""
-        A.check(self._lib.ss_batch_set_workgroups(self._h, int(n)))
+        self.set_launch(0, n)
+
+    def autotune(self, sample: int = 1024, trial_steps: int = 400) -> dict:
+        """Measures each launch shape on the first `sample` instances of the work order for trial_steps steps (on a
+        trial copy) and sets the fastest; returns the choice and the measured rates (steps/s)."""
+        r = A.SsTuneResult()
+        A.check(self._lib.ss_batch_autotune(self._h, int(sample), int(trial_steps), C.byref(r)))
+        return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "rate_256": r.rate[0], "rate_512": r.rate[1]}
 
     # ------------------------------------------------------------------ run and results
 

@@ -51,7 +51,7 @@ __device__ __forceinline__ void write_record(SSG double* rec, const Blk& B, int 
 // One instance on one workgroup: load it into LDS, step it until it founders, reaches tMax or has taken maxSteps
 // steps in this launch, store it back. Traces and curves are recorded as in the reference
 // (device_step.cuh:379-416); the readouts of the state at the end of the launch go to final7.
-template <bool Mixed>
+template <bool Mixed, int TPB>
 __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double tMax, SSL double* lds, int tid) {
   const int NN = P.S.nn, NC = P.M.nc, evCap = P.M.evCap, nmon = P.M.nmon, nmark = P.M.nmark;
   const size_t iN = static_cast<size_t>(Bt.nodeOff[inst]), iC = static_cast<size_t>(Bt.connOff[inst]);
@@ -64,6 +64,7 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     B.scal[O_INFLOW + tid] = Bt.out[inst * 7 + tid];
   }
   if (tid == 0) {
+    *B.nextGroup = 0;
     B.scal[S_EVCOUNT] = Bt.evCount[inst];
     B.scal[S_FOUNDERED] = Bt.foundered[inst];
     B.scal[S_FOUNDERT] = Bt.founderT[inst];
@@ -73,29 +74,29 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
       B.scal[S_CURVECOUNT] = Bt.curveCount[inst];
     }
   }
-  for (int n = tid; n < NN; n += kThreads) {
+  for (int n = tid; n < NN; n += TPB) {
     B.vol[n] = Bt.vol[iN + n]; B.level[n] = Bt.level[iN + n];
     B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n];
     B.area[n] = 0; B.zmin[n] = 0; B.zmax[n] = 0; B.hEff[n] = 0; B.aEff[n] = 0; B.excess[n] = 0; B.deg[n] = 0;
     B.tX[n] = 0; B.tY[n] = 0; B.tF[n] = 0; B.tM[n] = 0; B.tIp[n] = 0; B.tIr[n] = 0;
   }
-  for (int c = tid; c < NC; c += kThreads) {
+  for (int c = tid; c < NC; c += TPB) {
     B.cen[c] = Bt.en[iC + c]; B.carea[c] = Bt.area[iC + c]; B.ctOn[c] = Bt.tOn[iC + c];
     B.dV[c] = 0; B.q[c] = 0;
   }
-  for (int g = tid; g < P.M.ng; g += kThreads) {
+  for (int g = tid; g < P.M.ng; g += TPB) {
     B.merged[g] = 0;
     B.gStart[g] = P.M.gStart[g]; B.gCount[g] = P.M.gCount[g]; B.gMode[g] = P.M.gMode[g];
     B.gpx[g] = P.M.gpx[g]; B.gpy[g] = P.M.gpy[g]; B.gpz[g] = P.M.gpz[g];
   }
-  for (int k = tid; k < P.M.nMembers; k += kThreads) B.gNodes[k] = P.M.gNodes[k];
-  for (int n = tid; n < NN; n += kThreads) {
+  for (int k = tid; k < P.M.nMembers; k += TPB) B.gNodes[k] = P.M.gNodes[k];
+  for (int n = tid; n < NN; n += TPB) {
     B.nSegStart[n] = P.S.segStart[n]; B.nSegCount[n] = P.S.segCount[n];
     B.nMu[n] = P.S.mu[n]; B.nVmax[n] = P.S.vmax[n]; B.nAov[n] = P.S.aov[n]; B.nAmin[n] = P.S.amin[n];
   }
-  for (int e = tid; e < evCap; e += kThreads) { B.evT[e] = Bt.evT[iE + e]; B.evId[e] = Bt.evId[iE + e]; B.evKind[e] = Bt.evKind[iE + e]; }
-  for (int m = tid; m < nmon; m += kThreads) { B.overT[m] = Bt.overT[iMon + m]; B.over[m] = 0; }
-  for (int m = tid; m < nmark; m += kThreads) B.markT[m] = Bt.markT[iMark + m];
+  for (int e = tid; e < evCap; e += TPB) { B.evT[e] = Bt.evT[iE + e]; B.evId[e] = Bt.evId[iE + e]; B.evKind[e] = Bt.evKind[iE + e]; }
+  for (int m = tid; m < nmon; m += TPB) { B.overT[m] = Bt.overT[iMon + m]; B.over[m] = 0; }
+  for (int m = tid; m < nmark; m += TPB) B.markT[m] = Bt.markT[iMark + m];
   __syncthreads();
   if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);  // the loaded state's pose
   __syncthreads();
@@ -103,8 +104,9 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
   const bool traced = inst < Bt.traced;
   long long done = Bt.steps[inst];
   if (done == 0 && traced) {
-    // the initial record: buoyancy at the initial pose, no inflow yet (simulation.cpp:137-145, 204-206)
-    buoyancy_partials<Mixed>(P, B, tid);
+    // the initial record: buoyancy at the initial pose, no inflow yet (simulation.cpp:137-145, 204-206); the
+    // canonical partials are the first 256 threads'
+    if (tid < kThreads) buoyancy_partials<Mixed>(P, B, tid);
     __syncthreads();
     if (tid == 0) {
       double V, bx, by, bz, top;
@@ -139,7 +141,7 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
       }
       // read again only at the next iteration, after the step's barriers
     }
-    device_step<Mixed>(pa, lds, tid);
+    device_step<Mixed, TPB>(pa, lds, tid);
     ++done;
     if (traced) {
       if (tid == 0) {
@@ -174,13 +176,13 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     }
     readouts7(P, B.scal, B.vol, Bt.final7 + static_cast<size_t>(inst) * 7);
   }
-  for (int n = tid; n < NN; n += kThreads) {
+  for (int n = tid; n < NN; n += TPB) {
     Bt.vol[iN + n] = B.vol[n]; Bt.level[iN + n] = B.level[n];
     Bt.cx[iN + n] = B.cx[n]; Bt.cy[iN + n] = B.cy[n]; Bt.cz[iN + n] = B.cz[n];
   }
-  for (int e = tid; e < evCap; e += kThreads) { Bt.evT[iE + e] = B.evT[e]; Bt.evId[iE + e] = B.evId[e]; Bt.evKind[iE + e] = B.evKind[e]; }
-  for (int m = tid; m < nmon; m += kThreads) Bt.overT[iMon + m] = B.overT[m];
-  for (int m = tid; m < nmark; m += kThreads) Bt.markT[iMark + m] = B.markT[m];
+  for (int e = tid; e < evCap; e += TPB) { Bt.evT[iE + e] = B.evT[e]; Bt.evId[iE + e] = B.evId[e]; Bt.evKind[iE + e] = B.evKind[e]; }
+  for (int m = tid; m < nmon; m += TPB) Bt.overT[iMon + m] = B.overT[m];
+  for (int m = tid; m < nmark; m += TPB) Bt.markT[iMark + m] = B.markT[m];
 }
 
 // The batch kernel. Persistent: each workgroup takes the next work item from the batch's counter, runs that
@@ -188,8 +190,8 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
 // whose instance founders or reaches tMax early refills itself with the next one, and every workgroup leaves the
 // loop once the counter has passed the item count. Otherwise (when every instance has a resident workgroup of its
 // own, so there is nothing to refill) workgroup k runs work item k; that form schedules a little better.
-template <bool Mixed, bool Persistent>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_batch(const ModelParams* __restrict__ models,
+template <bool Mixed, bool Persistent, int TPB>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_batch(const ModelParams* __restrict__ models,
                                                                                                   const DevBatch* __restrict__ batch,
                                                                                                   long long maxSteps, double tMax) {
   BC& Bt = *(BC*)batch;
@@ -200,7 +202,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
   if constexpr (!Persistent) {
     const int inst = __builtin_amdgcn_readfirstlane(Bt.order[blockIdx.x]);
     PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(Bt.instModel[inst]) * sizeof(ModelParams));
-    run_instance<Mixed>(P, Bt, inst, maxSteps, tMax, lds, tid);
+    run_instance<Mixed, TPB>(P, Bt, inst, maxSteps, tMax, lds, tid);
     return;
   }
   for (;;) {
@@ -211,7 +213,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     if (k >= Bt.count) break;
     const int inst = __builtin_amdgcn_readfirstlane(Bt.order[k]);
     PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(Bt.instModel[inst]) * sizeof(ModelParams));
-    run_instance<Mixed>(P, Bt, inst, maxSteps, tMax, lds, tid);
+    run_instance<Mixed, TPB>(P, Bt, inst, maxSteps, tMax, lds, tid);
     __syncthreads();  // the instance's LDS is free for the next one
   }
 }
@@ -314,12 +316,39 @@ int device_count(std::string* why) {
 
 using namespace ssa;
 
+namespace {
+
+// LDS layout of one simulation, in doubles; byte and int arrays are packed into whole doubles.
+LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t nmark, int32_t maxGroup, int32_t evCap, int32_t nMembers,
+                     int32_t units) {
+  LdsLayout L{};
+  int32_t off = 0;
+  auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
+  auto ints = [](int32_t n) { return (n + 1) / 2; };
+  auto bytes = [](int32_t n) { return (n + 7) / 8; };
+  L.vol = take(NN); L.level = take(NN); L.cx = take(NN); L.cy = take(NN); L.cz = take(NN); L.area = take(NN); L.zmin = take(NN);
+  L.zmax = take(NN); L.hEff = take(NN); L.aEff = take(NN); L.excess = take(NN);
+  L.tX = take(NN); L.tY = take(NN); L.tF = take(NN); L.tM = take(NN); L.tIp = take(NN); L.tIr = take(NN);
+  L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC);
+  L.passes = take(7 * units * maxGroup);  // each solver unit's pass scratch
+  L.warpHyd = take(5 * kUnits);
+  L.frame = take(F_COUNT);
+  L.scal = take(S_COUNT);
+  L.over = take(nmon); L.overT = take(nmon); L.markT = take(nmark); L.evT = take(evCap); L.nextGroup = take(1);
+  L.nMu = take(NN); L.nVmax = take(NN); L.nAov = take(NN); L.nAmin = take(NN); L.gpx = take(NG); L.gpy = take(NG); L.gpz = take(NG);
+  L.deg = take(ints(NN)); L.evId = take(ints(evCap)); L.nSegStart = take(ints(NN)); L.nSegCount = take(ints(NN));
+  L.gStart = take(ints(NG)); L.gCount = take(ints(NG)); L.gNodes = take(ints(nMembers));
+  L.merged = take(bytes(NG)); L.cen = take(bytes(NC)); L.evKind = take(bytes(evCap)); L.gMode = take(bytes(NG));
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+
 // One model (a hull and a flooding scenario) on the device: the geometry records, the network tables and their
 // views, with the scenario's per-instance defaults kept on the host.
 struct Model {
   int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0, members = 0;
-  LdsLayout layout{};
-  size_t ldsBytes = 0;
   double cost = 0;  // work per step, relative: the order in which a batch hands its instances out
   // ship
   Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record
@@ -336,7 +365,11 @@ struct Model {
   std::vector<uint8_t> en0;
   std::vector<double> area0, tOn0, state0, vol0, level0, cx0, cy0, cz0;
 
-  ModelParams params() const {
+  // the LDS layout of one simulation on a workgroup of tpb threads (tpb / 32 solver units), and its size
+  LdsLayout layout(int tpb) const { return lds_layout(NN, NC, NG, nmon, nmark, maxGroup, evCap, members, tpb / kLanes); }
+  size_t lds_bytes(int tpb) const { return static_cast<size_t>(layout(tpb).total) * sizeof(double); }
+
+  ModelParams params(int tpb) const {
     ModelParams m{};
     DevShip& s = m.S;
     s.ncol = ncol; s.nseg = nseg; s.nn = NN;
@@ -345,7 +378,7 @@ struct Model {
     s.segStart = G(segStart.p); s.segCount = G(segCount.p);
     s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
     m.M = net;
-    m.L = layout;
+    m.L = layout(tpb);
     return m;
   }
 };
@@ -353,12 +386,13 @@ struct Model {
 struct ss_batch {
   ss_batch_options opt{};
   bool mixed = true;
-  std::vector<std::unique_ptr<Model>> models;
+  std::vector<std::shared_ptr<const Model>> models;
   // per instance: its model and the offsets of its rows; the work order
   std::vector<int32_t> instModel, nodeOff, connOff, evOff, monOff, markOff, order;
   size_t nodeTotal = 0, connTotal = 0, evTotal = 0, monTotal = 0, markTotal = 0;
-  size_t ldsBytes = 0;  // the largest model's
+  size_t ldsBytes = 0;  // the largest model's, at the launch's workgroup size
   int32_t recSize = 0, maxSamples = 0, curveMax = 0;
+  int32_t tpb = kThreads;  // threads per simulation (workgroup size): 256 or 512
   int32_t blocks = 0;   // workgroups of the persistent launch (0: as many as fit on the device at once)
   bool refill = false;  // whether the last run used the refilling (persistent) kernel
   int64_t totalSteps = 0;
@@ -409,29 +443,6 @@ struct ss_batch {
 
 namespace {
 
-// LDS layout of one simulation, in doubles; byte and int arrays are packed into whole doubles.
-LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t nmark, int32_t maxGroup, int32_t evCap, int32_t nMembers) {
-  LdsLayout L{};
-  int32_t off = 0;
-  auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
-  auto ints = [](int32_t n) { return (n + 1) / 2; };
-  auto bytes = [](int32_t n) { return (n + 7) / 8; };
-  L.vol = take(NN); L.level = take(NN); L.cx = take(NN); L.cy = take(NN); L.cz = take(NN); L.area = take(NN); L.zmin = take(NN);
-  L.zmax = take(NN); L.hEff = take(NN); L.aEff = take(NN); L.excess = take(NN);
-  L.tX = take(NN); L.tY = take(NN); L.tF = take(NN); L.tM = take(NN); L.tIp = take(NN); L.tIr = take(NN);
-  L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC);
-  L.passes = take(7 * kUnits * maxGroup);
-  L.warpHyd = take(5 * kUnits);
-  L.frame = take(F_COUNT);
-  L.scal = take(S_COUNT);
-  L.over = take(nmon); L.overT = take(nmon); L.markT = take(nmark); L.evT = take(evCap);
-  L.nMu = take(NN); L.nVmax = take(NN); L.nAov = take(NN); L.nAmin = take(NN); L.gpx = take(NG); L.gpy = take(NG); L.gpz = take(NG);
-  L.deg = take(ints(NN)); L.evId = take(ints(evCap)); L.nSegStart = take(ints(NN)); L.nSegCount = take(ints(NN));
-  L.gStart = take(ints(NG)); L.gCount = take(ints(NG)); L.gNodes = take(ints(nMembers));
-  L.merged = take(bytes(NG)); L.cen = take(bytes(NC)); L.evKind = take(bytes(evCap)); L.gMode = take(bytes(NG));
-  L.total = off;
-  return L;
-}
 
 void need(bool ok, const std::string& what) {
   if (!ok) fail(SS_INVALID_ARGUMENT, what);
@@ -570,8 +581,8 @@ void check_options(const ss_batch_options* opt, const char* who) {
 }
 
 // Uploads one model: geometry records, network tables (model.cpp:137-167 for the incidence), LDS layout.
-std::unique_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t ldsLimit) {
-  std::unique_ptr<Model> mp(new Model);
+std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t ldsLimit) {
+  std::shared_ptr<Model> mp(new Model);
   Model& b = *mp;
   b.NN = S.nnode; b.NC = M.nconn; b.NG = M.ngroup; b.nmon = M.nmonitor; b.nmark = M.nmark; b.ncol = S.ncol; b.nseg = S.nseg;
   b.evCap = b.nmon + b.nmark + 1;
@@ -579,11 +590,9 @@ std::unique_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
   size_t members = 0;  // length of the group member list the groups index into
   for (int32_t g = 0; g < b.NG; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
   b.members = static_cast<int32_t>(members);
-  b.layout = lds_layout(b.NN, b.NC, b.NG, b.nmon, b.nmark, b.maxGroup, b.evCap, b.members);
-  b.ldsBytes = static_cast<size_t>(b.layout.total) * sizeof(double);
-  if (b.ldsBytes > ldsLimit)
-    fail(SS_RUNTIME, "ss_batch_create: this model needs " + std::to_string(b.ldsBytes) + " bytes of LDS per simulation, the device allows " +
-                         std::to_string(ldsLimit));
+  if (b.lds_bytes(kThreads) > ldsLimit)
+    fail(SS_RUNTIME, "ss_batch_create: this model needs " + std::to_string(b.lds_bytes(kThreads)) +
+                         " bytes of LDS per simulation, the device allows " + std::to_string(ldsLimit));
   b.cost = static_cast<double>(S.ncol) + 2.0 * static_cast<double>(S.nseg) + 50.0 * static_cast<double>(S.nnode);
 
   const size_t nc = static_cast<size_t>(S.ncol), ns = static_cast<size_t>(S.nseg), nn = static_cast<size_t>(S.nnode);
@@ -676,7 +685,7 @@ std::unique_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
 // A batch over uploaded models: per-instance offsets, the work order (largest models first, so that the long
 // simulations start early and the small ones fill in at the end), and the host staging of every instance's tables
 // and state from its model's defaults.
-std::unique_ptr<ss_batch> make_batch(const ss_batch_options& opt, std::vector<std::unique_ptr<Model>> models, std::vector<int32_t> instModel) {
+std::unique_ptr<ss_batch> make_batch(const ss_batch_options& opt, std::vector<std::shared_ptr<const Model>> models, std::vector<int32_t> instModel) {
   std::unique_ptr<ss_batch> B(new ss_batch);
   ss_batch& b = *B;
   b.opt = opt;
@@ -705,7 +714,6 @@ std::unique_ptr<ss_batch> make_batch(const ss_batch_options& opt, std::vector<st
   }
   if (b.nodeTotal > static_cast<size_t>(INT32_MAX) || b.connTotal > static_cast<size_t>(INT32_MAX))
     fail(SS_INVALID_ARGUMENT, "ss_batch_create: the batch's node or connection rows exceed 2^31 entries");
-  for (const auto& m : b.models) b.ldsBytes = std::max(b.ldsBytes, m->ldsBytes);
   b.order.resize(count);
   for (size_t i = 0; i < count; ++i) b.order[i] = static_cast<int32_t>(i);
   std::stable_sort(b.order.begin(), b.order.end(), [&](int32_t x, int32_t y) { return b.model(x).cost > b.model(y).cost; });
@@ -734,7 +742,7 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     validate(*ship, *sim);
     std::string why;
     if (device_count(&why) == 0) fail(SS_RUNTIME, "ss_batch_create: " + why);
-    std::vector<std::unique_ptr<Model>> models;
+    std::vector<std::shared_ptr<const Model>> models;
     models.push_back(build_model(*ship, *sim, lds_limit()));
     *out = make_batch(*opt, std::move(models), std::vector<int32_t>(static_cast<size_t>(opt->count), 0)).release();
   });
@@ -755,7 +763,7 @@ ss_status ss_batch_create_fleet(const ss_ship* ships, const ss_sim* sims, int32_
     std::string why;
     if (device_count(&why) == 0) fail(SS_RUNTIME, "ss_batch_create_fleet: " + why);
     const size_t limit = lds_limit();
-    std::vector<std::unique_ptr<Model>> models;
+    std::vector<std::shared_ptr<const Model>> models;
     for (int32_t k = 0; k < nmodels; ++k) models.push_back(build_model(ships[k], sims[k], limit));
     *out = make_batch(*opt, std::move(models), std::move(im)).release();
   });
@@ -803,12 +811,194 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
   });
 }
 
-ss_status ss_batch_set_workgroups(ss_batch* b, int32_t workgroups) {
+}  // extern "C"
+
+namespace {
+
+using BatchKernel = void (*)(const ModelParams*, const DevBatch*, long long, double);
+
+BatchKernel batch_kernel(bool mixed, bool persistent, int tpb) {
+  if (tpb == 512) {
+    if (persistent) return mixed ? run_batch<true, true, 512> : run_batch<false, true, 512>;
+    return mixed ? run_batch<true, false, 512> : run_batch<false, false, 512>;
+  }
+  if (persistent) return mixed ? run_batch<true, true, kThreads> : run_batch<false, true, kThreads>;
+  return mixed ? run_batch<true, false, kThreads> : run_batch<false, false, kThreads>;
+}
+
+// Uploads the staged instances, runs them to their end (one graph launch, or one per chunk of stepsPerLaunch
+// steps; at most maxLaunches) and reads the results back. The launch shape is the batch's (tpb, blocks).
+void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
+  const size_t count = static_cast<size_t>(b->opt.count);
+  b->ldsBytes = 0;
+  for (const auto& m : b->models) b->ldsBytes = std::max(b->ldsBytes, m->lds_bytes(b->tpb));
+  if (b->ldsBytes > lds_limit())
+    fail(SS_RUNTIME, "ss_batch_run: " + std::to_string(b->ldsBytes) + " bytes of LDS per simulation at " + std::to_string(b->tpb) +
+                         " threads exceed the device's limit");
+  b->en.upload(b->hEn); b->area.upload(b->hArea); b->tOn.upload(b->hTOn);
+  b->st.upload(b->hSt);
+  b->out.fill(0.0, count * 7);
+  b->vol.upload(b->hVol); b->level.upload(b->hLevel); b->ccx.upload(b->hCx); b->ccy.upload(b->hCy); b->ccz.upload(b->hCz);
+  b->evT.fill(0.0, b->evTotal); b->evId.fill(0, b->evTotal); b->evKind.fill(0, b->evTotal); b->evCount.fill(0, count);
+  b->overT.fill(-1.0, b->monTotal);
+  b->markT.fill(-1.0, b->markTotal);
+  b->foundered.fill(0, count); b->founderT.fill(-1.0, count); b->steps.fill(0, count); b->final7.fill(0.0, count * 7);
+  const size_t traced = static_cast<size_t>(b->opt.traced), rs = static_cast<size_t>(b->recSize);
+  b->denseRec.alloc(traced * static_cast<size_t>(b->opt.denseSteps + 1) * rs);
+  b->sampleRec.alloc(traced * static_cast<size_t>(b->maxSamples) * rs);
+  b->denseCount.fill(0, traced); b->sampleCount.fill(0, traced); b->nextSample.fill(b->opt.sampleEvery, traced);
+  if (b->opt.curveEvery > 0) {
+    b->curveMax = static_cast<int32_t>(std::floor(b->opt.tMax / b->opt.curveEvery)) + 2;
+    b->curveRec.alloc(count * static_cast<size_t>(b->curveMax) * kCurveValues);
+    b->curveCount.fill(0, count);
+    b->nextCurve.fill(0.0, count);  // the first record is the initial state, as in the oracle's run
+  }
+  b->dInstModel.upload(b->instModel); b->dOrder.upload(b->order);
+  b->dNodeOff.upload(b->nodeOff); b->dConnOff.upload(b->connOff); b->dEvOff.upload(b->evOff);
+  b->dMonOff.upload(b->monOff); b->dMarkOff.upload(b->markOff);
+  b->counter.fill(0, 1);
+  std::vector<ModelParams> mp;
+  for (const auto& m : b->models) mp.push_back(m->params(b->tpb));
+  b->dModels.upload(mp);
+  const DevBatch bv = b->batchView();
+  b->dBatch.upload(&bv, 1);
+
+  const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
+  for (int tpb : {kThreads, 512})
+    for (bool pers : {false, true})
+      for (bool mixed : {false, true})
+        if (b->ldsBytes > 64 * 1024)
+          hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(batch_kernel(mixed, pers, tpb)), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        static_cast<int>(b->ldsBytes)),
+                    "hipFuncSetAttribute");
+  // the persistent grid: as many workgroups as are resident on the device at once (or the set count)
+  int perCU = 0, cus = 0;
+  hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(batch_kernel(b->mixed, true, b->tpb)), b->tpb,
+                                                         b->ldsBytes),
+            "occupancy");
+  hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+  const int resident = std::max(1, perCU) * std::max(1, cus);
+  const int grid = static_cast<int>(std::min<long long>(static_cast<long long>(count), b->blocks > 0 ? b->blocks : resident));
+  // one workgroup per instance when all of them are resident at once; the refilling loop otherwise
+  b->refill = grid < static_cast<int>(count);
+  const BatchKernel kernel = batch_kernel(b->mixed, b->refill, b->tpb);
+
+  // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
+  // replayed for every chunk: identical launches, one submission each.
+  hipStream_t stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Release {
+    hipStream_t& s; hipGraph_t& g; hipGraphExec_t& x; hipEvent_t& a; hipEvent_t& c;
+    ~Release() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (g) (void)hipGraphDestroy(g);
+      if (a) (void)hipEventDestroy(a);
+      if (c) (void)hipEventDestroy(c);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } release{stream, graph, exec, e0, e1};
+  hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, b->counter.p);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
+                     static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
+  hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  const auto t0 = std::chrono::steady_clock::now();
+  b->kernelMs = 0;
+  b->launches = 0;
+  std::vector<uint8_t> f(count);
+  std::vector<double> st(count * 7);
+  for (;;) {
+    hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+    hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+    hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+    hip_check(hipEventSynchronize(e1), "kernel execution");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    b->kernelMs += ms;
+    b->launches++;
+    if (maxSteps == LLONG_MAX || b->launches >= maxLaunches) break;
+    b->foundered.download(f.data(), count);
+    b->st.download(st.data(), count * 7);
+    bool active = false;
+    for (size_t i = 0; i < count; ++i)
+      if (!f[i] && st[i * 7] < b->opt.tMax) { active = true; break; }
+    if (!active) break;
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  read_back(b);
+  if (seconds) *seconds = secs;
+}
+
+}  // namespace
+
+extern "C" {
+
+ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgroups) {
   return guarded([&] {
-    need(b != nullptr, "ss_batch_set_workgroups: null batch");
-    if (b->ran) fail(SS_LOGIC, "ss_batch_set_workgroups: set after run");
-    need(workgroups >= 0, "ss_batch_set_workgroups: negative count");
+    need(b != nullptr, "ss_batch_set_launch: null batch");
+    if (b->ran) fail(SS_LOGIC, "ss_batch_set_launch: set after run");
+    need(threadsPerSim == 0 || threadsPerSim == kThreads || threadsPerSim == 512, "ss_batch_set_launch: threadsPerSim must be 0, 256 or 512");
+    need(workgroups >= 0, "ss_batch_set_launch: negative workgroup count");
+    b->tpb = threadsPerSim == 0 ? kThreads : threadsPerSim;
     b->blocks = workgroups;
+  });
+}
+
+ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_autotune: null batch");
+    if (b->ran) fail(SS_LOGIC, "ss_batch_autotune: tune after run");
+    need(sample >= 1 && trialSteps >= 1, "ss_batch_autotune: sample and trialSteps must be positive");
+    // the trial batch: the first `sample` instances of the work order (the largest models), with their staged
+    // tables and states, over the same uploaded models
+    const int32_t n = std::min(sample, b->opt.count);
+    std::vector<int32_t> im(static_cast<size_t>(n));
+    for (int32_t k = 0; k < n; ++k) im[static_cast<size_t>(k)] = b->instModel[static_cast<size_t>(b->order[static_cast<size_t>(k)])];
+    ss_batch_options o = b->opt;
+    o.count = n;
+    o.traced = 0;
+    o.curveEvery = 0;
+    o.stepsPerLaunch = trialSteps;
+    const int tpbs[2] = {kThreads, 512};
+    double best = -1;
+    int32_t bestTpb = kThreads, bestBlocks = 0;
+    ss_tune_result res{};
+    for (int c = 0; c < 2; ++c) {
+      std::unique_ptr<ss_batch> t = make_batch(o, b->models, im);
+      for (int32_t k = 0; k < n; ++k) {  // copy the staged tables and states of the sampled instances
+        const size_t src = static_cast<size_t>(b->order[static_cast<size_t>(k)]), dst = static_cast<size_t>(k);
+        const Model& m = b->model(static_cast<int32_t>(src));
+        std::copy_n(b->hEn.begin() + b->connOff[src], m.NC, t->hEn.begin() + t->connOff[dst]);
+        std::copy_n(b->hArea.begin() + b->connOff[src], m.NC, t->hArea.begin() + t->connOff[dst]);
+        std::copy_n(b->hTOn.begin() + b->connOff[src], m.NC, t->hTOn.begin() + t->connOff[dst]);
+        std::copy_n(b->hSt.begin() + 7 * src, 7, t->hSt.begin() + 7 * dst);
+        for (auto [from, to] : {std::make_pair(&b->hVol, &t->hVol), std::make_pair(&b->hLevel, &t->hLevel), std::make_pair(&b->hCx, &t->hCx),
+                                std::make_pair(&b->hCy, &t->hCy), std::make_pair(&b->hCz, &t->hCz)})
+          std::copy_n(from->begin() + b->nodeOff[src], m.NN, to->begin() + t->nodeOff[dst]);
+      }
+      t->tpb = tpbs[c];
+      size_t lds = 0;
+      for (const auto& m : t->models) lds = std::max(lds, m->lds_bytes(t->tpb));
+      if (lds > lds_limit()) continue;  // this shape does not fit the largest model
+      // one launch of trialSteps steps for every sampled instance
+      double secs = 0;
+      execute(t.get(), &secs, 1);
+      const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
+      res.rate[c] = rate;
+      if (rate > best) { best = rate; bestTpb = t->tpb; bestBlocks = 0; }
+    }
+    b->tpb = bestTpb;
+    b->blocks = bestBlocks;
+    res.threadsPerSim = bestTpb;
+    res.workgroups = bestBlocks;
+    if (out) *out = res;
   });
 }
 
@@ -817,103 +1007,7 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
     need(b != nullptr, "ss_batch_run: null batch");
     if (b->ran) fail(SS_LOGIC, "ss_batch_run: run twice");
     b->ran = true;
-    const size_t count = static_cast<size_t>(b->opt.count);
-    b->en.upload(b->hEn); b->area.upload(b->hArea); b->tOn.upload(b->hTOn);
-    b->st.upload(b->hSt);
-    b->out.fill(0.0, count * 7);
-    b->vol.upload(b->hVol); b->level.upload(b->hLevel); b->ccx.upload(b->hCx); b->ccy.upload(b->hCy); b->ccz.upload(b->hCz);
-    b->evT.fill(0.0, b->evTotal); b->evId.fill(0, b->evTotal); b->evKind.fill(0, b->evTotal); b->evCount.fill(0, count);
-    b->overT.fill(-1.0, b->monTotal);
-    b->markT.fill(-1.0, b->markTotal);
-    b->foundered.fill(0, count); b->founderT.fill(-1.0, count); b->steps.fill(0, count); b->final7.fill(0.0, count * 7);
-    const size_t traced = static_cast<size_t>(b->opt.traced), rs = static_cast<size_t>(b->recSize);
-    b->denseRec.alloc(traced * static_cast<size_t>(b->opt.denseSteps + 1) * rs);
-    b->sampleRec.alloc(traced * static_cast<size_t>(b->maxSamples) * rs);
-    b->denseCount.fill(0, traced); b->sampleCount.fill(0, traced); b->nextSample.fill(b->opt.sampleEvery, traced);
-    if (b->opt.curveEvery > 0) {
-      b->curveMax = static_cast<int32_t>(std::floor(b->opt.tMax / b->opt.curveEvery)) + 2;
-      b->curveRec.alloc(count * static_cast<size_t>(b->curveMax) * kCurveValues);
-      b->curveCount.fill(0, count);
-      b->nextCurve.fill(0.0, count);  // the first record is the initial state, as in the oracle's run
-    }
-    b->dInstModel.upload(b->instModel); b->dOrder.upload(b->order);
-    b->dNodeOff.upload(b->nodeOff); b->dConnOff.upload(b->connOff); b->dEvOff.upload(b->evOff);
-    b->dMonOff.upload(b->monOff); b->dMarkOff.upload(b->markOff);
-    b->counter.fill(0, 1);
-    std::vector<ModelParams> mp;
-    for (const auto& m : b->models) mp.push_back(m->params());
-    b->dModels.upload(mp);
-    const DevBatch bv = b->batchView();
-    b->dBatch.upload(&bv, 1);
-
-    const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
-    auto persistent = b->mixed ? run_batch<true, true> : run_batch<false, true>;
-    // the persistent grid: as many workgroups as are resident on the device at once (or the tuned count)
-    int perCU = 0, cus = 0;
-    for (auto k : {run_batch<true, true>, run_batch<false, true>, run_batch<true, false>, run_batch<false, false>})
-      if (b->ldsBytes > 64 * 1024)
-        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(k), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(b->ldsBytes)),
-                  "hipFuncSetAttribute");
-    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(persistent), kThreads, b->ldsBytes), "occupancy");
-    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
-    const int resident = std::max(1, perCU) * std::max(1, cus);
-    const int grid = static_cast<int>(std::min<long long>(static_cast<long long>(count), b->blocks > 0 ? b->blocks : resident));
-    // one workgroup per instance when all of them are resident at once; the refilling loop otherwise
-    const bool refill = grid < static_cast<int>(count);
-    auto kernel = refill ? persistent : (b->mixed ? run_batch<true, false> : run_batch<false, false>);
-    b->refill = refill;
-
-    // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
-    // replayed for every chunk: identical launches, one submission each.
-    hipStream_t stream = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    struct Release {
-      hipStream_t& s; hipGraph_t& g; hipGraphExec_t& x; hipEvent_t& a; hipEvent_t& b;
-      ~Release() {
-        if (x) (void)hipGraphExecDestroy(x);
-        if (g) (void)hipGraphDestroy(g);
-        if (a) (void)hipEventDestroy(a);
-        if (b) (void)hipEventDestroy(b);
-        if (s) (void)hipStreamDestroy(s);
-      }
-    } release{stream, graph, exec, e0, e1};
-    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
-    hip_check(hipEventCreate(&e0), "hipEventCreate");
-    hip_check(hipEventCreate(&e1), "hipEventCreate");
-    hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, b->counter.p);
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
-                       static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
-    hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
-    hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-    hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
-    const auto t0 = std::chrono::steady_clock::now();
-    b->kernelMs = 0;
-    b->launches = 0;
-    std::vector<uint8_t> f(count);
-    std::vector<double> st(count * 7);
-    for (;;) {
-      hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-      hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
-      hip_check(hipEventRecord(e1, stream), "hipEventRecord");
-      hip_check(hipEventSynchronize(e1), "kernel execution");
-      float ms = 0;
-      hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
-      b->kernelMs += ms;
-      b->launches++;
-      if (maxSteps == LLONG_MAX) break;
-      b->foundered.download(f.data(), count);
-      b->st.download(st.data(), count * 7);
-      bool active = false;
-      for (size_t i = 0; i < count; ++i)
-        if (!f[i] && st[i * 7] < b->opt.tMax) { active = true; break; }
-      if (!active) break;
-    }
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    read_back(b);
-    if (seconds) *seconds = secs;
+    execute(b, seconds);
   });
 }
 

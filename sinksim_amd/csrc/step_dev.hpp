@@ -32,9 +32,9 @@
 
 namespace ssa {
 
-constexpr int kThreads = 256;  // logical block of the canonical buoyancy order
+constexpr int kThreads = 256;  // logical block of the canonical buoyancy order (the first 256 threads of a workgroup)
 constexpr int kLanes = 32;     // logical warp of the canonical trees
-constexpr int kUnits = kThreads / kLanes;
+constexpr int kUnits = kThreads / kLanes;  // buoyancy's 32-lane groups
 
 // scheme v1 (reference: engine/include/sinksim/kernel/scheme.hpp:11-28)
 constexpr double kEmptyVolume = 1e-6;
@@ -149,7 +149,7 @@ struct DevBatch {
 struct LdsLayout {
   int32_t vol, level, cx, cy, cz, area, zmin, zmax, hEff, aEff, excess;
   int32_t tX, tY, tF, tM, tIp, tIr;
-  int32_t dV, q, carea, ctOn, passes, warpHyd, frame, scal, over, overT, markT, evT;
+  int32_t dV, q, carea, ctOn, passes, warpHyd, frame, scal, over, overT, markT, evT, nextGroup;
   int32_t nMu, nVmax, nAov, nAmin, gpx, gpy, gpz;  // copies of the node and group tables
   int32_t deg, evId, nSegStart, nSegCount, gStart, gCount, gNodes;  // int32 arrays, offsets in doubles
   int32_t merged, cen, evKind, gMode;                               // byte arrays, offsets in doubles
@@ -177,6 +177,7 @@ struct Blk {
   SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
   SSL double *tX, *tY, *tF, *tM, *tIp, *tIr;
   SSL double *dV, *q, *carea, *ctOn, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
+  SSL int32_t* nextGroup;  // the next free-surface group a solver unit takes (reset to 0 before each step's solve)
   SSL double *nMu, *nVmax, *nAov, *nAmin, *gpx, *gpy, *gpz;
   SSL int32_t *deg, *evId, *nSegStart, *nSegCount, *gStart, *gCount, *gNodes;
   SSL uint8_t *merged, *cen, *evKind, *gMode;
@@ -192,6 +193,7 @@ __device__ inline Blk carve(SSL double* base, PC& P) {
   b.dV = base + L.dV; b.q = base + L.q; b.carea = base + L.carea; b.ctOn = base + L.ctOn; b.passes = base + L.passes;
   b.warpHyd = base + L.warpHyd; b.frame = base + L.frame; b.scal = base + L.scal; b.over = base + L.over; b.overT = base + L.overT;
   b.markT = base + L.markT; b.evT = base + L.evT;
+  b.nextGroup = reinterpret_cast<SSL int32_t*>(base + L.nextGroup);
   b.nMu = base + L.nMu; b.nVmax = base + L.nVmax; b.nAov = base + L.nAov; b.nAmin = base + L.nAmin;
   b.gpx = base + L.gpx; b.gpy = base + L.gpy; b.gpz = base + L.gpz;
   b.nSegStart = reinterpret_cast<SSL int32_t*>(base + L.nSegStart);
@@ -539,7 +541,7 @@ struct Solver {
     S.V = B.vol[n]; S.Vmax = B.nVmax[n]; S.h = B.level[n];
     S.st = L_S_FIRST;
   }
-  __device__ __forceinline__ void next_group(LevelSolve& S) const { S.g += kUnits; S.st = L_START; }
+  __device__ __forceinline__ void next_group(LevelSolve& S) const { S.st = L_START; }
   __device__ __forceinline__ void next_single(LevelSolve& S) const {
     if (++S.k < S.N) start_single(S);
     else next_group(S);
@@ -593,9 +595,13 @@ struct Solver {
     S.st = L_G_FIRST;
   }
 
-  // Moves L_START on to the next state that waits for a pass, or to L_DONE.
+  // Moves L_START on to the next state that waits for a pass, or to L_DONE: the unit takes the next group from the
+  // workgroup's counter (its lead lane's LDS atomic, broadcast to the unit), so units that finish early take more.
   __device__ __forceinline__ void resolve(LevelSolve& S, int NG) const {
     while (S.st == L_START) {
+      int g = 0;
+      if (lead) g = atomicAdd((int*)B.nextGroup, 1);
+      S.g = static_cast<int>(lead32(static_cast<uint32_t>(g)));
       if (S.g >= NG) { S.st = L_DONE; break; }
       begin(S);
     }
@@ -781,8 +787,8 @@ struct Solver {
   }
 };
 
-// The level solve of every group: logical group `unit` takes groups unit, unit + 8, ...; the two groups of a wave
-// issue their passes together, round after round, until both are done.
+// The level solve of every group by the workgroup's 32-lane units, each taking the next group from the counter
+// when it is free; the two units of a wave issue their passes together, round after round, until both are done.
 template <bool Mixed>
 __device__ void solve_levels(PC& P, const Blk& B, int tid) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
@@ -790,7 +796,7 @@ __device__ void solve_levels(PC& P, const Blk& B, int tid) {
   const Solver sv{P, B, B.passes + 7 * P.M.maxGroup * unit, lane == 0};
   LevelSolve S{};
   S.st = L_START;
-  S.g = unit;
+  S.g = -1;
   while (true) {
     sv.resolve(S, NG);
     int n;
@@ -1048,6 +1054,7 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
   if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; }
   const double tn = t0 + dt;
   if (lane == 0) {
+    *B.nextGroup = 0;  // for the next step's solve; read after the closing barrier
     B.scal[O_INFLOW] = seaIn / dt;
     B.scal[S_T] = tn; B.scal[S_ZO] = zO; B.scal[S_PITCH] = pitch; B.scal[S_ROLL] = roll;
     B.scal[S_VZ] = vz; B.scal[S_WTH] = wth; B.scal[S_WPH] = wph;
@@ -1111,12 +1118,14 @@ __device__ __forceinline__ PC& model_at(uint64_t& pa) {
 
 // One step of the model at address pa. On entry the frame slots hold the pose of the current state (written by the
 // previous step's tail or at load time); every thread may read them. Ends with a barrier.
-template <bool Mixed>
+// TPB is the workgroup's size: 256, or 512, where the second 256 threads start the level solve while the first
+// 256 compute the buoyancy partials (both depend only on the pose and the state), then join it.
+template <bool Mixed, int TPB>
 __device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
-  // (a) buoyancy partials, then (b) the free surfaces: logical group `unit` solves groups unit, unit + 8, ...
+  // (a) buoyancy partials (threads 0..255), (b) the free surfaces (every unit, taking groups as it is free)
   {
     PC& P = model_at(pa);
-    buoyancy_partials<Mixed>(P, carve(lds, P), tid);
+    if (TPB == kThreads || tid < kThreads) buoyancy_partials<Mixed>(P, carve(lds, P), tid);
   }
   {
     PC& P = model_at(pa);
@@ -1128,7 +1137,7 @@ __device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
-    for (int n = tid; n < P.S.nn; n += kThreads) node_heads(P, B, n);
+    for (int n = tid; n < P.S.nn; n += TPB) node_heads(P, B, n);
   }
   __syncthreads();
 
@@ -1137,7 +1146,7 @@ __device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
     const double t = B.scal[S_T];
-    for (int c = tid; c < P.M.nc; c += kThreads) B.dV[c] = connection_flow(P, B.frame, t, B, c);
+    for (int c = tid; c < P.M.nc; c += TPB) B.dV[c] = connection_flow(P, B.frame, t, B, c);
   }
   __syncthreads();
 
@@ -1146,8 +1155,8 @@ __device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
-    for (int n = tid; n < P.S.nn; n += kThreads) node_update(P, B, n);
-    for (int m = tid; m < P.M.nmon; m += kThreads) {
+    for (int n = tid; n < P.S.nn; n += TPB) node_update(P, B, n);
+    for (int m = tid; m < P.M.nmon; m += TPB) {
       double o = 0;
       const int s = P.M.monStart[m], e = s + P.M.monCount[m];
       for (int k = s; k < e; ++k) {
@@ -1156,7 +1165,7 @@ __device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
       }
       B.over[m] = o;
     }
-    if (tid == kThreads - 1) {
+    if (tid == TPB - 1) {
       double s = 0;
       for (int k = 0; k < P.M.seaCount; ++k) s += B.dV[P.M.sea[k]];
       B.scal[S_SEA] = s;

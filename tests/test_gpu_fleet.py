@@ -32,14 +32,14 @@ def _tables(models, inst_model, seed):
     return en, area, tOn
 
 
-def _check_fleet(models, inst_model, numerics, t_max, every, workgroups):
+def _check_fleet(models, inst_model, numerics, t_max, every, workgroups, tpb=0):
     import pyoracle
     from sinksim_amd.batch import Batch, BatchOptions
     en, area, tOn = _tables(models, inst_model, seed=zlib.crc32(numerics.encode()))
     b = Batch.fleet(models, inst_model, BatchOptions(count=len(inst_model), mixed_precision=numerics == "portable32",
                                                      traced_instances=0, curve_every=every, t_max=t_max))
     b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
-    b.set_workgroups(workgroups)
+    b.set_launch(tpb, workgroups)
     b.run()
     for i, m in enumerate(inst_model):
         ship, sim = models[m]
@@ -57,11 +57,12 @@ def _check_fleet(models, inst_model, numerics, t_max, every, workgroups):
     b.close()
 
 
+@pytest.mark.parametrize("tpb", [256, 512])
 @pytest.mark.parametrize("numerics", ["portable", "portable32"])
-def test_fleet_matches_oracle_with_refill(numerics):
+def test_fleet_matches_oracle_with_refill(numerics, tpb):
     models = _models()
     inst_model = np.array([4, 0, 1, 2, 3, 0, 2, 1, 3, 4, 0, 2], np.int32)
-    _check_fleet(models, inst_model, numerics, t_max=1200.0, every=30.0, workgroups=3)
+    _check_fleet(models, inst_model, numerics, t_max=1200.0, every=30.0, workgroups=3, tpb=tpb)
 
 
 def test_fleet_matches_oracle_full_grid():
@@ -113,3 +114,41 @@ def test_chunked_launches_equal_one_launch(fleet):
     for i in range(len(inst_model)):
         assert np.array_equal(c0[i], c1[i]), i
         assert r0[i].steps == r1[i].steps and r0[i].events == r1[i].events and np.array_equal(r0[i].vol, r1[i].vol), i
+
+
+@pytest.mark.parametrize("numerics", ["portable", "portable32"])
+def test_512_threads_reproduce_golden_trace(titanic, goldens, numerics):
+    """Threads-per-simulation 512 (buoyancy and level solve side by side) on the reference's golden traces."""
+    from sinksim_amd.batch import Batch, BatchOptions
+    ship, sim = titanic
+    gold = goldens[numerics]
+    b = Batch(ship, sim, BatchOptions(count=4, mixed_precision=numerics == "portable32", traced_instances=1,
+                                      dense_steps=gold.denseSteps, sample_every=gold.sampleEvery, t_max=5 * 3600.0))
+    b.set_launch(512, 0)
+    b.run()
+    dense, samples = b.trace(0)
+    assert np.array_equal(dense, gold.dense) and np.array_equal(samples, gold.samples)
+    r = b.result(0)
+    assert r.steps == gold.steps and r.founderT == gold.founderT
+
+
+def test_autotune_picks_a_shape_and_keeps_results():
+    """The autotuner measures both shapes on a trial copy, sets the faster one, and the tuned run equals an
+    untuned one instance for instance."""
+    from sinksim_amd.batch import Batch, BatchOptions
+    models = _models()
+    inst_model = np.array([2, 1, 0, 3, 4, 2, 0, 1, 3, 4], np.int32)
+    en, area, tOn = _tables(models, inst_model, seed=7)
+    out = []
+    for tune in (False, True):
+        b = Batch.fleet(models, inst_model, BatchOptions(count=len(inst_model), mixed_precision=False, traced_instances=0,
+                                                         curve_every=60.0, t_max=900.0))
+        b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
+        if tune:
+            r = b.autotune(sample=8, trial_steps=200)
+            assert r["threads_per_sim"] in (256, 512) and r["rate_256"] > 0 and r["rate_512"] > 0
+        b.run()
+        out.append([b.curve(i) for i in range(len(inst_model))])
+        b.close()
+    for a, c in zip(*out):
+        assert np.array_equal(a, c)

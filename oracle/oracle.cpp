@@ -3,13 +3,16 @@
 //
 // The host side of the CPU restatement: a simulation holder with the reference's Simulation semantics
 // (engine/src/simulation.cpp:34-221: reset, restore, step, run with the 30 s history, record_trace) and the
-// threaded sweep runner of engine/src/sweep.cpp:231-271, exported as a C API for ctypes.
+// threaded sweep runner of engine/src/sweep.cpp:231-271, exported as a C API for ctypes; and the driver of the
+// dual-number build of the portable step (dual_step.hpp): a run with its loss and one step with its tangent.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
+#include "dual_step.hpp"
 #include "oracle_step.hpp"
 
 using namespace orc;
@@ -178,6 +181,9 @@ class Sim {
   const Events& events() const { return ev_; }
   const Scratch& scratch() const { return sc_; }
   const std::vector<Real>& flows() const { return q_; }
+  const Ship& ship_view() const { return ship_; }
+  const Net& net_view() const { return net_; }
+  Index max_group() const { return static_cast<Index>(passes_.size()); }
 
  private:
   int numerics_;
@@ -429,6 +435,196 @@ void orc_count_end(int64_t* out10) {
   const long long v[10] = {g_ops.cwet, g_ops.cdry, g_ops.spart, g_ops.sfull, g_ops.sdry, g_ops.passes, g_ops.steps, g_ops.active, g_ops.pows,
                            g_ops.wetNodes};
   for (int k = 0; k < 10; ++k) out10[k] = v[k];
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------------------- the dual-number build's driver
+
+extern "C" {
+
+// Scale-factor parameters: connection c's area takes theta[pa[c]], its coefficient theta[pc[c]] (-1: none).
+struct orc_params {
+  int32_t np;
+  const int32_t *pa, *pc;
+  int32_t relaxed;  // 1: relaxed level derivatives (dual_step.hpp relax_level)
+};
+
+// The loss of a run (dual_step.hpp LossSpec).
+struct orc_loss {
+  int32_t nTrim;
+  const int64_t* trimStep;
+  const double *trimObs, *trimW;
+  double trimAfter;
+  int32_t nEv;
+  const int32_t *evKind, *evIdx;
+  const double *evObs, *evW;
+};
+
+}  // extern "C"
+
+namespace {
+
+template <class T>
+void lift(const double* v, int nd, const double* dv, size_t stride, T* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    out[i] = T(v[i]);
+    if constexpr (!std::is_same_v<T, double>)
+      for (int k = 0; k < nd; ++k) out[i].d[k] = dv ? dv[k * stride + i] : 0.0;
+  }
+}
+
+template <class T>
+void drop(const T& x, int nd, double* v, double* dv, size_t stride, size_t i) {
+  v[i] = dual::val(x);
+  if constexpr (!std::is_same_v<T, double>)
+    if (dv)
+      for (int k = 0; k < nd; ++k) dv[k * stride + i] = x.d[k];
+}
+
+// the state as a trace record (trace.cpp:57-67 layout; inflow is not carried and stays 0)
+template <class T>
+void state_record(const dual::Sim<T>& d, Index NN, int nd, double* rec, double* drec) {
+  const size_t R = 10 + 5 * static_cast<size_t>(NN);
+  if (drec) std::fill(drec, drec + nd * R, 0.0);
+  rec[0] = d.t;
+  drop(d.zO, nd, rec, drec, R, 1); drop(d.pitch, nd, rec, drec, R, 2); drop(d.roll, nd, rec, drec, R, 3);
+  drop(d.vz, nd, rec, drec, R, 4); drop(d.wth, nd, rec, drec, R, 5); drop(d.wph, nd, rec, drec, R, 6);
+  rec[7] = 0;
+  drop(d.Vb, nd, rec, drec, R, 8); drop(d.hullTop, nd, rec, drec, R, 9);
+  for (Index n = 0; n < NN; ++n) {
+    drop(d.vol[n], nd, rec, drec, R, 10 + n); drop(d.level[n], nd, rec, drec, R, 10 + NN + n);
+    drop(d.cx[n], nd, rec, drec, R, 10 + 2 * NN + n); drop(d.cy[n], nd, rec, drec, R, 10 + 3 * NN + n);
+    drop(d.cz[n], nd, rec, drec, R, 10 + 4 * NN + n);
+  }
+}
+
+template <class T>
+void restore_record(dual::Sim<T>& d, Index NN, int nd, const double* rec, const double* drec) {
+  const size_t R = 10 + 5 * static_cast<size_t>(NN);
+  std::vector<T> s6(6), v(5 * NN);
+  const int at6[6] = {1, 2, 3, 4, 5, 6};
+  for (int k = 0; k < 6; ++k) {
+    s6[k] = T(rec[at6[k]]);
+    if constexpr (!std::is_same_v<T, double>)
+      for (int j = 0; j < nd; ++j) s6[k].d[j] = drec ? drec[j * R + at6[k]] : 0.0;
+  }
+  lift(rec + 10, nd, drec ? drec + 10 : nullptr, R, v.data(), 5 * static_cast<size_t>(NN));
+  d.restore(rec[0], s6.data(), v.data(), v.data() + NN, v.data() + 2 * NN, v.data() + 3 * NN, v.data() + 4 * NN);
+}
+
+// The event signal of loss event e in the last step: monitor sum - threshold, or -z / +z of an under / above mark.
+template <class T>
+T event_signal(const dual::Sim<T>& d, const Net& M, int32_t kind, int32_t idx) {
+  if (kind == 0) return d.over_[idx] - M.monThreshold;
+  return M.mkWhen[idx] == 0 ? -d.markZ_[idx] : d.markZ_[idx];
+}
+
+template <int N>
+int64_t dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, const double* area, const double* tOn,
+                 const orc_params* prm, const double* theta, int32_t nd, const double* seed, double tMax, int64_t maxSteps,
+                 const orc_loss* L, double* lossOut, double* sigOut, double* recOut) {
+  using T = dual::D<N>;
+  Sim o(s, m, kPortable);
+  for (int32_t c = 0; c < m->nc; ++c) o.set_conn(c, en[c], area[c], tOn[c]);
+  const Ship& S = o.ship_view();
+  const Net& M = o.net_view();
+  const Index NN = S.nn;
+  const Index gmax = o.max_group();
+  dual::Sim<T> d(S, M, dual::Params{prm->np, prm->pa, prm->pc, prm->relaxed}, m->coef, &gmax);
+  std::vector<T> th(prm->np);
+  lift(theta, nd, seed, static_cast<size_t>(prm->np), th.data(), static_cast<size_t>(prm->np));
+  d.set_params(th.data());
+  std::vector<T> s6(6), v(5 * NN);
+  for (int k = 0; k < 6; ++k) s6[k] = T(m->state0[1 + k]);
+  lift(m->vol0, 0, nullptr, 0, v.data(), NN);
+  lift(m->level0, 0, nullptr, 0, v.data() + NN, NN);
+  lift(m->cx0, 0, nullptr, 0, v.data() + 2 * NN, NN);
+  lift(m->cy0, 0, nullptr, 0, v.data() + 3 * NN, NN);
+  lift(m->cz0, 0, nullptr, 0, v.data() + 4 * NN, NN);
+  d.restore(m->state0[0], s6.data(), v.data(), v.data() + NN, v.data() + 2 * NN, v.data() + 3 * NN, v.data() + 4 * NN);
+
+  const int32_t nT = L ? L->nTrim : 0, nE = L ? L->nEv : 0;
+  std::vector<T> trim(nT), tstar(nE), prev(nE);
+  std::vector<char> trimSet(nT, 0), fired(nE, 0);
+  int64_t steps = 0;
+  while (d.t < tMax && !d.foundered && steps < maxSteps) {
+    d.step();
+    ++steps;
+    for (int32_t j = 0; j < nT; ++j)
+      if (L->trimStep[j] == steps) { trim[j] = d.pitch * 180.0 / kPi; trimSet[j] = 1; }
+    for (int32_t e = 0; e < nE; ++e) {
+      const T sg = event_signal(d, M, L->evKind[e], L->evIdx[e]);
+      if (!fired[e] && sg > 0) {
+        fired[e] = 1;
+        tstar[e] = steps == 1 ? T(d.t) : d.t - M.dt * sg / (sg - prev[e]);
+      }
+      prev[e] = sg;
+    }
+  }
+  T loss = T(0);
+  for (int32_t j = 0; j < nT; ++j) {
+    if (!trimSet[j]) trim[j] = T(L->trimAfter);
+    const T r = trim[j] - L->trimObs[j];
+    loss += L->trimW[j] * (r * r);
+  }
+  for (int32_t e = 0; e < nE; ++e) {
+    if (!fired[e]) tstar[e] = T(d.t);
+    const T r = tstar[e] - L->evObs[e];
+    loss += L->evW[e] * (r * r);
+  }
+  if (lossOut) drop(loss, nd, lossOut, lossOut + 1, 1, 0);
+  if (sigOut) {
+    const size_t K = static_cast<size_t>(nT + nE);
+    for (int32_t j = 0; j < nT; ++j) drop(trim[j], nd, sigOut, sigOut + K, K, j);
+    for (int32_t e = 0; e < nE; ++e) drop(tstar[e], nd, sigOut, sigOut + K, K, nT + e);
+  }
+  if (recOut) {
+    const size_t R = 10 + 5 * static_cast<size_t>(NN);
+    state_record(d, NN, nd, recOut, recOut + R);
+  }
+  return steps;
+}
+
+}  // namespace
+
+extern "C" {
+
+// A run of one instance from the scenario's initial state (to tMax, founder or maxSteps) in the dual build, with
+// theta and nd <= 24 tangent directions of theta (seed[k * np + i]: direction k, parameter i). Outputs (each
+// optional): lossOut[1 + nd] the loss and its directional derivatives; sigOut[(1 + nd) * (nTrim + nEv)] the
+// trim values and event times, then their derivatives direction by direction; recOut[(1 + nd) * (10 + 5 NN)] the
+// final state as a trace record, then its derivatives. Returns the step count, or -1 for nd > 24.
+int64_t orc_dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, const double* area, const double* tOn,
+                     const orc_params* prm, const double* theta, int32_t nd, const double* seed, double tMax, int64_t maxSteps,
+                     const orc_loss* L, double* lossOut, double* sigOut, double* recOut) {
+  if (nd <= 1) return dual_run<1>(s, m, en, area, tOn, prm, theta, nd, seed, tMax, maxSteps, L, lossOut, sigOut, recOut);
+  if (nd <= 12) return dual_run<12>(s, m, en, area, tOn, prm, theta, nd, seed, tMax, maxSteps, L, lossOut, sigOut, recOut);
+  if (nd <= 24) return dual_run<24>(s, m, en, area, tOn, prm, theta, nd, seed, tMax, maxSteps, L, lossOut, sigOut, recOut);
+  return -1;
+}
+
+// One step of the dual build from a state record and its tangent along one direction (drec, same layout;
+// entries 0 and 7-9 are ignored), with theta and its tangent dtheta. Writes the state after the step and its
+// tangent, and the step's signals: sig[nmon + nmark] = monitor flow sums then mark world z, dsig their tangents.
+void orc_dual_step(const orc_ship* s, const orc_sim* m, const uint8_t* en, const double* area, const double* tOn,
+                   const orc_params* prm, const double* theta, const double* dtheta, const double* rec, const double* drec,
+                   double* recOut, double* drecOut, double* sig, double* dsig) {
+  using T = dual::D<1>;
+  Sim o(s, m, kPortable);
+  for (int32_t c = 0; c < m->nc; ++c) o.set_conn(c, en[c], area[c], tOn[c]);
+  const Ship& S = o.ship_view();
+  const Net& M = o.net_view();
+  const Index NN = S.nn, gmax = o.max_group();
+  dual::Sim<T> d(S, M, dual::Params{prm->np, prm->pa, prm->pc, prm->relaxed}, m->coef, &gmax);
+  std::vector<T> th(prm->np);
+  lift(theta, 1, dtheta, static_cast<size_t>(prm->np), th.data(), static_cast<size_t>(prm->np));
+  d.set_params(th.data());
+  restore_record(d, NN, 1, rec, drec);
+  d.step();
+  state_record(d, NN, 1, recOut, drecOut);
+  for (Index k = 0; k < M.nmon; ++k) drop(d.over_[k], 1, sig, dsig, 0, k);
+  for (Index k = 0; k < M.nmark; ++k) drop(d.markZ_[k], 1, sig, dsig, 0, M.nmon + k);
 }
 
 }  // extern "C"

@@ -41,6 +41,15 @@ class OrcSim(C.Structure):
                 ("vol0", _dp), ("level0", _dp), ("cx0", _dp), ("cy0", _dp), ("cz0", _dp)]
 
 
+class OrcParams(C.Structure):
+    _fields_ = [("np", C.c_int32), ("pa", _ip), ("pc", _ip), ("relaxed", C.c_int32)]
+
+
+class OrcLoss(C.Structure):
+    _fields_ = [("nTrim", C.c_int32), ("trimStep", C.POINTER(C.c_int64)), ("trimObs", _dp), ("trimW", _dp),
+                ("trimAfter", C.c_double), ("nEv", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evObs", _dp), ("evW", _dp)]
+
+
 _lib = None
 
 
@@ -84,6 +93,11 @@ def lib() -> C.CDLL:
     L.orc_fdl_scalbn.argtypes = [C.c_double, C.c_int]
     L.orc_pow_v8.restype = C.c_double
     L.orc_pow_v8.argtypes = [C.c_double, C.c_double]
+    L.orc_dual_run.restype = C.c_int64
+    L.orc_dual_run.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), _bp, _dp, _dp, C.POINTER(OrcParams), _dp, C.c_int32, _dp,
+                               C.c_double, C.c_int64, C.POINTER(OrcLoss), _dp, _dp, _dp]
+    L.orc_dual_step.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), _bp, _dp, _dp, C.POINTER(OrcParams), _dp, _dp, _dp, _dp,
+                                _dp, _dp, _dp, _dp]
     _lib = L
     return L
 
@@ -338,3 +352,74 @@ def flops_per_step(c: dict, nn: int, nconn: int, nsea: int) -> float:
     main = 18 * c["cwet"] + 7 * c["cdry"] + 22 * c["spart"] + 21 * c["sfull"] + 10 * c["sdry"] + 162 * c["passes"]
     net = 35 * c["active"] + 60 * c["pows"] + 37 * c["wetNodes"]
     return (main + net) / s + 1023 + 2 * nconn + nsea + 250
+
+
+# --------------------------------------------------------------------------- the dual-number build
+
+def _params(pa, pc, np_, relaxed=False):
+    pa, pc = _i(pa), _i(pc)
+    p = OrcParams()
+    p.np, p.pa, p.pc, p.relaxed = np_, _p(pa, C.c_int32), _p(pc, C.c_int32), 1 if relaxed else 0
+    return p, (pa, pc)
+
+
+def _loss(spec):
+    """An OrcLoss over a loss specification with attributes trim_steps, trim_obs, trim_w, trim_after, ev_kind,
+    ev_idx, ev_obs, ev_w (the arrays are kept alive in the returned tuple)."""
+    if spec is None:
+        return None, ()
+    ts = np.ascontiguousarray(spec.trim_steps, dtype=np.int64)
+    to, tw = _f(spec.trim_obs), _f(spec.trim_w)
+    ek, ei, eo, ew = _i(spec.ev_kind), _i(spec.ev_idx), _f(spec.ev_obs), _f(spec.ev_w)
+    L = OrcLoss()
+    L.nTrim, L.trimStep, L.trimObs, L.trimW = len(ts), ts.ctypes.data_as(C.POINTER(C.c_int64)), _p(to, C.c_double), _p(tw, C.c_double)
+    L.trimAfter = float(spec.trim_after)
+    L.nEv, L.evKind, L.evIdx, L.evObs, L.evW = len(ek), _p(ek, C.c_int32), _p(ei, C.c_int32), _p(eo, C.c_double), _p(ew, C.c_double)
+    return L, (ts, to, tw, ek, ei, eo, ew)
+
+
+def dual_run(model: Model, en, area, tOn, pa, pc, theta, seed=None, t_max: float = 1e30, max_steps: int = 2**62,
+             spec=None, relaxed: bool = False) -> dict:
+    """One instance from the scenario's initial state in the dual-number build (dual_step.hpp), with parameters
+    theta and tangent directions seed [nd, np] (default: the identity, the full gradient by forward mode, nd = np
+    <= 24). Returns steps, loss and dloss [nd], signals [nTrim + nEv] and dsignals [nd, ...], the final state
+    record and drecord [nd, 10 + 5 NN]."""
+    theta = _f(theta)
+    np_ = theta.shape[0]
+    seed = np.eye(np_) if seed is None else np.atleast_2d(_f(seed))
+    nd = seed.shape[0]
+    seed = _f(seed)
+    prm, keep = _params(pa, pc, np_, relaxed)
+    L, keep2 = _loss(spec)
+    en, area, tOn = _u(en), _f(area), _f(tOn)
+    nsig = (len(spec.trim_steps) + len(spec.ev_kind)) if spec is not None else 0
+    R = 10 + 5 * model.ship.nodes
+    loss = np.zeros(1 + nd)
+    sig = np.zeros((1 + nd) * max(nsig, 1))
+    rec = np.zeros((1 + nd) * R)
+    steps = lib().orc_dual_run(C.byref(model.s), C.byref(model.m), _p(en, C.c_uint8), _p(area, C.c_double), _p(tOn, C.c_double),
+                               C.byref(prm), _p(theta, C.c_double), nd, _p(seed, C.c_double), float(t_max), int(max_steps),
+                               C.byref(L) if L is not None else None, _p(loss, C.c_double), _p(sig, C.c_double),
+                               _p(rec, C.c_double))
+    if steps < 0:
+        raise ValueError("at most 24 tangent directions")
+    return {"steps": int(steps), "loss": float(loss[0]), "dloss": loss[1:].copy(),
+            "signals": sig[:nsig].copy(), "dsignals": sig[max(nsig, 1):].reshape(nd, -1)[:, :nsig].copy() if nsig else np.zeros((nd, 0)),
+            "record": rec[:R].copy(), "drecord": rec[R:].reshape(nd, R).copy()}
+
+
+def dual_step(model: Model, en, area, tOn, pa, pc, theta, dtheta, rec, drec, relaxed: bool = False):
+    """One step of the dual build from a state record and one tangent of it (and of theta). Returns (record after,
+    its tangent, signals [nmon + nmark] (monitor flow sums, mark world z), their tangents)."""
+    theta, dtheta = _f(theta), _f(dtheta)
+    prm, keep = _params(pa, pc, theta.shape[0], relaxed)
+    en, area, tOn = _u(en), _f(area), _f(tOn)
+    rec, drec = _f(rec), _f(drec)
+    R = rec.shape[0]
+    ro, dro = np.zeros(R), np.zeros(R)
+    ns = max(model.sim.monitors + model.sim.marks, 1)
+    sg, dsg = np.zeros(ns), np.zeros(ns)
+    lib().orc_dual_step(C.byref(model.s), C.byref(model.m), _p(en, C.c_uint8), _p(area, C.c_double), _p(tOn, C.c_double),
+                        C.byref(prm), _p(theta, C.c_double), _p(dtheta, C.c_double), _p(rec, C.c_double), _p(drec, C.c_double),
+                        _p(ro, C.c_double), _p(dro, C.c_double), _p(sg, C.c_double), _p(dsg, C.c_double))
+    return ro, dro, sg, dsg

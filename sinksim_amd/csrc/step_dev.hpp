@@ -839,10 +839,11 @@ __device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
   B.deg[n] = d;
 }
 
-// flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter
+// flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter. carea
+// holds the instance's products coef * area, the first multiplication of both flow laws, formed on the host.
 __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, const Blk& B, int c) {
   const i4v ci = P.M.conI[c];
-  const d4v cd = P.M.conD[c];
+  const d4v cd = P.M.conD[c];  // x, y, z (and the coefficient, already in carea)
   if (!B.cen[c] || B.ctOn[c] > t) { B.q[c] = 0; return 0; }
   if (ci.z >= 0 && B.merged[ci.z]) { B.q[c] = 0; return 0; }
   const int a = ci.x, b = ci.y;
@@ -853,7 +854,7 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
   double q = 0;
   if (ci.w == 0) {
     const double d = Ha - Hb;
-    if (d != 0) q = cd.w * B.carea[c] * sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
+    if (d != 0) q = B.carea[c] * sq2g * d / __builtin_sqrt(jmax(__builtin_fabs(d), kOrificeHeadFloor));
   } else {
     const double H1 = Ha > Hb ? Ha : Hb;
     if (H1 > 0) {
@@ -863,7 +864,7 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
         const double r = H2 / H1;
         f = dpow(jmax(0.0, 1 - dpow(r, kWeirExponent)), kVillemonteExponent);
       }
-      q = cd.w * B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
+      q = B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
     }
   }
   if (q == 0) { B.q[c] = 0; return 0; }

@@ -220,6 +220,61 @@ ss_status ss_batch_final_readouts(const ss_batch* b, int32_t instance, double* r
  * No reference counterpart: the measurement hook of the benchmark. */
 ss_status ss_batch_kernel_time(const ss_batch* b, double* ms, int32_t* launches);
 
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * The gradient (no counterpart in the reference; the reference calibrates by batched search over sweeps,
+ * tools/py/calibrate_gpu.py:95-124, which this replaces with gradients).
+ *
+ * Parameters theta are positive scale factors: connection c of instance i takes area * theta[i][pa[c]] and discharge
+ * coefficient coef * theta[i][pc[c]] (-1: no factor); pa and pc are laid out like the connection tables.
+ *
+ * The loss of an instance is
+ *   sum_j trimW[j] * (trimDeg after step trimStep[j] - obs_j)^2  +  sum_e evW[e] * (t*_e - obs_e)^2,
+ * trimDeg = pitch * 180 / pi (trimAfter when the run ended before that step), and t*_e the time at which monitor
+ * (evKind 0) or mark (evKind 1) evIdx first fires, interpolated linearly between the step before and the step at which
+ * it fires from the event's signed signal (monitor: flow sum - threshold; mark: -world z, or +world z for an 'above'
+ * mark): t* = t_k - dt * s_k / (s_k - s_{k-1}). An event that never fires scores the run's end time. */
+typedef struct ss_loss {
+  int32_t nTrim;
+  const int64_t* trimStep;
+  const double* trimW;
+  double trimAfter;
+  int32_t nEvent;
+  const int32_t* evKind;
+  const int32_t* evIdx;
+  const double* evW;
+} ss_loss;
+
+/* checkpointEvery: K, the steps between stored states (the reverse sweep recomputes each segment of K steps from its
+ * checkpoint); relaxed: 0 for the derivative of the discrete program, 1 for relaxed level derivatives (each level's
+ * tangent taken at the exact root of its volume, as if the level solve had converged); tMax (0: the batch's) and
+ * maxSteps (<= 0: none) end the run. */
+typedef struct ss_gradient_options {
+  int32_t checkpointEvery;
+  int32_t relaxed;
+  double tMax;
+  int64_t maxSteps;
+} ss_gradient_options;
+
+/* Runs every instance from its staged initial state and returns its loss [count], the loss's gradient with respect to
+ * theta [count * nparams] (null: forward only, no reverse sweep), the loss's signals [count * (nTrim + nEvent)]: trim
+ * values then event times (may be null), the steps run [count], the recomputed steps whose values differed from the
+ * forward's [count] (always 0 for a correct kernel; may be null) and the kernel time (ms, may be null). FP64
+ * (portable) batches on 256-thread workgroups only. The batch may be run any number of times this way; its own
+ * run() is unaffected. */
+ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta,
+                            const ss_loss* loss, const double* observations, const ss_gradient_options* opt, double* lossOut,
+                            double* grad, double* signals, int64_t* steps, int32_t* mismatches, double* kernelMs);
+
+/* One step of instance 0 in reverse, the test hook of the adjoint: from a state record (the trace layout), the
+ * adjoint of the state after the step (same layout; entries 0 and 7..9 ignored) and seeds on the step's monitor flow
+ * sums [nmonitor] and marks' world z [nmark] (either may be null), the adjoint of the state before it (lambdaOut,
+ * trace layout), the loss's derivative with respect to log theta (gradLog [nparams]) and the recomputed state after
+ * the step (recordOut, trace layout; inflow and hull top 0). */
+ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta, int32_t relaxed,
+                                const double* record, const double* lambda, const double* monitorSeed, const double* markSeed,
+                                double* lambdaOut, double* gradLog, double* recordOut);
+
 #ifdef __cplusplus
 }
 #endif

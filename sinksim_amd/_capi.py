@@ -66,6 +66,15 @@ class SsEvent(C.Structure):
     _fields_ = [("t", C.c_double), ("id", C.c_int32), ("kind", C.c_int32)]
 
 
+class SsLoss(C.Structure):
+    _fields_ = [("nTrim", C.c_int32), ("trimStep", C.POINTER(C.c_int64)), ("trimW", _dp), ("trimAfter", C.c_double),
+                ("nEvent", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evW", _dp)]
+
+
+class SsGradientOptions(C.Structure):
+    _fields_ = [("checkpointEvery", C.c_int32), ("relaxed", C.c_int32), ("tMax", C.c_double), ("maxSteps", C.c_int64)]
+
+
 # every exported entry point and its signature (the test of the library's exports reads this table)
 SIGNATURES = {
     "ss_last_error": (C.c_char_p, []),
@@ -92,6 +101,9 @@ SIGNATURES = {
     "ss_batch_curve": (C.c_int, [C.c_void_p, C.c_int32, _dp, C.c_int32, _ip]),
     "ss_batch_final_readouts": (C.c_int, [C.c_void_p, C.c_int32, _dp]),
     "ss_batch_kernel_time": (C.c_int, [C.c_void_p, _dp, _ip]),
+    "ss_batch_gradient": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.POINTER(SsLoss), _dp, C.POINTER(SsGradientOptions), _dp,
+                                    _dp, _dp, C.POINTER(C.c_int64), _ip, _dp]),
+    "ss_batch_adjoint_step": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
 }
 
 _lib = None

@@ -238,3 +238,66 @@ class Batch:
 
     def event_label(self, kind: int, id_: int) -> str:
         return self.sim.event_label(kind, id_)
+
+    # ------------------------------------------------------------------ the gradient
+
+    def _param_tables(self, params, theta):
+        """Per-instance connection tables of parameter indices (laid out like the connection tables) and theta
+        [count, np]; params: one calib.ParamMap, or one per model of a fleet."""
+        maps = [params[m] if isinstance(params, (list, tuple)) else params for m in self.inst_model]
+        npar = maps[0].count
+        if any(p.count != npar for p in maps):
+            raise ValueError("every model's parameter map must have the same parameter count")
+        pa = A.i32(np.concatenate([np.asarray(p.pa, np.int32) for p in maps]))
+        pc = A.i32(np.concatenate([np.asarray(p.pc, np.int32) for p in maps]))
+        th = A.f64(np.broadcast_to(np.asarray(theta, np.float64), (self.count, npar)))
+        return pa, pc, th, npar
+
+    def gradient(self, params, theta, spec, observations=None, checkpoint_every: int = 256, relaxed: bool = False,
+                 t_max: float = 0.0, max_steps: int = 0, want_grad: bool = True) -> dict:
+        """Every instance's loss (spec: a calib.LossSpec, whose observations are replaced by `observations`
+        [count, signals] when given) and its gradient with respect to theta [count, np], by the adjoint kernel with a
+        checkpoint every `checkpoint_every` steps. params: a calib.ParamMap (one model) or one per model of a fleet.
+        Returns loss [count], grad [count, np] (None without want_grad), signals [count, signals], steps [count],
+        mismatches [count] (recomputed steps whose values differed from the forward's: 0) and the kernel time (ms)."""
+        pa, pc, th, npar = self._param_tables(params, theta)
+        nT, nE = len(spec.trim_steps), len(spec.ev_kind)
+        ts = np.ascontiguousarray(spec.trim_steps, np.int64)
+        tw, ek, ei, ew = A.f64(spec.trim_w), A.i32(spec.ev_kind), A.i32(spec.ev_idx), A.f64(spec.ev_w)
+        L = A.SsLoss(nT, ts.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(tw, C.c_double), float(spec.trim_after), nE,
+                     A.ptr(ek, C.c_int32), A.ptr(ei, C.c_int32), A.ptr(ew, C.c_double))
+        nsig = nT + nE
+        if observations is None:
+            obs = np.tile(np.concatenate([np.asarray(spec.trim_obs, np.float64), np.asarray(spec.ev_obs, np.float64)]), (self.count, 1))
+        else:
+            obs = A.f64(observations).reshape(self.count, nsig)
+        obs = A.f64(obs)
+        opt = A.SsGradientOptions(int(checkpoint_every), 1 if relaxed else 0, float(t_max), int(max_steps))
+        loss = np.zeros(self.count)
+        grad = np.zeros((self.count, npar)) if want_grad else None
+        sig = np.zeros((self.count, max(nsig, 1)))
+        steps = np.zeros(self.count, np.int64)
+        mism = np.zeros(self.count, np.int32)
+        ms = C.c_double()
+        A.check(self._lib.ss_batch_gradient(self._h, npar, A.ptr(pa, C.c_int32), A.ptr(pc, C.c_int32), A.ptr(th, C.c_double), C.byref(L),
+                                            A.ptr(obs, C.c_double), C.byref(opt), A.ptr(loss, C.c_double),
+                                            A.ptr(grad, C.c_double) if grad is not None else None, A.ptr(sig, C.c_double),
+                                            steps.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(mism, C.c_int32), C.byref(ms)))
+        return {"loss": loss, "grad": grad, "signals": sig[:, :nsig], "steps": steps, "mismatches": mism, "ms": ms.value}
+
+    def adjoint_step(self, params, theta, record, lam, monitor_seed=None, mark_seed=None, relaxed: bool = False):
+        """One step of instance 0 in reverse (the adjoint's test hook): from a state record (trace layout) and the
+        adjoint of the state after the step, the adjoint of the state before it, d loss / d log theta, and the
+        recomputed state after the step."""
+        pa, pc, th, npar = self._param_tables(params, theta)
+        rec, lm = A.f64(record), A.f64(lam)
+        ms = A.f64(monitor_seed) if monitor_seed is not None else None
+        mk = A.f64(mark_seed) if mark_seed is not None else None
+        lo, gl, ro = np.zeros_like(rec), np.zeros(npar), np.zeros_like(rec)
+        A.check(self._lib.ss_batch_adjoint_step(self._h, npar, A.ptr(pa, C.c_int32), A.ptr(pc, C.c_int32), A.ptr(th, C.c_double),
+                                                1 if relaxed else 0, A.ptr(rec, C.c_double), A.ptr(lm, C.c_double),
+                                                A.ptr(ms, C.c_double) if ms is not None else None,
+                                                A.ptr(mk, C.c_double) if mk is not None else None, A.ptr(lo, C.c_double),
+                                                A.ptr(gl, C.c_double), A.ptr(ro, C.c_double)))
+        return lo, gl, ro
+

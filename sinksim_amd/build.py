@@ -1,6 +1,6 @@
 """Build recipe of the native parts, in-tree (the built files travel with the repository to a GPU host):
 
-  sinksim_amd/libsinksim_hip.so   the gfx950 batch kernel and the C ABI (include/sinksim_hip.h)
+  sinksim_amd/libsinksim_hip.so   the gfx950 batch and adjoint kernels and the C ABI (include/sinksim_hip.h)
   sinksim_amd/bin/sinksim_hip_run, sinksim_amd/bin/sinksim_sweep
                                   the C++ command-line drop-ins over the C ABI (host code: g++, no HIP headers)
 
@@ -28,8 +28,8 @@ HIP_FLAGS = [
 ]
 
 LIB = os.path.join(HERE, "libsinksim_hip.so")
-LIB_SOURCES = [os.path.join(CSRC, "batch.hip")]
-LIB_DEPS = LIB_SOURCES + [os.path.join(CSRC, f) for f in ("step_dev.hpp", "fdlibm_dev.hpp")] + [
+LIB_SOURCES = [os.path.join(CSRC, "batch.hip"), os.path.join(CSRC, "adjoint.hip")]
+LIB_DEPS = LIB_SOURCES + [os.path.join(CSRC, f) for f in ("step_dev.hpp", "fdlibm_dev.hpp", "adjoint_dev.hpp", "batch_host.hpp")] + [
     os.path.join(ROOT, "include", "sinksim_hip.h")]
 
 

@@ -1,0 +1,607 @@
+// The gradient of a batch (no counterpart in the reference): every instance's loss and its derivative with respect
+// to connection scale factors, by a reverse-mode adjoint of the step with checkpoints.
+//
+// One workgroup per instance runs three sweeps over its own trajectory:
+//   1. forward: the plain step (device_step, the batch kernel's step) from the initial state to founder / tMax /
+//      maxSteps, storing the state every K steps (a checkpoint) and the loss's ingredients (trim after given steps,
+//      the signed event signals at and before each event's first firing step);
+//   2. loss: the loss, the signals and the seeds of the reverse sweep (the loss's derivative with respect to the
+//      pitch after a trim step, and to a monitor's flow sum or a mark's world z at an event step and the one before);
+//   3. reverse: for each segment of K steps, last first: the segment recomputed from its checkpoint with the plain
+//      step into a ring of K + 1 states, then its steps in reverse order, each recomputed with its local Jacobians
+//      (jac_step, whose values are compared with the ring's next state) and pulled back (reverse_step).
+// The parameters enter each connection's product coef * area (the host forms the scaled products); the gradient
+// is the sum of the log-derivatives over the connections of a parameter, divided by the parameter.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/sinksim_hip.h"
+#include "adjoint_dev.hpp"
+#include "batch_host.hpp"
+
+namespace ssa {
+
+// The gradient run's per-batch view.
+struct DevAdj {
+  int32_t K, wantGrad;
+  long long maxSteps;
+  double tMax;
+  gcd* theta;                      // [count * np]
+  gci *pa, *pc;                    // per instance connection (connOff)
+  gcd* ca;                         // the scaled products coef * area per instance connection
+  int32_t nTrim, nEv;
+  const SSG long long* trimStep;
+  gcd* trimW;
+  double trimAfter;
+  gci *evKind, *evIdx;
+  gcd* evW;
+  gcd* obs;                        // [count * (nTrim + nEv)]
+  SSG double *loss, *grad, *signals;
+  SSG long long* steps;
+  SSG int32_t* mismatch;           // per instance: recomputed steps whose values differ from the forward's
+  SSG double *ckpt, *ring;
+  const SSG long long *ckptOff, *ringOff;  // per instance, in doubles
+  // the one-step form
+  gcd *stepRec, *stepLam, *stepMon, *stepMark;
+  SSG double *stepLamOut, *stepRecOut;
+};
+
+using DC = const SSC DevAdj;
+using AMC = const SSC AdjModel;
+
+// the state of the forward layout as a record of 7 + 5 NN values (t zO pitch roll vz wth wph, vol, level, cx, cy, cz)
+__device__ inline void save_state(PC& P, const Blk& B, SSG double* dst, int tid, int tpb) {
+  const int NN = P.S.nn;
+  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
+    double v;
+    if (i < 7) v = B.scal[S_T + i];
+    else {
+      const int j = i - 7, a = j / NN, n = j - a * NN;
+      const SSL double* src = a == 0 ? B.vol : a == 1 ? B.level : a == 2 ? B.cx : a == 3 ? B.cy : B.cz;
+      v = src[n];
+    }
+    dst[i] = v;
+  }
+}
+__device__ inline void restore_state(PC& P, const Blk& B, const SSG double* src, int tid, int tpb) {
+  const int NN = P.S.nn;
+  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
+    const double v = src[i];
+    if (i < 7) B.scal[S_T + i] = v;
+    else {
+      const int j = i - 7, a = j / NN, n = j - a * NN;
+      SSL double* d = a == 0 ? B.vol : a == 1 ? B.level : a == 2 ? B.cx : a == 3 ? B.cy : B.cz;
+      d[n] = v;
+    }
+  }
+}
+__device__ inline void load_tape_state(PC& P, const Tape& T, const SSG double* src, int tid, int tpb) {
+  const int NN = P.S.nn;
+  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
+    const double v = src[i];
+    if (i < 7) T.sv[i] = v;
+    else {
+      const int j = i - 7, a = j / NN, n = j - a * NN;
+      SSL double* d = a == 0 ? T.iVol : a == 1 ? T.iLvl : a == 2 ? T.iCx : a == 3 ? T.iCy : T.iCz;
+      d[n] = v;
+    }
+  }
+}
+
+// the recomputed step's values against the stored next state; counts differences into *bad
+__device__ inline void verify_step(PC& P, const Tape& T, const SSG double* next, int tid, int tpb, SSL int32_t* bad) {
+  const int NN = P.S.nn;
+  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
+    double v;
+    if (i < 7) {
+      const SSL double* o = T.tot;
+      v = i == 0 ? T.sv[0] + P.M.dt : i == 1 ? o[T_ZO1] : i == 2 ? o[T_P1] : i == 3 ? o[T_R1] : i == 4 ? o[T_VZ1] : i == 5 ? o[T_WTH1] : o[T_WPH1];
+    } else {
+      const int j = i - 7, a = j / NN, n = j - a * NN;
+      const SSL double* s = a == 0 ? T.vol1 : a == 1 ? T.oLvl : a == 2 ? T.oCx : a == 3 ? T.oCy : T.oCz;
+      v = s[n];
+    }
+    if (__builtin_bit_cast(uint64_t, v) != __builtin_bit_cast(uint64_t, next[i])) atomicAdd((int32_t*)bad, 1);
+  }
+}
+
+// Loads instance inst into the forward layout (run_instance's load, with the scaled products as carea).
+__device__ inline void load_instance(PC& P, BC& Bt, DC& D, const Blk& B, int inst, int tid, int tpb) {
+  const int NN = P.S.nn, NC = P.M.nc;
+  const size_t iN = static_cast<size_t>(Bt.nodeOff[inst]), iC = static_cast<size_t>(Bt.connOff[inst]);
+  if (tid < 7) { B.scal[S_T + tid] = Bt.st[inst * 7 + tid]; B.scal[O_INFLOW + tid] = 0; }
+  if (tid == 0) {
+    *B.nextGroup = 0;
+    B.scal[S_EVCOUNT] = 0; B.scal[S_FOUNDERED] = 0; B.scal[S_FOUNDERT] = -1; B.scal[S_SEA] = 0;
+    B.scal[S_NEXTCURVE] = 0; B.scal[S_CURVECOUNT] = 0;
+  }
+  for (int n = tid; n < NN; n += tpb) {
+    B.vol[n] = Bt.vol[iN + n]; B.level[n] = Bt.level[iN + n];
+    B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n];
+    B.area[n] = 0; B.zmin[n] = 0; B.zmax[n] = 0; B.hEff[n] = 0; B.aEff[n] = 0; B.excess[n] = 0; B.deg[n] = 0;
+    B.tX[n] = 0; B.tY[n] = 0; B.tF[n] = 0; B.tM[n] = 0; B.tIp[n] = 0; B.tIr[n] = 0;
+    B.nSegStart[n] = P.S.segStart[n]; B.nSegCount[n] = P.S.segCount[n];
+    B.nMu[n] = P.S.mu[n]; B.nVmax[n] = P.S.vmax[n]; B.nAov[n] = P.S.aov[n]; B.nAmin[n] = P.S.amin[n];
+  }
+  for (int c = tid; c < NC; c += tpb) {
+    B.cen[c] = Bt.en[iC + c]; B.carea[c] = D.ca[iC + c]; B.ctOn[c] = Bt.tOn[iC + c];
+    B.dV[c] = 0; B.q[c] = 0;
+  }
+  for (int g = tid; g < P.M.ng; g += tpb) {
+    B.merged[g] = 0;
+    B.gStart[g] = P.M.gStart[g]; B.gCount[g] = P.M.gCount[g]; B.gMode[g] = P.M.gMode[g];
+    B.gpx[g] = P.M.gpx[g]; B.gpy[g] = P.M.gpy[g]; B.gpz[g] = P.M.gpz[g];
+  }
+  for (int k = tid; k < P.M.nMembers; k += tpb) B.gNodes[k] = P.M.gNodes[k];
+  for (int e = tid; e < P.M.evCap; e += tpb) { B.evT[e] = 0; B.evId[e] = 0; B.evKind[e] = 0; }
+  for (int m = tid; m < P.M.nmon; m += tpb) { B.overT[m] = -1; B.over[m] = 0; }
+  for (int m = tid; m < P.M.nmark; m += tpb) B.markT[m] = -1;
+}
+
+// the signed event signal of loss event e in the step just taken (mark world z taken before the step, in zpre)
+__device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double zpre) {
+  if (kind == 0) return 0;  // monitors: read from B.over by the caller
+  return P.M.mkWhen[idx] == 0 ? -zpre : zpre;
+}
+
+constexpr int kTPB = kThreads;
+
+__global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
+                                                     const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
+  BC& Bt = *(BC*)batch;
+  DC& D = *(DC*)adj;
+  extern __shared__ double lds_raw[];
+  __shared__ int32_t bad;
+  SSL double* lds = (SSL double*)lds_raw;
+  const int tid = threadIdx.x;
+  const int inst = blockIdx.x;
+  const int mi = Bt.instModel[inst];
+  PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(mi) * sizeof(ModelParams));
+  AMC& AMs = *(AMC*)((const SSC char*)amodels + static_cast<size_t>(mi) * sizeof(AdjModel));
+  AdjModel AM{};
+  AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
+  const Blk B = carve(lds, P);
+  const Tape T = carve_tape(lds + P.L.total, AMs.A);
+  const int NN = P.S.nn;
+  const int SS = 7 + 5 * NN;
+  const int nT = D.nTrim, nE = D.nEv, nSig = nT + nE;
+  SSG double* ckpt = D.ckpt + D.ckptOff[inst];
+  SSG double* ring = D.ring + D.ringOff[inst];
+  const long long K = D.K;
+  // signals: trim value, trim set; per event: fired, step, s_k, s_{k-1}, previous, time at firing, z before the step
+  SSL double* trimV = T.sig;
+  SSL double* trimSet = trimV + nT;
+  SSL double* ev = trimSet + nT;  // 7 per event
+
+  load_instance(P, Bt, D, B, inst, tid, kTPB);
+  for (int i = tid; i < 2 * nT + 7 * nE; i += kTPB) T.sig[i] = 0;
+  if (tid == 0) bad = 0;
+  __syncthreads();
+  if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
+  __syncthreads();
+
+  // 1. forward with checkpoints and the loss's ingredients
+  const uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P));
+  long long done = 0;
+  for (;;) {
+    if (B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= D.tMax || done >= D.maxSteps) break;
+    if (done % K == 0) save_state(P, B, ckpt + (done / K) * SS, tid, kTPB);
+    if (tid == 0)
+      for (int e = 0; e < nE; ++e)
+        if (D.evKind[e] == 1) { const int i = D.evIdx[e]; ev[7 * e + 6] = world_z(B.frame, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]); }
+    __syncthreads();
+    device_step<false, kTPB>(pa, lds, tid);
+    ++done;
+    if (tid == 0) {
+      for (int j = 0; j < nT; ++j)
+        if (D.trimStep[j] == done) { trimV[j] = B.scal[S_PITCH] * 180 / kPi; trimSet[j] = 1; }
+      for (int e = 0; e < nE; ++e) {
+        SSL double* r = ev + 7 * e;
+        const double s = D.evKind[e] == 0 ? B.over[D.evIdx[e]] - P.M.monThreshold : event_signal(P, 1, D.evIdx[e], r[6]);
+        if (r[0] == 0 && s > 0) { r[0] = 1; r[1] = static_cast<double>(done); r[2] = s; r[3] = r[4]; r[5] = B.scal[S_T]; }
+        r[4] = s;
+      }
+    }
+    __syncthreads();
+  }
+
+  // 2. loss, signals, seeds
+  if (tid == 0) {
+    const double tEnd = B.scal[S_T], dt = P.M.dt;
+    SSG double* sigOut = D.signals ? D.signals + static_cast<size_t>(inst) * nSig : nullptr;
+    gcd* obs = D.obs + static_cast<size_t>(inst) * nSig;
+    double L = 0;
+    int ns = 0;
+    for (int j = 0; j < nT; ++j) {
+      const double v = trimSet[j] != 0 ? trimV[j] : D.trimAfter;
+      const double r = v - obs[j];
+      L += D.trimW[j] * (r * r);
+      if (sigOut) sigOut[j] = v;
+      if (trimSet[j] != 0) {
+        SSL double* sd = T.seeds + 4 * ns++;
+        sd[0] = static_cast<double>(D.trimStep[j]); sd[1] = 0; sd[2] = 0; sd[3] = 2 * D.trimW[j] * r * 180 / kPi;
+      }
+    }
+    for (int e = 0; e < nE; ++e) {
+      const SSL double* r = ev + 7 * e;
+      const bool fired = r[0] != 0;
+      const long long k = static_cast<long long>(r[1]);
+      double ts;
+      if (fired && k > 1) ts = r[5] - dt * r[2] / (r[2] - r[3]);
+      else if (fired) ts = r[5];
+      else ts = tEnd;
+      const double res = ts - obs[nT + e];
+      L += D.evW[e] * (res * res);
+      if (sigOut) sigOut[nT + e] = ts;
+      if (fired && k > 1) {
+        const double Gt = 2 * D.evW[e] * res, s = r[2], sp = r[3], del = s - sp;
+        const double gs = Gt * dt * sp / (del * del), gsp = -Gt * dt * s / (del * del);
+        const int kind = D.evKind[e] == 0 ? 1 : 2, idx = D.evIdx[e];
+        const double sign = D.evKind[e] == 0 ? 1.0 : (P.M.mkWhen[idx] == 0 ? -1.0 : 1.0);
+        SSL double* a = T.seeds + 4 * ns++;
+        a[0] = static_cast<double>(k); a[1] = kind; a[2] = idx; a[3] = sign * gs;
+        SSL double* b = T.seeds + 4 * ns++;
+        b[0] = static_cast<double>(k - 1); b[1] = kind; b[2] = idx; b[3] = sign * gsp;
+      }
+    }
+    T.nseeds[0] = ns;
+    D.loss[inst] = L;
+    D.steps[inst] = done;
+  }
+  __syncthreads();
+  if (!D.wantGrad) return;
+
+  // 3. reverse
+  for (int i = tid; i < 6 + 5 * NN; i += kTPB) { T.lamA[i] = 0; T.lamB[i] = 0; }
+  for (int p = tid; p < AM.np; p += kTPB) T.G[p] = 0;
+  __syncthreads();
+  SSL double* lamCur = T.lamA;
+  SSL double* lamNext = T.lamB;
+  const size_t iC = static_cast<size_t>(Bt.connOff[inst]);
+  const long long nSeg = (done + K - 1) / K;
+  for (long long seg = nSeg - 1; seg >= 0; --seg) {
+    restore_state(P, B, ckpt + seg * SS, tid, kTPB);
+    if (tid == 0) *B.nextGroup = 0;
+    __syncthreads();
+    if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
+    __syncthreads();
+    const long long len = std::min(K, done - seg * K);
+    for (long long i = 0; i < len; ++i) {
+      save_state(P, B, ring + i * SS, tid, kTPB);
+      __syncthreads();
+      device_step<false, kTPB>(pa, lds, tid);
+    }
+    save_state(P, B, ring + len * SS, tid, kTPB);
+    __threadfence_block();
+    __syncthreads();
+    for (long long i = len - 1; i >= 0; --i) {
+      const long long k = seg * K + i + 1;
+      load_tape_state(P, T, ring + i * SS, tid, kTPB);
+      __syncthreads();
+      jac_step<kTPB>(P, AM, B, T, tid);
+      verify_step(P, T, ring + (i + 1) * SS, tid, kTPB, (SSL int32_t*)&bad);
+      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext, D.pa + iC, D.pc + iC);
+      SSL double* sw = lamCur;
+      lamCur = lamNext;
+      lamNext = sw;
+    }
+  }
+  __syncthreads();
+  if (tid < AM.np) {
+    const double th = D.theta[static_cast<size_t>(inst) * AM.np + tid];
+    D.grad[static_cast<size_t>(inst) * AM.np + tid] = T.G[tid] / th;
+  }
+  if (tid == 0) D.mismatch[inst] = bad;
+}
+
+// One step of instance 0 from a given state record, with a given adjoint of the state after it and given seeds on
+// the step's monitor sums and mark world z: the adjoint of the state before it, the parameters' log-derivatives and
+// the recomputed state after the step (the test hook of the step's reverse).
+__global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
+                                                          const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
+  BC& Bt = *(BC*)batch;
+  DC& D = *(DC*)adj;
+  extern __shared__ double lds_raw[];
+  SSL double* lds = (SSL double*)lds_raw;
+  const int tid = threadIdx.x;
+  const int inst = 0;
+  const int mi = Bt.instModel[inst];
+  PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(mi) * sizeof(ModelParams));
+  AMC& AMs = *(AMC*)((const SSC char*)amodels + static_cast<size_t>(mi) * sizeof(AdjModel));
+  AdjModel AM{};
+  AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
+  const Blk B = carve(lds, P);
+  const Tape T = carve_tape(lds + P.L.total, AMs.A);
+  const int NN = P.S.nn;
+  load_instance(P, Bt, D, B, inst, tid, kTPB);
+  // the record (t zO pitch roll vz wth wph inflow Vb hullTop, vol level cx cy cz) into the tape's input slots
+  for (int i = tid; i < 7 + 5 * NN; i += kTPB) {
+    const double v = D.stepRec[i < 7 ? i : i + 3];
+    if (i < 7) T.sv[i] = v;
+    else {
+      const int j = i - 7, a = j / NN, n = j - a * NN;
+      SSL double* d = a == 0 ? T.iVol : a == 1 ? T.iLvl : a == 2 ? T.iCx : a == 3 ? T.iCy : T.iCz;
+      d[n] = v;
+    }
+  }
+  for (int i = tid; i < 6 + 5 * NN; i += kTPB) T.lamA[i] = D.stepLam[i < 6 ? i + 1 : i + 4];
+  for (int p = tid; p < AM.np; p += kTPB) T.G[p] = 0;
+  if (tid == 0) {
+    int ns = 0;
+    for (int m = 0; m < P.M.nmon; ++m)
+      if (D.stepMon[m] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 1; s[2] = m; s[3] = D.stepMon[m]; }
+    for (int m = 0; m < P.M.nmark; ++m)
+      if (D.stepMark[m] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 2; s[2] = m; s[3] = D.stepMark[m]; }
+    T.nseeds[0] = ns;
+  }
+  __syncthreads();
+  jac_step<kTPB>(P, AM, B, T, tid);
+  reverse_step<kTPB>(P, AM, B, T, tid, 1, T.lamA, T.lamB, D.pa, D.pc);
+  for (int i = tid; i < 10 + 5 * NN; i += kTPB) {
+    double lam = 0, rec = 0;
+    if (i >= 1 && i <= 6) lam = T.lamB[i - 1];
+    else if (i >= 10) lam = T.lamB[i - 4];
+    const SSL double* o = T.tot;
+    if (i == 0) rec = T.sv[0] + P.M.dt;
+    else if (i <= 6) rec = i == 1 ? o[T_ZO1] : i == 2 ? o[T_P1] : i == 3 ? o[T_R1] : i == 4 ? o[T_VZ1] : i == 5 ? o[T_WTH1] : o[T_WPH1];
+    else if (i == 8) rec = o[T_V];
+    else if (i >= 10) {
+      const int j = i - 10, a = j / NN, n = j - a * NN;
+      const SSL double* s = a == 0 ? T.vol1 : a == 1 ? T.oLvl : a == 2 ? T.oCx : a == 3 ? T.oCy : T.oCz;
+      rec = s[n];
+    }
+    D.stepLamOut[i] = lam;
+    D.stepRecOut[i] = rec;
+  }
+  for (int p = tid; p < AM.np; p += kTPB) D.grad[p] = T.G[p];
+}
+
+}  // namespace ssa
+
+namespace {
+
+// LDS layout of the adjoint's tape, in doubles (after the forward layout).
+AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig) {
+  AdjLayout L{};
+  int32_t off = 0;
+  auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
+  L.sv = take(8); L.iVol = take(NN); L.iLvl = take(NN); L.iCx = take(NN); L.iCy = take(NN); L.iCz = take(NN);
+  L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(16 * kUnits); L.bt = take(16);
+  L.oLvl = take(NN); L.oVol = take(NN); L.oArea = take(NN); L.oCx = take(NN); L.oCy = take(NN); L.oCz = take(NN); L.oZmax = take(NN);
+  L.jac = take(NN * kSolveOut * kDof); L.carry = take(NN); L.gMer = take(NG); L.gH0 = take(NG);
+  L.mp = take(kUnits * maxGroup * kPassRec);
+  L.hEff = take(NN); L.aEff = take(NN); L.aSrc = take(NN); L.deg = take(NN); L.dV = take(NC); L.cp = take(6 * NC);
+  L.acc = take(NN); L.vol1 = take(NN); L.clip = take(NN); L.tot = take(T_COUNT);
+  L.lamA = take(6 + 5 * NN); L.lamB = take(6 + 5 * NN); L.G = take(np); L.gOut = take(kSolveOut * NN); L.gVol1 = take(NN);
+  L.gAcc = take(NN); L.cg = take(5 * NC); L.red = take(3 * (kThreads / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
+  L.sig = take(nsig);
+  L.total = off;
+  return L;
+}
+
+}  // namespace
+
+namespace ssa {
+namespace {
+
+// The gradient run's device buffers and views, for one call.
+struct AdjRun {
+  Dev<AdjModel> dAdjModels;
+  Dev<DevAdj> dAdj;
+  Dev<double> theta, ca, trimW, evW, obs, loss, grad, signals, ckpt, ring, stepRec, stepLam, stepMon, stepMark, stepLamOut, stepRecOut;
+  Dev<int32_t> pa, pc, evKind, evIdx, mismatch;
+  Dev<long long> trimStep, steps, ckptOff, ringOff;
+  size_t lds = 0;
+};
+
+struct Setup {
+  int32_t np;
+  const int32_t *pa, *pc;
+  const double* theta;
+  int32_t relaxed;
+};
+
+// Checks, uploads the batch and the adjoint's tables; returns the launch's LDS size.
+void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const double* obs, int32_t K, double tMax, long long maxSteps,
+             bool oneStep) {
+  const size_t count = static_cast<size_t>(b->opt.count);
+  need(s.np >= 1 && s.np <= 64, "ss_batch_gradient: the parameter count must be 1..64");
+  need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
+  need(b->tpb == kThreads, "ss_batch_gradient: the gradient runs on 256-thread workgroups (set_launch 256 or 0)");
+  need(!b->mixed, "ss_batch_gradient: the gradient is FP64 (portable numerics)");
+  const int32_t nT = loss ? loss->nTrim : 0, nE = loss ? loss->nEvent : 0;
+  need(nT >= 0 && nE >= 0 && nT + nE <= 256, "ss_batch_gradient: at most 256 loss terms");
+  const size_t nconn = b->hArea.size();
+  for (size_t k = 0; k < nconn; ++k) need(s.pa[k] >= -1 && s.pa[k] < s.np && s.pc[k] >= -1 && s.pc[k] < s.np, "ss_batch_gradient: parameter index out of range");
+  for (size_t k = 0; k < count * static_cast<size_t>(s.np); ++k) need(s.theta[k] > 0 && std::isfinite(s.theta[k]), "ss_batch_gradient: parameters must be positive");
+  for (int32_t e = 0; e < nE; ++e) {
+    need(loss->evKind[e] == 0 || loss->evKind[e] == 1, "ss_batch_gradient: event kind must be 0 (monitor) or 1 (mark)");
+    for (size_t i = 0; i < count; ++i) {
+      const Model& m = b->model(static_cast<int32_t>(i));
+      need(loss->evIdx[e] >= 0 && loss->evIdx[e] < (loss->evKind[e] == 0 ? m.nmon : m.nmark), "ss_batch_gradient: event index out of range");
+    }
+  }
+  // the scaled products coef * theta[pc] * area * theta[pa], formed as the sweep forms area * scale and the flow law
+  // multiplies coef by area
+  std::vector<double> ca(nconn);
+  for (size_t i = 0; i < count; ++i) {
+    const Model& m = b->model(static_cast<int32_t>(i));
+    const size_t o = static_cast<size_t>(b->connOff[i]);
+    const double* th = s.theta + i * static_cast<size_t>(s.np);
+    for (int32_t c = 0; c < m.NC; ++c) {
+      const size_t k = o + static_cast<size_t>(c);
+      const double area = s.pa[k] >= 0 ? b->hArea[k] * th[s.pa[k]] : b->hArea[k];
+      const double coef = s.pc[k] >= 0 ? m.coef0[static_cast<size_t>(c)] * th[s.pc[k]] : m.coef0[static_cast<size_t>(c)];
+      ca[k] = coef * area;
+    }
+  }
+  upload(b);
+  R.ca.upload(ca);
+  R.theta.upload(s.theta, count * static_cast<size_t>(s.np));
+  R.pa.upload(s.pa, nconn);
+  R.pc.upload(s.pc, nconn);
+  // per model: the tape layout
+  const int32_t nseeds = nT + 2 * nE + 64 + 64;  // the one-step form seeds every monitor and mark
+  std::vector<AdjModel> am;
+  size_t lds = 0;
+  for (const auto& m : b->models) {
+    need(m->maxGroup <= 8, "ss_batch_gradient: free-surface groups of more than 8 spaces are not supported");
+    need(m->nmon <= 64 && m->nmark <= 64, "ss_batch_gradient: at most 64 monitors and 64 marks");
+    AdjModel a{};
+    a.A = adj_layout(m->NN, m->NC, m->NG, m->maxGroup, s.np, nseeds, 2 * nT + 7 * nE);
+    a.maxGroup = m->maxGroup;
+    a.np = s.np;
+    a.mode = s.relaxed ? 1 : 0;
+    am.push_back(a);
+    lds = std::max(lds, (static_cast<size_t>(m->layout(kThreads).total) + static_cast<size_t>(a.A.total)) * sizeof(double));
+  }
+  if (lds > lds_limit())
+    fail(SS_RUNTIME, "ss_batch_gradient: " + std::to_string(lds) + " bytes of LDS per simulation exceed the device's limit");
+  R.lds = lds;
+  R.dAdjModels.upload(am);
+  // checkpoints and rings per instance
+  std::vector<long long> ckOff(count), rgOff(count);
+  long long ck = 0, rg = 0;
+  for (size_t i = 0; i < count; ++i) {
+    const Model& m = b->model(static_cast<int32_t>(i));
+    const long long SS = 7 + 5 * static_cast<long long>(m.NN);
+    const double span = std::max(0.0, tMax - b->hSt[i * 7]);
+    const double bound = span / m.net.dt + 2;
+    long long cap = maxSteps > 0 ? maxSteps : LLONG_MAX;
+    if (bound < 1e9) cap = std::min(cap, static_cast<long long>(std::ceil(bound)));
+    need(cap <= 100000000LL, "ss_batch_gradient: the run is unbounded (set tMax or maxSteps to at most 1e8 steps)");
+    ckOff[i] = ck;
+    rgOff[i] = rg;
+    if (!oneStep) {
+      ck += (cap / K + 1) * SS;
+      rg += (static_cast<long long>(K) + 1) * SS;
+    }
+  }
+  R.ckpt.alloc(static_cast<size_t>(std::max(ck, 1LL)));
+  R.ring.alloc(static_cast<size_t>(std::max(rg, 1LL)));
+  R.ckptOff.upload(ckOff);
+  R.ringOff.upload(rgOff);
+  if (nT > 0) { R.trimStep.upload(reinterpret_cast<const long long*>(loss->trimStep), static_cast<size_t>(nT)); R.trimW.upload(loss->trimW, static_cast<size_t>(nT)); }
+  if (nE > 0) {
+    R.evKind.upload(loss->evKind, static_cast<size_t>(nE)); R.evIdx.upload(loss->evIdx, static_cast<size_t>(nE));
+    R.evW.upload(loss->evW, static_cast<size_t>(nE));
+  }
+  const size_t nSig = static_cast<size_t>(nT + nE);
+  if (nSig > 0) {
+    if (obs) R.obs.upload(obs, count * nSig);
+    else R.obs.fill(0.0, count * nSig);
+  }
+  R.loss.fill(0.0, count);
+  R.grad.fill(0.0, std::max<size_t>(count * static_cast<size_t>(s.np), static_cast<size_t>(s.np)));
+  R.signals.fill(0.0, std::max<size_t>(count * nSig, 1));
+  R.steps.fill(0, count);
+  R.mismatch.fill(0, count);
+  DevAdj d{};
+  d.K = K; d.wantGrad = 1; d.maxSteps = maxSteps > 0 ? maxSteps : LLONG_MAX; d.tMax = tMax;
+  d.theta = G(R.theta.p); d.pa = G(R.pa.p); d.pc = G(R.pc.p); d.ca = G(R.ca.p);
+  d.nTrim = nT; d.nEv = nE;
+  d.trimStep = (const SSG long long*)R.trimStep.p; d.trimW = G(R.trimW.p); d.trimAfter = loss ? loss->trimAfter : 0;
+  d.evKind = G(R.evKind.p); d.evIdx = G(R.evIdx.p); d.evW = G(R.evW.p); d.obs = G(R.obs.p);
+  d.loss = W(R.loss.p); d.grad = W(R.grad.p); d.signals = W(R.signals.p); d.steps = W(R.steps.p); d.mismatch = W(R.mismatch.p);
+  d.ckpt = W(R.ckpt.p); d.ring = W(R.ring.p);
+  d.ckptOff = (const SSG long long*)R.ckptOff.p; d.ringOff = (const SSG long long*)R.ringOff.p;
+  R.dAdj.upload(&d, 1);
+}
+
+void launch(void (*kernel)(const ModelParams*, const AdjModel*, const DevBatch*, const DevAdj*), ss_batch* b, AdjRun& R, int grid, double* ms) {
+  hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),
+            "hipFuncSetAttribute");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), R.lds, nullptr, static_cast<const ModelParams*>(b->dModels.p),
+                     static_cast<const AdjModel*>(R.dAdjModels.p), static_cast<const DevBatch*>(b->dBatch.p),
+                     static_cast<const DevAdj*>(R.dAdj.p));
+  hip_check(hipGetLastError(), "adjoint launch");
+  hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+  hip_check(hipEventSynchronize(e1), "adjoint kernel");
+  float t = 0;
+  hip_check(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms) *ms = t;
+}
+
+}  // namespace
+}  // namespace ssa
+
+extern "C" {
+
+ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta, const ss_loss* loss,
+                            const double* observations, const ss_gradient_options* opt, double* lossOut, double* grad, double* signals,
+                            int64_t* steps, int32_t* mismatches, double* kernelMs) {
+  return guarded([&] {
+    need(b != nullptr && opt != nullptr, "ss_batch_gradient: null argument");
+    need(lossOut != nullptr, "ss_batch_gradient: null loss output");
+    need(opt->checkpointEvery >= 1 && opt->checkpointEvery <= 65536, "ss_batch_gradient: checkpointEvery must be 1..65536");
+    need(opt->relaxed == 0 || opt->relaxed == 1, "ss_batch_gradient: relaxed must be 0 or 1");
+    const double tMax = opt->tMax > 0 ? opt->tMax : b->opt.tMax;
+    AdjRun R;
+    prepare(b, R, Setup{nparams, pa, pc, theta, opt->relaxed}, loss, observations, opt->checkpointEvery, tMax, opt->maxSteps, false);
+    if (!grad) {
+      DevAdj d{};
+      R.dAdj.download(&d, 1);
+      d.wantGrad = 0;
+      R.dAdj.upload(&d, 1);
+    }
+    double ms = 0;
+    launch(adjoint_run, b, R, b->opt.count, &ms);
+    const size_t count = static_cast<size_t>(b->opt.count);
+    R.loss.download(lossOut, count);
+    if (grad) R.grad.download(grad, count * static_cast<size_t>(nparams));
+    const size_t nSig = static_cast<size_t>(loss ? loss->nTrim + loss->nEvent : 0);
+    if (signals && nSig) R.signals.download(signals, count * nSig);
+    if (steps) {
+      std::vector<long long> s(count);
+      R.steps.download(s.data(), count);
+      for (size_t i = 0; i < count; ++i) steps[i] = s[i];
+    }
+    if (mismatches) R.mismatch.download(mismatches, count);
+    if (kernelMs) *kernelMs = ms;
+  });
+}
+
+ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta, int32_t relaxed,
+                                const double* record, const double* lambda, const double* monitorSeed, const double* markSeed, double* lambdaOut,
+                                double* gradLog, double* recordOut) {
+  return guarded([&] {
+    need(b != nullptr && record != nullptr && lambda != nullptr && lambdaOut != nullptr && gradLog != nullptr && recordOut != nullptr,
+         "ss_batch_adjoint_step: null argument");
+    AdjRun R;
+    prepare(b, R, Setup{nparams, pa, pc, theta, relaxed}, nullptr, nullptr, 1, b->opt.tMax, 1, true);
+    const Model& m = b->model(0);
+    const size_t rs = 10 + 5 * static_cast<size_t>(m.NN);
+    R.stepRec.upload(record, rs);
+    R.stepLam.upload(lambda, rs);
+    std::vector<double> mon(static_cast<size_t>(std::max(m.nmon, 1)), 0.0), mark(static_cast<size_t>(std::max(m.nmark, 1)), 0.0);
+    if (monitorSeed) std::copy(monitorSeed, monitorSeed + m.nmon, mon.begin());
+    if (markSeed) std::copy(markSeed, markSeed + m.nmark, mark.begin());
+    R.stepMon.upload(mon);
+    R.stepMark.upload(mark);
+    R.stepLamOut.fill(0.0, rs);
+    R.stepRecOut.fill(0.0, rs);
+    DevAdj d{};
+    R.dAdj.download(&d, 1);
+    d.stepRec = G(R.stepRec.p); d.stepLam = G(R.stepLam.p); d.stepMon = G(R.stepMon.p); d.stepMark = G(R.stepMark.p);
+    d.stepLamOut = W(R.stepLamOut.p); d.stepRecOut = W(R.stepRecOut.p);
+    R.dAdj.upload(&d, 1);
+    launch(adjoint_one_step, b, R, 1, nullptr);
+    R.stepLamOut.download(lambdaOut, rs);
+    R.stepRecOut.download(recordOut, rs);
+    R.grad.download(gradLog, static_cast<size_t>(nparams));
+  });
+}
+
+}  // extern "C"

@@ -1122,7 +1122,7 @@ __device__ __forceinline__ PC& model_at(uint64_t& pa) {
 // TPB is the workgroup's size: 256, or 512, where the second 256 threads start the level solve while the first
 // 256 compute the buoyancy partials (both depend only on the pose and the state), then join it.
 template <bool Mixed, int TPB>
-__device__ void device_step(uint64_t pa, SSL double* lds, int tid) {
+__device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int tid) {
   // (a) buoyancy partials (threads 0..255), (b) the free surfaces (every unit, taking groups as it is free)
   {
     PC& P = model_at(pa);

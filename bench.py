@@ -42,9 +42,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven"],
+    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint"],
                     help="titanic: the 1,024-instance titanic64 batch (the headline); uneven: the persistent batch of "
-                         "seeded synthetic hulls of uneven size")
+                         "seeded synthetic hulls of uneven size; adjoint: gradients of the twin-experiment loss of 256 "
+                         "titanic64 instances")
+    ap.add_argument("--checkpoint", type=int, default=256, help="adjoint: steps between checkpoints")
+    ap.add_argument("--exact", action="store_true", help="adjoint: the discrete program's derivative instead of the relaxed one")
     ap.add_argument("--count", type=int, default=None, help="instances (default 1024 titanic, 4096 uneven)")
     ap.add_argument("--tmax", type=float, default=None, help="s (default 4 h titanic, 2 h uneven)")
     ap.add_argument("--numerics", default="portable", choices=["portable", "portable32"])
@@ -165,6 +168,65 @@ def bench_uneven(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_adjoint(args):
+    """Gradients of the twin-experiment loss: `count` titanic64 instances (default tables), 12 scale factors (the area
+    of breach, over, down and top; the discharge coefficient of every kind but open), loss = squared trim error at
+    15, 30, 45 and 75 min and squared error of the times of overF, well and bridge, weighted as the reference's
+    objective weighs them, against observations from seeded hidden parameters; the gradient at an independent seeded
+    guess, relaxed level derivatives, a checkpoint every K steps, to founder or 260 min. One bench step = one gradient
+    of every instance (ss_batch_gradient). value = gradients per second."""
+    import numpy as np
+    from sinksim_amd.batch import Batch, BatchOptions
+    from sinksim_amd.calib import kind_params, titanic_loss
+    from sinksim_amd.data_paths import titanic
+    from sinksim_amd.workloads import twin_experiment
+    count = args.count or 256
+    t_max = args.tmax or 260 * 60.0
+    ship, sim = titanic("titanic")
+    pm = kind_params(sim)
+    spec = titanic_loss(sim)
+    truth, guess = twin_experiment(count, pm.count)
+    b = Batch(ship, sim, BatchOptions(count=count, mixed_precision=False, traced_instances=0, curve_every=0.0, t_max=t_max))
+    twin = b.gradient(pm, truth, spec, t_max=t_max, want_grad=False)
+    obs = twin["signals"]
+    print(f"observations from the hidden parameters: forward {twin['ms']:.0f} ms, {int(twin['steps'].sum())} steps", file=sys.stderr, flush=True)
+    fwd = b.gradient(pm, guess, spec, observations=obs, t_max=t_max, want_grad=False)
+    print(f"forward only at the guess: {fwd['ms']:.0f} ms", file=sys.stderr, flush=True)
+
+    def one():
+        return b.gradient(pm, guess, spec, observations=obs, checkpoint_every=args.checkpoint, relaxed=not args.exact, t_max=t_max)
+
+    for w in range(args.warmup):
+        r = one()
+        print(f"warmup {w}: {r['ms']:.0f} ms", file=sys.stderr, flush=True)
+    tot_ms = tot_wall = 0.0
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        r = one()
+        wall = time.perf_counter() - t0
+        tot_ms += r["ms"]
+        tot_wall += wall
+        if int(r["mismatches"].sum()) != 0:
+            raise RuntimeError(f"{int(r['mismatches'].sum())} recomputed steps differ from the forward trajectory")
+        print(f"step {k}: {count} gradients in {r['ms']:.0f} ms kernel, {wall:.2f} s wall; mean loss {r['loss'].mean():.3f}",
+              file=sys.stderr, flush=True)
+    b.close()
+    steps = int(r["steps"].sum())
+    secs = tot_ms * 1e-3 / args.steps
+    out = {
+        "metric": "adjoint gradients/s (twin-experiment loss, 12 parameters)", "value": round(count / secs, 3),
+        "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * tot_wall / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded hidden parameters of the titanic64 model)",
+        "config": {"workload": f"{count} titanic64 instances, twin experiment, 12 scale factors, trim at 15/30/45/75 min + "
+                               f"overF/well/bridge times, relaxed={not args.exact}, checkpoint every {args.checkpoint} steps, "
+                               f"to founder or {t_max / 60:g} min", "instances": count, "sim_steps": steps,
+                   "forward_only_ms": round(fwd["ms"], 1), "gradient_over_forward": round(tot_ms / args.steps / fwd["ms"], 2),
+                   "device": _device()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -173,6 +235,9 @@ def main():
         sys.exit(2)
     if args.workload == "uneven":
         bench_uneven(args)
+        return
+    if args.workload == "adjoint":
+        bench_adjoint(args)
         return
     args.count = args.count or 1024
     from sinksim_amd import sweep as S

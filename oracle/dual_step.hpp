@@ -377,14 +377,17 @@ class Sim {
 
   // The relaxed derivative of a level: the Newton solve stops within a volume tolerance, so a level's derivative
   // in the discrete program is that of the last iterate (often the previous step's level, carried unchanged);
-  // relaxed, it is that of the exact root of vol(h) = V at the final level h: the tangent gains one Newton step,
-  // d h += (dV - dvol) / dv, with dv the pass's slope at h. The value of h is untouched. Returns whether it acted.
-  static bool relax_level(T& h, const T& resid, const T& dv) {
+  // relaxed, it is that of the root of vol(h) = V at the final level h: the tangent gains one Newton step,
+  // d h += (dV - dvol) / a, with a the pass's slope at h floored at the space's minimum effective area amin (the
+  // floor the flow limiter puts under the free-surface area, flows.hpp:211-216: near the top or bottom of a space
+  // the slope vanishes and 1 / dv would make the linearised flows unstable). The value of h is untouched.
+  static bool relax_level(T& h, const T& resid, const T& dv, double amin) {
     if constexpr (std::is_same_v<T, double>) {
       return false;
     } else {
-      if (!(dv > kSlopeMin)) return false;
-      for (int i = 0; i < (int)(sizeof(h.d) / sizeof(double)); ++i) h.d[i] += resid.d[i] / dv.v;
+      const double a = jmax(dv.v, amin);
+      if (!(a > kSlopeMin)) return false;
+      for (int i = 0; i < (int)(sizeof(h.d) / sizeof(double)); ++i) h.d[i] += resid.d[i] / a;
       return true;
     }
   }
@@ -424,7 +427,7 @@ class Sim {
       h = hn;
       p = node_pass(S_, F, n, h);
     }
-    if (P_.relaxed && relax_level(h, V - p.vol, p.dv)) p = node_pass(S_, F, n, h);
+    if (P_.relaxed && relax_level(h, V - p.vol, p.dv, S_.amin[n])) p = node_pass(S_, F, n, h);
     level[n] = h; area_[n] = p.dv;
     set_centroid(n, p);
   }
@@ -499,8 +502,9 @@ class Sim {
     }
     if (P_.relaxed) {
       T sum = T(0), dv = T(0);
-      for (Index k = 0; k < N; ++k) { sum += P[k].vol; dv += P[k].dv; }
-      if (relax_level(h, V - sum, dv))
+      double amin = 0;
+      for (Index k = 0; k < N; ++k) { sum += P[k].vol; dv += P[k].dv; amin += S_.amin[members[k]]; }
+      if (relax_level(h, V - sum, dv, amin))
         for (Index k = 0; k < N; ++k) P[k] = node_pass(S_, F, members[k], h);
     }
     T tot = T(0);

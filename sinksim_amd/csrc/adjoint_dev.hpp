@@ -17,8 +17,8 @@
 // Conventions (identical in the CPU dual-number build, oracle/dual_step.hpp, which the tests hold this to):
 // sin/cos differentiate to the fdlibm cos/-sin of the same argument; pow(x, y) to y * pow(x, y) / x (0 at x = 0);
 // sqrt to 0.5 / sqrt(x) (0 at 0); a selection or clamp to its selected operand. The relaxed level derivative
-// replaces a level's tangent by that of the exact root of vol(h) = V at the final level (one Newton step on the
-// tangent), as if the solve had converged.
+// replaces a level's tangent by that of the root of vol(h) = V at the final level (one Newton step on the tangent,
+// with the slope floored at the space's minimum effective area amin), as if the solve had converged.
 #pragma once
 
 #include "step_dev.hpp"
@@ -298,9 +298,12 @@ __device__ inline void solve_single_tv(PC& P, const Blk& B, const Tape& T, int l
     r = pass_rec(P, B, fr, lane, n, h.v);
     p = pass_tv(r, h.d, f, mu, iR);
   }
-  if (relaxed && p.dv.v > kSlopeMin) {
-    h.d += (V.d - p.vol.d) / p.dv.v;
-    p = pass_tv(r, h.d, f, mu, iR);
+  if (relaxed) {
+    const double as = jmax(p.dv.v, B.nAmin[n]);
+    if (as > kSlopeMin) {
+      h.d += (V.d - p.vol.d) / as;
+      p = pass_tv(r, h.d, f, mu, iR);
+    }
   }
   const bool c = !(p.vol.v > kCentroidMinVolume);
   put_out(T, n, lane, h, V, p.dv, c ? cx0 : p.mx / p.vol, c ? cy0 : p.my / p.vol, c ? cz0 : p.mz / p.vol, c, hi);
@@ -409,9 +412,11 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const Tape& T, SSL do
   }
   if (relaxed) {
     TV sum = tc(0.0), dv = tc(0.0);
-    for (int k = 0; k < N; ++k) { const PassTV q = Pk(k); sum = sum + q.vol; dv = dv + q.dv; }
-    if (dv.v > kSlopeMin) {
-      h.d += (V.d - sum.d) / dv.v;
+    double amin = 0;
+    for (int k = 0; k < N; ++k) { const PassTV q = Pk(k); sum = sum + q.vol; dv = dv + q.dv; amin += B.nAmin[members[k]]; }
+    const double as = jmax(dv.v, amin);
+    if (as > kSlopeMin) {
+      h.d += (V.d - sum.d) / as;
       hP = h;
     }
   }

@@ -69,3 +69,13 @@ def uneven_fleet(count: int = 4096, seed: int = 20261008, processes: int = 0):
     else:
         models = [synthetic_hull(s) for s in seeds]
     return models, np.arange(count, dtype=np.int32)
+
+
+def twin_experiment(count: int = 256, nparams: int = 12, seed: int = 20261008, span: float = 0.3):
+    """The gradient workload's parameters: per instance hidden true scale factors and an independent starting guess,
+    both log-uniform in [exp(-span), exp(span)] (the reference's search box, tools/py/calibrate_gpu.py:90-91),
+    from one seed. Returns (theta_true, theta_guess), each [count, nparams]."""
+    rng = np.random.default_rng(seed)
+    truth = np.exp(rng.uniform(-span, span, (count, nparams)))
+    guess = np.exp(rng.uniform(-span, span, (count, nparams)))
+    return truth, guess

@@ -42,10 +42,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint"],
+    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint", "calibration"],
                     help="titanic: the 1,024-instance titanic64 batch (the headline); uneven: the persistent batch of "
                          "seeded synthetic hulls of uneven size; adjoint: gradients of the twin-experiment loss of 256 "
-                         "titanic64 instances")
+                         "titanic64 instances; calibration: the reference's 1,320-point search against gradient L-BFGS "
+                         "on the reference objective")
+    ap.add_argument("--starts", type=int, default=8, help="calibration: L-BFGS starts")
+    ap.add_argument("--iterations", type=int, default=12, help="calibration: adjoint calls of the L-BFGS at most")
     ap.add_argument("--checkpoint", type=int, default=256, help="adjoint: steps between checkpoints")
     ap.add_argument("--exact", action="store_true", help="adjoint: the discrete program's derivative instead of the relaxed one")
     ap.add_argument("--count", type=int, default=None, help="instances (default 1024 titanic, 4096 uneven)")
@@ -227,6 +230,40 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_calibration(args):
+    """The reference's calibration (tools/py/calibrate_gpu.py: 5 rounds of 264 log-uniform points around the best,
+    span 0.3 shrinking by half, seed 20261008, objective J against the 1912 observations) on the GPU batch, then
+    batched L-BFGS with adjoint gradients of the same J from scales 1 and seeded points of the first box, until it
+    reaches the search's best J. value = the search's device time / the L-BFGS device time to that objective."""
+    import numpy as np
+    from sinksim_amd.calibrate import compare
+    from sinksim_amd.data_paths import titanic
+    ship, sim = titanic("titanic")
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    r = compare(ship, sim, starts=args.starts, iterations=args.iterations, log=log)
+    sw, lb = r["search"], r["lbfgs"]
+    reached = r["lbfgs_ms_to_search_J"]
+    out = {
+        "metric": "calibration device time, reference search / adjoint L-BFGS at equal objective",
+        "value": round(r["search_ms"] / reached, 3) if reached else None, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * r["wall_s"], 1), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "titanic64 model, 1912 observations (data/observations/titanic-1912.json)",
+        "config": {"workload": "titanic64 scenario titanic, 4 kind scales (breach over down top), objective J of the reference",
+                   "search": {"evaluations": sw.evaluations[-1], "best_J": round(sw.best_J[-1], 4), "device_s": round(sw.device_ms[-1] / 1e3, 3),
+                              "sim_steps": sw.steps[-1], "best_scales": [round(float(np.exp(v)), 4) for v in sw.best_x[-1]],
+                              "J_by_round": [round(j, 4) for j in sw.best_J]},
+                   "lbfgs": {"starts": args.starts, "calls": len(lb.best_J), "best_J": round(lb.best_J[-1], 4),
+                             "device_s": round(lb.device_ms[-1] / 1e3, 3), "device_s_to_search_J": round(reached / 1e3, 3) if reached else None,
+                             "sim_steps": lb.steps[-1], "best_scales": [round(float(np.exp(v)), 4) for v in lb.best_x[-1]],
+                             "J_by_call": [round(j, 4) for j in lb.best_J]},
+                   "device": _device()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -238,6 +275,9 @@ def main():
         return
     if args.workload == "adjoint":
         bench_adjoint(args)
+        return
+    if args.workload == "calibration":
+        bench_calibration(args)
         return
     args.count = args.count or 1024
     from sinksim_amd import sweep as S

@@ -228,21 +228,27 @@ ss_status ss_batch_kernel_time(const ss_batch* b, double* ms, int32_t* launches)
  * Parameters theta are positive scale factors: connection c of instance i takes area * theta[i][pa[c]] and discharge
  * coefficient coef * theta[i][pc[c]] (-1: no factor); pa and pc are laid out like the connection tables.
  *
- * The loss of an instance is
- *   sum_j trimW[j] * (trimDeg after step trimStep[j] - obs_j)^2  +  sum_e evW[e] * (t*_e - obs_e)^2,
- * trimDeg = pitch * 180 / pi (trimAfter when the run ended before that step), and t*_e the time at which monitor
- * (evKind 0) or mark (evKind 1) evIdx first fires, interpolated linearly between the step before and the step at which
- * it fires from the event's signed signal (monitor: flow sum - threshold; mark: -world z, or +world z for an 'above'
- * mark): t* = t_k - dt * s_k / (s_k - s_{k-1}). An event that never fires scores the run's end time. */
+ * The loss of an instance is a weighted sum of squared errors against observations (extending the reference's
+ * calibration objective, tools/py/calibrate_gpu.py:35-64, to terms with derivatives):
+ *   readout terms j: the trim (rdKind 0: pitch * 180 / pi) or list (rdKind 1: roll * 180 / pi) after step rdStep[j];
+ *                    rdAfter[j] when the run ended before that step (the reference's values after foundering);
+ *   event terms e:   the time at which monitor evIdx (evKind 0) first exceeds the threshold, mark evIdx (evKind 1)
+ *                    first goes under (or above), or the ship founders (evKind 2), interpolated linearly between the
+ *                    step before and the step at which it fires from the event's signed signal s (monitor: flow sum
+ *                    - threshold; mark: -world z, or +world z for an 'above' mark; founder: the largest margin of the
+ *                    founder rule): t* = t_k - dt * s_k / (s_k - s_{k-1}). An event that never fires scores the founder
+ *                    time if the ship foundered, else evMissing[e]. */
 typedef struct ss_loss {
-  int32_t nTrim;
-  const int64_t* trimStep;
-  const double* trimW;
-  double trimAfter;
+  int32_t nRead;
+  const int64_t* rdStep;
+  const int32_t* rdKind;
+  const double* rdW;
+  const double* rdAfter;
   int32_t nEvent;
   const int32_t* evKind;
   const int32_t* evIdx;
   const double* evW;
+  const double* evMissing;
 } ss_loss;
 
 /* checkpointEvery: K, the steps between stored states (the reverse sweep recomputes each segment of K steps from its
@@ -257,7 +263,7 @@ typedef struct ss_gradient_options {
 } ss_gradient_options;
 
 /* Runs every instance from its staged initial state and returns its loss [count], the loss's gradient with respect to
- * theta [count * nparams] (null: forward only, no reverse sweep), the loss's signals [count * (nTrim + nEvent)]: trim
+ * theta [count * nparams] (null: forward only, no reverse sweep), the loss's signals [count * (nRead + nEvent)]: readout
  * values then event times (may be null), the steps run [count], the recomputed steps whose values differed from the
  * forward's [count] (always 0 for a correct kernel; may be null) and the kernel time (ms, may be null). FP64
  * (portable) batches on 256-thread workgroups only. The batch may be run any number of times this way; its own
@@ -267,7 +273,8 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
                             double* grad, double* signals, int64_t* steps, int32_t* mismatches, double* kernelMs);
 
 /* One step of instance 0 in reverse, the test hook of the adjoint: from a state record (the trace layout), the
- * adjoint of the state after the step (same layout; entries 0 and 7..9 ignored) and seeds on the step's monitor flow
+ * adjoint of the state after the step (same layout; entries 8 and 9 seed the step's buoyancy volume and hull top,
+ * entries 0 and 7 are ignored) and seeds on the step's monitor flow
  * sums [nmonitor] and marks' world z [nmark] (either may be null), the adjoint of the state before it (lambdaOut,
  * trace layout), the loss's derivative with respect to log theta (gradLog [nparams]) and the recomputed state after
  * the step (recordOut, trace layout; inflow and hull top 0). */

@@ -284,26 +284,6 @@ struct Params {
   int relaxed;      // 1: the level's derivative as if each solve converged (see relax_level)
 };
 
-// The loss of a run (no counterpart in the reference): squared errors of the trim at given steps and of
-// interpolated event times against observations, weighted.
-//   trim term j:  w * (trimDeg after step k_j - obs)^2, trimDeg = pitch * 180 / pi (readouts.hpp:181); a step
-//                 after the end of the run contributes trimAfter (no derivative);
-//   event term e: w * (t* - obs)^2 for monitor (kind 0) or mark (kind 1) idx. With s_k the event's signed
-//                 signal at step k (monitor: flow sum - threshold; mark: -world z, or +world z for an
-//                 'above' mark) and k the first step at which it fires (s_k > 0, the rule of events.hpp:11-38),
-//                 t* = t_k - dt * s_k / (s_k - s_{k-1}), the linear interpolation of the crossing; an event
-//                 that fires at step 1 gives t_1, and one that never fires the run's end time (no
-//                 derivative).
-struct LossSpec {
-  Index nTrim;
-  const std::int64_t* trimStep;
-  const double *trimObs, *trimW;
-  double trimAfter;
-  Index nEv;
-  const Index *evKind, *evIdx;
-  const double *evObs, *evW;
-};
-
 template <class T>
 class Sim {
  public:

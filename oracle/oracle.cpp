@@ -450,15 +450,27 @@ struct orc_params {
   int32_t relaxed;  // 1: relaxed level derivatives (dual_step.hpp relax_level)
 };
 
-// The loss of a run (dual_step.hpp LossSpec).
+// The loss of a run (no counterpart in the reference; it extends the reference's calibration objective,
+// tools/py/calibrate_gpu.py:35-64, to terms with derivatives): weighted squared errors against observations of
+//   readout terms j: the trim (kind 0: pitch * 180 / pi) or list (kind 1: roll * 180 / pi) after step rdStep[j]
+//                    (readouts.hpp:181-182); a step after the end of the run scores rdAfter[j] (no derivative), as
+//                    the reference objective scores trim and list after foundering;
+//   event terms e:   the time at which monitor idx (kind 0) first exceeds the threshold, mark idx (kind 1) first
+//                    goes under (or above), or the ship founders (kind 2), from the event's signed signal s_k at
+//                    step k: monitor flow sum - threshold; -world z of an 'under' mark, +world z of an 'above' one;
+//                    for foundering the largest margin of the founder rule (events.hpp:31-37): hullTopBelow - hull
+//                    top, |pitch| - pitchAbsAbove, and after minT buoyancyBelow - V (a tie keeps the earlier). With
+//                    k the first step at which s_k > 0, t* = t_k - dt * s_k / (s_k - s_{k-1}), the interpolated
+//                    crossing (t_1 if k = 1). An event that never fires scores the founder time if the ship
+//                    foundered and evMissing[e] if it stayed afloat, as the reference objective does (no derivative).
 struct orc_loss {
-  int32_t nTrim;
-  const int64_t* trimStep;
-  const double *trimObs, *trimW;
-  double trimAfter;
+  int32_t nRead;
+  const int64_t* rdStep;
+  const int32_t* rdKind;
+  const double *rdObs, *rdW, *rdAfter;
   int32_t nEv;
   const int32_t *evKind, *evIdx;
-  const double *evObs, *evW;
+  const double *evObs, *evW, *evMissing;
 };
 
 }  // extern "C"
@@ -513,11 +525,19 @@ void restore_record(dual::Sim<T>& d, Index NN, int nd, const double* rec, const 
   d.restore(rec[0], s6.data(), v.data(), v.data() + NN, v.data() + 2 * NN, v.data() + 3 * NN, v.data() + 4 * NN);
 }
 
-// The event signal of loss event e in the last step: monitor sum - threshold, or -z / +z of an under / above mark.
+// The signed signal of loss event (kind, idx) in the last step (see orc_loss).
 template <class T>
 T event_signal(const dual::Sim<T>& d, const Net& M, int32_t kind, int32_t idx) {
   if (kind == 0) return d.over_[idx] - M.monThreshold;
-  return M.mkWhen[idx] == 0 ? -d.markZ_[idx] : d.markZ_[idx];
+  if (kind == 1) return M.mkWhen[idx] == 0 ? -d.markZ_[idx] : d.markZ_[idx];
+  T s = M.hullTopBelow - d.hullTop;
+  const T sp = dual::dfabs(d.pitch) - M.pitchAbsAbove;
+  if (sp > s) s = sp;
+  if (d.t > M.minT) {
+    const T sb = M.buoyancyBelow - d.Vb;
+    if (sb > s) s = sb;
+  }
+  return s;
 }
 
 template <int N>
@@ -544,15 +564,15 @@ int64_t dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, const d
   lift(m->cz0, 0, nullptr, 0, v.data() + 4 * NN, NN);
   d.restore(m->state0[0], s6.data(), v.data(), v.data() + NN, v.data() + 2 * NN, v.data() + 3 * NN, v.data() + 4 * NN);
 
-  const int32_t nT = L ? L->nTrim : 0, nE = L ? L->nEv : 0;
-  std::vector<T> trim(nT), tstar(nE), prev(nE);
-  std::vector<char> trimSet(nT, 0), fired(nE, 0);
+  const int32_t nT = L ? L->nRead : 0, nE = L ? L->nEv : 0;
+  std::vector<T> rd(nT), tstar(nE), prev(nE);
+  std::vector<char> rdSet(nT, 0), fired(nE, 0);
   int64_t steps = 0;
   while (d.t < tMax && !d.foundered && steps < maxSteps) {
     d.step();
     ++steps;
     for (int32_t j = 0; j < nT; ++j)
-      if (L->trimStep[j] == steps) { trim[j] = d.pitch * 180.0 / kPi; trimSet[j] = 1; }
+      if (L->rdStep[j] == steps) { rd[j] = (L->rdKind[j] == 0 ? d.pitch : d.roll) * 180.0 / kPi; rdSet[j] = 1; }
     for (int32_t e = 0; e < nE; ++e) {
       const T sg = event_signal(d, M, L->evKind[e], L->evIdx[e]);
       if (!fired[e] && sg > 0) {
@@ -564,19 +584,19 @@ int64_t dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, const d
   }
   T loss = T(0);
   for (int32_t j = 0; j < nT; ++j) {
-    if (!trimSet[j]) trim[j] = T(L->trimAfter);
-    const T r = trim[j] - L->trimObs[j];
-    loss += L->trimW[j] * (r * r);
+    if (!rdSet[j]) rd[j] = T(L->rdAfter[j]);
+    const T r = rd[j] - L->rdObs[j];
+    loss += L->rdW[j] * (r * r);
   }
   for (int32_t e = 0; e < nE; ++e) {
-    if (!fired[e]) tstar[e] = T(d.t);
+    if (!fired[e]) tstar[e] = T(d.foundered ? d.founderT : L->evMissing[e]);
     const T r = tstar[e] - L->evObs[e];
     loss += L->evW[e] * (r * r);
   }
   if (lossOut) drop(loss, nd, lossOut, lossOut + 1, 1, 0);
   if (sigOut) {
     const size_t K = static_cast<size_t>(nT + nE);
-    for (int32_t j = 0; j < nT; ++j) drop(trim[j], nd, sigOut, sigOut + K, K, j);
+    for (int32_t j = 0; j < nT; ++j) drop(rd[j], nd, sigOut, sigOut + K, K, j);
     for (int32_t e = 0; e < nE; ++e) drop(tstar[e], nd, sigOut, sigOut + K, K, nT + e);
   }
   if (recOut) {
@@ -592,8 +612,8 @@ extern "C" {
 
 // A run of one instance from the scenario's initial state (to tMax, founder or maxSteps) in the dual build, with
 // theta and nd <= 24 tangent directions of theta (seed[k * np + i]: direction k, parameter i). Outputs (each
-// optional): lossOut[1 + nd] the loss and its directional derivatives; sigOut[(1 + nd) * (nTrim + nEv)] the
-// trim values and event times, then their derivatives direction by direction; recOut[(1 + nd) * (10 + 5 NN)] the
+// optional): lossOut[1 + nd] the loss and its directional derivatives; sigOut[(1 + nd) * (nRead + nEv)] the
+// readout values and event times, then their derivatives direction by direction; recOut[(1 + nd) * (10 + 5 NN)] the
 // final state as a trace record, then its derivatives. Returns the step count, or -1 for nd > 24.
 int64_t orc_dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, const double* area, const double* tOn,
                      const orc_params* prm, const double* theta, int32_t nd, const double* seed, double tMax, int64_t maxSteps,

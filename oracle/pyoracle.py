@@ -46,8 +46,9 @@ class OrcParams(C.Structure):
 
 
 class OrcLoss(C.Structure):
-    _fields_ = [("nTrim", C.c_int32), ("trimStep", C.POINTER(C.c_int64)), ("trimObs", _dp), ("trimW", _dp),
-                ("trimAfter", C.c_double), ("nEv", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evObs", _dp), ("evW", _dp)]
+    _fields_ = [("nRead", C.c_int32), ("rdStep", C.POINTER(C.c_int64)), ("rdKind", _ip), ("rdObs", _dp), ("rdW", _dp),
+                ("rdAfter", _dp), ("nEv", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evObs", _dp), ("evW", _dp),
+                ("evMissing", _dp)]
 
 
 _lib = None
@@ -364,25 +365,26 @@ def _params(pa, pc, np_, relaxed=False):
 
 
 def _loss(spec):
-    """An OrcLoss over a loss specification with attributes trim_steps, trim_obs, trim_w, trim_after, ev_kind,
-    ev_idx, ev_obs, ev_w (the arrays are kept alive in the returned tuple)."""
+    """An OrcLoss over a loss specification (sinksim_amd.calib.LossSpec: read_steps, read_kind, read_obs, read_w,
+    read_after, ev_kind, ev_idx, ev_obs, ev_w, ev_missing); the arrays are kept alive in the returned tuple."""
     if spec is None:
         return None, ()
-    ts = np.ascontiguousarray(spec.trim_steps, dtype=np.int64)
-    to, tw = _f(spec.trim_obs), _f(spec.trim_w)
-    ek, ei, eo, ew = _i(spec.ev_kind), _i(spec.ev_idx), _f(spec.ev_obs), _f(spec.ev_w)
+    ts = np.ascontiguousarray(spec.read_steps, dtype=np.int64)
+    tk, to, tw, ta = _i(spec.read_kind), _f(spec.read_obs), _f(spec.read_w), _f(spec.read_after)
+    ek, ei, eo, ew, em = _i(spec.ev_kind), _i(spec.ev_idx), _f(spec.ev_obs), _f(spec.ev_w), _f(spec.ev_missing)
     L = OrcLoss()
-    L.nTrim, L.trimStep, L.trimObs, L.trimW = len(ts), ts.ctypes.data_as(C.POINTER(C.c_int64)), _p(to, C.c_double), _p(tw, C.c_double)
-    L.trimAfter = float(spec.trim_after)
-    L.nEv, L.evKind, L.evIdx, L.evObs, L.evW = len(ek), _p(ek, C.c_int32), _p(ei, C.c_int32), _p(eo, C.c_double), _p(ew, C.c_double)
-    return L, (ts, to, tw, ek, ei, eo, ew)
+    L.nRead, L.rdStep, L.rdKind = len(ts), ts.ctypes.data_as(C.POINTER(C.c_int64)), _p(tk, C.c_int32)
+    L.rdObs, L.rdW, L.rdAfter = _p(to, C.c_double), _p(tw, C.c_double), _p(ta, C.c_double)
+    L.nEv, L.evKind, L.evIdx = len(ek), _p(ek, C.c_int32), _p(ei, C.c_int32)
+    L.evObs, L.evW, L.evMissing = _p(eo, C.c_double), _p(ew, C.c_double), _p(em, C.c_double)
+    return L, (ts, tk, to, tw, ta, ek, ei, eo, ew, em)
 
 
 def dual_run(model: Model, en, area, tOn, pa, pc, theta, seed=None, t_max: float = 1e30, max_steps: int = 2**62,
              spec=None, relaxed: bool = False) -> dict:
     """One instance from the scenario's initial state in the dual-number build (dual_step.hpp), with parameters
     theta and tangent directions seed [nd, np] (default: the identity, the full gradient by forward mode, nd = np
-    <= 24). Returns steps, loss and dloss [nd], signals [nTrim + nEv] and dsignals [nd, ...], the final state
+    <= 24). Returns steps, loss and dloss [nd], signals [nRead + nEv] and dsignals [nd, ...], the final state
     record and drecord [nd, 10 + 5 NN]."""
     theta = _f(theta)
     np_ = theta.shape[0]
@@ -392,7 +394,7 @@ def dual_run(model: Model, en, area, tOn, pa, pc, theta, seed=None, t_max: float
     prm, keep = _params(pa, pc, np_, relaxed)
     L, keep2 = _loss(spec)
     en, area, tOn = _u(en), _f(area), _f(tOn)
-    nsig = (len(spec.trim_steps) + len(spec.ev_kind)) if spec is not None else 0
+    nsig = spec.signals if spec is not None else 0
     R = 10 + 5 * model.ship.nodes
     loss = np.zeros(1 + nd)
     sig = np.zeros((1 + nd) * max(nsig, 1))

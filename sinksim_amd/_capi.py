@@ -67,8 +67,8 @@ class SsEvent(C.Structure):
 
 
 class SsLoss(C.Structure):
-    _fields_ = [("nTrim", C.c_int32), ("trimStep", C.POINTER(C.c_int64)), ("trimW", _dp), ("trimAfter", C.c_double),
-                ("nEvent", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evW", _dp)]
+    _fields_ = [("nRead", C.c_int32), ("rdStep", C.POINTER(C.c_int64)), ("rdKind", _ip), ("rdW", _dp), ("rdAfter", _dp),
+                ("nEvent", C.c_int32), ("evKind", _ip), ("evIdx", _ip), ("evW", _dp), ("evMissing", _dp)]
 
 
 class SsGradientOptions(C.Structure):

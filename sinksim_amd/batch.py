@@ -261,14 +261,15 @@ class Batch:
         Returns loss [count], grad [count, np] (None without want_grad), signals [count, signals], steps [count],
         mismatches [count] (recomputed steps whose values differed from the forward's: 0) and the kernel time (ms)."""
         pa, pc, th, npar = self._param_tables(params, theta)
-        nT, nE = len(spec.trim_steps), len(spec.ev_kind)
-        ts = np.ascontiguousarray(spec.trim_steps, np.int64)
-        tw, ek, ei, ew = A.f64(spec.trim_w), A.i32(spec.ev_kind), A.i32(spec.ev_idx), A.f64(spec.ev_w)
-        L = A.SsLoss(nT, ts.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(tw, C.c_double), float(spec.trim_after), nE,
-                     A.ptr(ek, C.c_int32), A.ptr(ei, C.c_int32), A.ptr(ew, C.c_double))
+        nT, nE = len(spec.read_steps), len(spec.ev_kind)
+        ts = np.ascontiguousarray(spec.read_steps, np.int64)
+        tk, tw, ta = A.i32(spec.read_kind), A.f64(spec.read_w), A.f64(spec.read_after)
+        ek, ei, ew, em = A.i32(spec.ev_kind), A.i32(spec.ev_idx), A.f64(spec.ev_w), A.f64(spec.ev_missing)
+        L = A.SsLoss(nT, ts.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(tk, C.c_int32), A.ptr(tw, C.c_double), A.ptr(ta, C.c_double), nE,
+                     A.ptr(ek, C.c_int32), A.ptr(ei, C.c_int32), A.ptr(ew, C.c_double), A.ptr(em, C.c_double))
         nsig = nT + nE
         if observations is None:
-            obs = np.tile(np.concatenate([np.asarray(spec.trim_obs, np.float64), np.asarray(spec.ev_obs, np.float64)]), (self.count, 1))
+            obs = np.tile(spec.observations, (self.count, 1))
         else:
             obs = A.f64(observations).reshape(self.count, nsig)
         obs = A.f64(obs)

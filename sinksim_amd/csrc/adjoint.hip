@@ -3,10 +3,11 @@
 //
 // One workgroup per instance runs three sweeps over its own trajectory:
 //   1. forward: the plain step (device_step, the batch kernel's step) from the initial state to founder / tMax /
-//      maxSteps, storing the state every K steps (a checkpoint) and the loss's ingredients (trim after given steps,
-//      the signed event signals at and before each event's first firing step);
+//      maxSteps, storing the state every K steps (a checkpoint) and the loss's ingredients (trim or list after given
+//      steps, the signed event signals at and before each event's first firing step);
 //   2. loss: the loss, the signals and the seeds of the reverse sweep (the loss's derivative with respect to the
-//      pitch after a trim step, and to a monitor's flow sum or a mark's world z at an event step and the one before);
+//      pitch or roll after a readout step, and to a monitor's flow sum, a mark's world z or the founder rule's
+//      deciding margin -- hull top, pitch or buoyancy volume -- at an event step and the one before);
 //   3. reverse: for each segment of K steps, last first: the segment recomputed from its checkpoint with the plain
 //      step into a ring of K + 1 states, then its steps in reverse order, each recomputed with its local Jacobians
 //      (jac_step, whose values are compared with the ring's next state) and pulled back (reverse_step).
@@ -35,13 +36,13 @@ struct DevAdj {
   gcd* theta;                      // [count * np]
   gci *pa, *pc;                    // per instance connection (connOff)
   gcd* ca;                         // the scaled products coef * area per instance connection
-  int32_t nTrim, nEv;
-  const SSG long long* trimStep;
-  gcd* trimW;
-  double trimAfter;
+  int32_t nRead, nEv;
+  const SSG long long* rdStep;
+  gci* rdKind;
+  gcd *rdW, *rdAfter;
   gci *evKind, *evIdx;
-  gcd* evW;
-  gcd* obs;                        // [count * (nTrim + nEv)]
+  gcd *evW, *evMissing;
+  gcd* obs;                        // [count * (nRead + nEv)]
   SSG double *loss, *grad, *signals;
   SSG long long* steps;
   SSG int32_t* mismatch;           // per instance: recomputed steps whose values differ from the forward's
@@ -170,17 +171,20 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
   const Tape T = carve_tape(lds + P.L.total, AMs.A);
   const int NN = P.S.nn;
   const int SS = 7 + 5 * NN;
-  const int nT = D.nTrim, nE = D.nEv, nSig = nT + nE;
+  const int nT = D.nRead, nE = D.nEv, nSig = nT + nE;
   SSG double* ckpt = D.ckpt + D.ckptOff[inst];
   SSG double* ring = D.ring + D.ringOff[inst];
   const long long K = D.K;
-  // signals: trim value, trim set; per event: fired, step, s_k, s_{k-1}, previous, time at firing, z before the step
-  SSL double* trimV = T.sig;
-  SSL double* trimSet = trimV + nT;
-  SSL double* ev = trimSet + nT;  // 7 per event
+  // signals: readout value, readout set; per event (kEvRec): fired, step, s_k, s_{k-1}, previous s, time at firing,
+  // z before the step, previous margin code, code at firing, code before firing (founder margins: 0 hull top,
+  // +1 / -1 pitch by its sign, 2 buoyancy)
+  constexpr int kEvRec = 10;
+  SSL double* rdV = T.sig;
+  SSL double* rdSet = rdV + nT;
+  SSL double* ev = rdSet + nT;
 
   load_instance(P, Bt, D, B, inst, tid, kTPB);
-  for (int i = tid; i < 2 * nT + 7 * nE; i += kTPB) T.sig[i] = 0;
+  for (int i = tid; i < 2 * nT + kEvRec * nE; i += kTPB) T.sig[i] = 0;
   if (tid == 0) bad = 0;
   __syncthreads();
   if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
@@ -194,18 +198,37 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
     if (done % K == 0) save_state(P, B, ckpt + (done / K) * SS, tid, kTPB);
     if (tid == 0)
       for (int e = 0; e < nE; ++e)
-        if (D.evKind[e] == 1) { const int i = D.evIdx[e]; ev[7 * e + 6] = world_z(B.frame, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]); }
+        if (D.evKind[e] == 1) { const int i = D.evIdx[e]; ev[kEvRec * e + 6] = world_z(B.frame, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]); }
     __syncthreads();
     device_step<false, kTPB>(pa, lds, tid);
     ++done;
     if (tid == 0) {
       for (int j = 0; j < nT; ++j)
-        if (D.trimStep[j] == done) { trimV[j] = B.scal[S_PITCH] * 180 / kPi; trimSet[j] = 1; }
+        if (D.rdStep[j] == done) { rdV[j] = (D.rdKind[j] == 0 ? B.scal[S_PITCH] : B.scal[S_ROLL]) * 180 / kPi; rdSet[j] = 1; }
       for (int e = 0; e < nE; ++e) {
-        SSL double* r = ev + 7 * e;
-        const double s = D.evKind[e] == 0 ? B.over[D.evIdx[e]] - P.M.monThreshold : event_signal(P, 1, D.evIdx[e], r[6]);
-        if (r[0] == 0 && s > 0) { r[0] = 1; r[1] = static_cast<double>(done); r[2] = s; r[3] = r[4]; r[5] = B.scal[S_T]; }
+        SSL double* r = ev + kEvRec * e;
+        const int kind = D.evKind[e];
+        double s, code = 0;
+        if (kind == 0) {
+          s = B.over[D.evIdx[e]] - P.M.monThreshold;
+        } else if (kind == 1) {
+          s = P.M.mkWhen[D.evIdx[e]] == 0 ? -r[6] : r[6];
+        } else {
+          // the founder rule's largest margin (events.hpp:31-37)
+          const double pitch = B.scal[S_PITCH];
+          s = P.M.hullTopBelow - B.scal[O_TOP];
+          const double sp = __builtin_fabs(pitch) - P.M.pitchAbsAbove;
+          if (sp > s) { s = sp; code = pitch < 0 ? -1 : 1; }
+          if (B.scal[S_T] > P.M.minT) {
+            const double sb = P.M.buoyancyBelow - B.scal[O_VB];
+            if (sb > s) { s = sb; code = 2; }
+          }
+        }
+        if (r[0] == 0 && s > 0) {
+          r[0] = 1; r[1] = static_cast<double>(done); r[2] = s; r[3] = r[4]; r[5] = B.scal[S_T]; r[8] = code; r[9] = r[7];
+        }
         r[4] = s;
+        r[7] = code;
       }
     }
     __syncthreads();
@@ -214,40 +237,48 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
   // 2. loss, signals, seeds
   if (tid == 0) {
     const double tEnd = B.scal[S_T], dt = P.M.dt;
+    const bool foundered = B.scal[S_FOUNDERED] != 0;
     SSG double* sigOut = D.signals ? D.signals + static_cast<size_t>(inst) * nSig : nullptr;
     gcd* obs = D.obs + static_cast<size_t>(inst) * nSig;
     double L = 0;
     int ns = 0;
+    auto seed = [&](long long step, int kind, int idx, double value) {
+      SSL double* sd = T.seeds + 4 * ns++;
+      sd[0] = static_cast<double>(step); sd[1] = kind; sd[2] = idx; sd[3] = value;
+    };
     for (int j = 0; j < nT; ++j) {
-      const double v = trimSet[j] != 0 ? trimV[j] : D.trimAfter;
+      const double v = rdSet[j] != 0 ? rdV[j] : D.rdAfter[j];
       const double r = v - obs[j];
-      L += D.trimW[j] * (r * r);
+      L += D.rdW[j] * (r * r);
       if (sigOut) sigOut[j] = v;
-      if (trimSet[j] != 0) {
-        SSL double* sd = T.seeds + 4 * ns++;
-        sd[0] = static_cast<double>(D.trimStep[j]); sd[1] = 0; sd[2] = 0; sd[3] = 2 * D.trimW[j] * r * 180 / kPi;
-      }
+      if (rdSet[j] != 0) seed(D.rdStep[j], D.rdKind[j] == 0 ? 0 : 3, 0, 2 * D.rdW[j] * r * 180 / kPi);
     }
+    // a founder margin's seed: hull top (s = hullTopBelow - top), pitch (s = |pitch| - pitchAbsAbove), buoyancy
+    auto margin_seed = [&](long long step, double code, double g) {
+      if (code == 0) seed(step, 4, 0, -g);
+      else if (code == 2) seed(step, 5, 0, -g);
+      else seed(step, 0, 0, g * code);
+    };
     for (int e = 0; e < nE; ++e) {
-      const SSL double* r = ev + 7 * e;
+      const SSL double* r = ev + kEvRec * e;
       const bool fired = r[0] != 0;
       const long long k = static_cast<long long>(r[1]);
       double ts;
       if (fired && k > 1) ts = r[5] - dt * r[2] / (r[2] - r[3]);
       else if (fired) ts = r[5];
-      else ts = tEnd;
+      else ts = foundered ? tEnd : D.evMissing[e];
       const double res = ts - obs[nT + e];
       L += D.evW[e] * (res * res);
       if (sigOut) sigOut[nT + e] = ts;
       if (fired && k > 1) {
-        const double Gt = 2 * D.evW[e] * res, s = r[2], sp = r[3], del = s - sp;
-        const double gs = Gt * dt * sp / (del * del), gsp = -Gt * dt * s / (del * del);
-        const int kind = D.evKind[e] == 0 ? 1 : 2, idx = D.evIdx[e];
-        const double sign = D.evKind[e] == 0 ? 1.0 : (P.M.mkWhen[idx] == 0 ? -1.0 : 1.0);
-        SSL double* a = T.seeds + 4 * ns++;
-        a[0] = static_cast<double>(k); a[1] = kind; a[2] = idx; a[3] = sign * gs;
-        SSL double* b = T.seeds + 4 * ns++;
-        b[0] = static_cast<double>(k - 1); b[1] = kind; b[2] = idx; b[3] = sign * gsp;
+        const double Gt = 2 * D.evW[e] * res, sk = r[2], sp = r[3], del = sk - sp;
+        const double gs = Gt * dt * sp / (del * del), gsp = -Gt * dt * sk / (del * del);
+        const int kind = D.evKind[e], idx = D.evIdx[e];
+        if (kind == 0) { seed(k, 1, idx, gs); seed(k - 1, 1, idx, gsp); }
+        else if (kind == 1) {
+          const double sign = P.M.mkWhen[idx] == 0 ? -1.0 : 1.0;
+          seed(k, 2, idx, sign * gs); seed(k - 1, 2, idx, sign * gsp);
+        } else { margin_seed(k, r[8], gs); margin_seed(k - 1, r[9], gsp); }
       }
     }
     T.nseeds[0] = ns;
@@ -338,6 +369,8 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
       if (D.stepMon[m] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 1; s[2] = m; s[3] = D.stepMon[m]; }
     for (int m = 0; m < P.M.nmark; ++m)
       if (D.stepMark[m] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 2; s[2] = m; s[3] = D.stepMark[m]; }
+    if (D.stepLam[8] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 5; s[2] = 0; s[3] = D.stepLam[8]; }
+    if (D.stepLam[9] != 0) { SSL double* s = T.seeds + 4 * ns++; s[0] = 1; s[1] = 4; s[2] = 0; s[3] = D.stepLam[9]; }
     T.nseeds[0] = ns;
   }
   __syncthreads();
@@ -372,7 +405,7 @@ AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t maxGroup, int32
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
   L.sv = take(8); L.iVol = take(NN); L.iLvl = take(NN); L.iCx = take(NN); L.iCy = take(NN); L.iCz = take(NN);
-  L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(16 * kUnits); L.bt = take(16);
+  L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(kBw * kUnits); L.bt = take(20);
   L.oLvl = take(NN); L.oVol = take(NN); L.oArea = take(NN); L.oCx = take(NN); L.oCy = take(NN); L.oCz = take(NN); L.oZmax = take(NN);
   L.jac = take(NN * kSolveOut * kDof); L.carry = take(NN); L.gMer = take(NG); L.gH0 = take(NG);
   L.mp = take(kUnits * maxGroup * kPassRec);
@@ -394,9 +427,10 @@ namespace {
 struct AdjRun {
   Dev<AdjModel> dAdjModels;
   Dev<DevAdj> dAdj;
-  Dev<double> theta, ca, trimW, evW, obs, loss, grad, signals, ckpt, ring, stepRec, stepLam, stepMon, stepMark, stepLamOut, stepRecOut;
-  Dev<int32_t> pa, pc, evKind, evIdx, mismatch;
-  Dev<long long> trimStep, steps, ckptOff, ringOff;
+  Dev<double> theta, ca, rdW, rdAfter, evW, evMissing, obs, loss, grad, signals, ckpt, ring, stepRec, stepLam, stepMon, stepMark, stepLamOut,
+      stepRecOut;
+  Dev<int32_t> pa, pc, rdKind, evKind, evIdx, mismatch;
+  Dev<long long> rdStep, steps, ckptOff, ringOff;
   size_t lds = 0;
 };
 
@@ -415,13 +449,15 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
   need(b->tpb == kThreads, "ss_batch_gradient: the gradient runs on 256-thread workgroups (set_launch 256 or 0)");
   need(!b->mixed, "ss_batch_gradient: the gradient is FP64 (portable numerics)");
-  const int32_t nT = loss ? loss->nTrim : 0, nE = loss ? loss->nEvent : 0;
+  const int32_t nT = loss ? loss->nRead : 0, nE = loss ? loss->nEvent : 0;
   need(nT >= 0 && nE >= 0 && nT + nE <= 256, "ss_batch_gradient: at most 256 loss terms");
   const size_t nconn = b->hArea.size();
   for (size_t k = 0; k < nconn; ++k) need(s.pa[k] >= -1 && s.pa[k] < s.np && s.pc[k] >= -1 && s.pc[k] < s.np, "ss_batch_gradient: parameter index out of range");
   for (size_t k = 0; k < count * static_cast<size_t>(s.np); ++k) need(s.theta[k] > 0 && std::isfinite(s.theta[k]), "ss_batch_gradient: parameters must be positive");
+  for (int32_t j = 0; j < nT; ++j) need(loss->rdKind[j] == 0 || loss->rdKind[j] == 1, "ss_batch_gradient: readout kind must be 0 (trim) or 1 (list)");
   for (int32_t e = 0; e < nE; ++e) {
-    need(loss->evKind[e] == 0 || loss->evKind[e] == 1, "ss_batch_gradient: event kind must be 0 (monitor) or 1 (mark)");
+    need(loss->evKind[e] >= 0 && loss->evKind[e] <= 2, "ss_batch_gradient: event kind must be 0 (monitor), 1 (mark) or 2 (founder)");
+    if (loss->evKind[e] == 2) continue;
     for (size_t i = 0; i < count; ++i) {
       const Model& m = b->model(static_cast<int32_t>(i));
       need(loss->evIdx[e] >= 0 && loss->evIdx[e] < (loss->evKind[e] == 0 ? m.nmon : m.nmark), "ss_batch_gradient: event index out of range");
@@ -447,14 +483,14 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   R.pa.upload(s.pa, nconn);
   R.pc.upload(s.pc, nconn);
   // per model: the tape layout
-  const int32_t nseeds = nT + 2 * nE + 64 + 64;  // the one-step form seeds every monitor and mark
+  const int32_t nseeds = nT + 2 * nE + 64 + 64 + 2;  // the one-step form seeds every monitor and mark, V and the top
   std::vector<AdjModel> am;
   size_t lds = 0;
   for (const auto& m : b->models) {
     need(m->maxGroup <= 8, "ss_batch_gradient: free-surface groups of more than 8 spaces are not supported");
     need(m->nmon <= 64 && m->nmark <= 64, "ss_batch_gradient: at most 64 monitors and 64 marks");
     AdjModel a{};
-    a.A = adj_layout(m->NN, m->NC, m->NG, m->maxGroup, s.np, nseeds, 2 * nT + 7 * nE);
+    a.A = adj_layout(m->NN, m->NC, m->NG, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
     a.maxGroup = m->maxGroup;
     a.np = s.np;
     a.mode = s.relaxed ? 1 : 0;
@@ -487,10 +523,15 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   R.ring.alloc(static_cast<size_t>(std::max(rg, 1LL)));
   R.ckptOff.upload(ckOff);
   R.ringOff.upload(rgOff);
-  if (nT > 0) { R.trimStep.upload(reinterpret_cast<const long long*>(loss->trimStep), static_cast<size_t>(nT)); R.trimW.upload(loss->trimW, static_cast<size_t>(nT)); }
+  if (nT > 0) {
+    R.rdStep.upload(reinterpret_cast<const long long*>(loss->rdStep), static_cast<size_t>(nT));
+    R.rdKind.upload(loss->rdKind, static_cast<size_t>(nT));
+    R.rdW.upload(loss->rdW, static_cast<size_t>(nT));
+    R.rdAfter.upload(loss->rdAfter, static_cast<size_t>(nT));
+  }
   if (nE > 0) {
     R.evKind.upload(loss->evKind, static_cast<size_t>(nE)); R.evIdx.upload(loss->evIdx, static_cast<size_t>(nE));
-    R.evW.upload(loss->evW, static_cast<size_t>(nE));
+    R.evW.upload(loss->evW, static_cast<size_t>(nE)); R.evMissing.upload(loss->evMissing, static_cast<size_t>(nE));
   }
   const size_t nSig = static_cast<size_t>(nT + nE);
   if (nSig > 0) {
@@ -505,9 +546,9 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   DevAdj d{};
   d.K = K; d.wantGrad = 1; d.maxSteps = maxSteps > 0 ? maxSteps : LLONG_MAX; d.tMax = tMax;
   d.theta = G(R.theta.p); d.pa = G(R.pa.p); d.pc = G(R.pc.p); d.ca = G(R.ca.p);
-  d.nTrim = nT; d.nEv = nE;
-  d.trimStep = (const SSG long long*)R.trimStep.p; d.trimW = G(R.trimW.p); d.trimAfter = loss ? loss->trimAfter : 0;
-  d.evKind = G(R.evKind.p); d.evIdx = G(R.evIdx.p); d.evW = G(R.evW.p); d.obs = G(R.obs.p);
+  d.nRead = nT; d.nEv = nE;
+  d.rdStep = (const SSG long long*)R.rdStep.p; d.rdKind = G(R.rdKind.p); d.rdW = G(R.rdW.p); d.rdAfter = G(R.rdAfter.p);
+  d.evKind = G(R.evKind.p); d.evIdx = G(R.evIdx.p); d.evW = G(R.evW.p); d.evMissing = G(R.evMissing.p); d.obs = G(R.obs.p);
   d.loss = W(R.loss.p); d.grad = W(R.grad.p); d.signals = W(R.signals.p); d.steps = W(R.steps.p); d.mismatch = W(R.mismatch.p);
   d.ckpt = W(R.ckpt.p); d.ring = W(R.ring.p);
   d.ckptOff = (const SSG long long*)R.ckptOff.p; d.ringOff = (const SSG long long*)R.ringOff.p;
@@ -561,7 +602,7 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
     const size_t count = static_cast<size_t>(b->opt.count);
     R.loss.download(lossOut, count);
     if (grad) R.grad.download(grad, count * static_cast<size_t>(nparams));
-    const size_t nSig = static_cast<size_t>(loss ? loss->nTrim + loss->nEvent : 0);
+    const size_t nSig = static_cast<size_t>(loss ? loss->nRead + loss->nEvent : 0);
     if (signals && nSig) R.signals.download(signals, count * nSig);
     if (steps) {
       std::vector<long long> s(count);

@@ -21,6 +21,8 @@
 // with the slope floored at the space's minimum effective area amin), as if the solve had converged.
 #pragma once
 
+#include <type_traits>
+
 #include "step_dev.hpp"
 
 namespace ssa {
@@ -29,6 +31,7 @@ constexpr int kDof = 5;     // local degrees of freedom of a level solve: V, h0,
 constexpr int kSolveOut = 7;  // per space: level, volume (after group sharing), free-surface area, cx, cy, cz, extent top
 constexpr int kPassRec = 23;  // a node pass's reduced record: 7 values, 10 moment sums, 6 extent coordinates
 constexpr int kFrameDeriv = 13;  // frame slots F_R00..F_CTH with derivatives along the pose
+constexpr int kBw = 18;          // a 32-lane group's buoyancy record: V Mx My Mz top, 10 moment sums, top column x y top
 
 // LDS layout of the adjoint's step tape, offsets in doubles from the end of the forward layout (the host computes
 // it: adj_layout in adjoint.hip).
@@ -103,7 +106,9 @@ enum : int { T_V = 0, T_MX, T_MY, T_MZ, T_IR, T_BX, T_BY, T_BZ, T_FB, T_BXW, T_B
              T_MH, T_ITH, T_IPH, T_AZ, T_ATH, T_APH, T_CLP, T_CLR, T_VZ1, T_WTH1, T_WPH1, T_ZO1, T_P1, T_R1, T_GFZ, T_GTX, T_GTY,
              T_GMW, T_GIWP, T_GIWR, T_SEA, T_COUNT = 48 };
 // moment sums of `bt` (buoyancy) after the 4 values: T00 T01 T02 T03 T11 T12 T13 T22 T23 T33
-// seeds: records of 4 doubles (step, kind 0 trim / 1 monitor / 2 mark, index, value)
+// moment sums are followed by the hull top's column (x, y, top) at bt[14..16]
+// seeds: records of 4 doubles (step, kind, index, value); kinds: 0 pitch after the step, 1 a monitor's flow sum,
+// 2 a mark's world z, 3 roll after the step, 4 the step's hull top, 5 the step's buoyancy volume
 
 // ------------------------------------------------------------------------------------------- recompute
 
@@ -490,20 +495,22 @@ __device__ inline void solve_levels_tv(PC& P, const Blk& B, const Tape& T, SSL d
   }
 }
 
-// buoyancy partials (hydro.hpp:242-258 values) with the moment sums of the partly wet columns; each 32-lane
-// group's tree result goes to bw (5 values, 10 sums)
+// buoyancy partials (hydro.hpp:242-258 values) with the moment sums of the partly wet columns and the column that
+// gives the hull top (carried through the canonical tree: a later lane replaces only a strictly higher top); each
+// 32-lane group's result goes to bw (5 values, 10 sums, the top column's x, y and top)
 __device__ inline void buoyancy_rec(PC& P, const Tape& T, int tid) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
   const SSL double* fr = T.fr;
   const Fr64 F = fr64(fr);
   HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
   double S[10] = {};
+  double tx = 0, ty = 0, th = 0;
   for (int i = tid; i < P.S.ncol; i += kThreads) {
     const Geo<double> g = geo(P.S.col, i);
     const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
     const double base = F.R20 * x + F.R21 * y + F.tz;
     const double wt = base + F.R22 * hi;
-    if (wt > p.top) p.top = wt;
+    if (wt > p.top) { p.top = wt; tx = x; ty = y; th = hi; }
     const double zp = -base * F.iR;
     if (zp <= lo) continue;
     double s;
@@ -518,18 +525,25 @@ __device__ inline void buoyancy_rec(PC& P, const Tape& T, int tid) {
     const double v = g.A * s;
     p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
   }
-  tree32(p);
-  auto sum32 = [&](double v) {
-    v += down<16>(v); v += down<8>(v); v += down<4>(v); v += down<2>(v); v += down<1>(v);
-    return v;
-  };
-  double Sr[10];
+  auto level = [&](auto off) {
+    constexpr int OFF = decltype(off)::value;
+    const double V = down<OFF>(p.V), Mx = down<OFF>(p.Mx), My = down<OFF>(p.My), Mz = down<OFF>(p.Mz), top = down<OFF>(p.top);
+    const double ox = down<OFF>(tx), oy = down<OFF>(ty), oh = down<OFF>(th);
+    double o[10];
 #pragma unroll
-  for (int k = 0; k < 10; ++k) Sr[k] = sum32(S[k]);
+    for (int k = 0; k < 10; ++k) o[k] = down<OFF>(S[k]);
+    p.V += V; p.Mx += Mx; p.My += My; p.Mz += Mz;
+    if (top > p.top) { p.top = top; tx = ox; ty = oy; th = oh; }
+#pragma unroll
+    for (int k = 0; k < 10; ++k) S[k] += o[k];
+  };
+  level(std::integral_constant<int, 16>{}); level(std::integral_constant<int, 8>{}); level(std::integral_constant<int, 4>{});
+  level(std::integral_constant<int, 2>{}); level(std::integral_constant<int, 1>{});
   if (lane == 0) {
-    SSL double* w = T.bw + unit * 16;
+    SSL double* w = T.bw + unit * kBw;
     w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top;
-    for (int k = 0; k < 10; ++k) w[5 + k] = Sr[k];
+    for (int k = 0; k < 10; ++k) w[5 + k] = S[k];
+    w[15] = tx; w[16] = ty; w[17] = th;
   }
 }
 
@@ -703,15 +717,18 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T,
   // (a) buoyancy totals, (d) loads and integration (body.hpp:11-57), one thread, in node order
   if (tid == 0) {
     const SSL double* fr = T.fr;
-    double tV = T.bw[0], tMx = T.bw[1], tMy = T.bw[2], tMz = T.bw[3];
+    double tV = T.bw[0], tMx = T.bw[1], tMy = T.bw[2], tMz = T.bw[3], tTop = T.bw[4];
+    double tc3[3] = {T.bw[15], T.bw[16], T.bw[17]};
     double S[10];
     for (int k = 0; k < 10; ++k) S[k] = T.bw[5 + k];
     for (int w = 1; w < kUnits; ++w) {
-      const SSL double* q = T.bw + 16 * w;
+      const SSL double* q = T.bw + kBw * w;
       tV += q[0]; tMx += q[1]; tMy += q[2]; tMz += q[3];
+      if (q[4] > tTop) { tTop = q[4]; tc3[0] = q[15]; tc3[1] = q[16]; tc3[2] = q[17]; }
       for (int k = 0; k < 10; ++k) S[k] += q[5 + k];
     }
     for (int k = 0; k < 10; ++k) T.bt[4 + k] = S[k];
+    T.bt[14] = tc3[0]; T.bt[15] = tc3[1]; T.bt[16] = tc3[2];
     T.bt[0] = tV; T.bt[1] = tMx; T.bt[2] = tMy; T.bt[3] = tMz;
     const double V = tV;
     const double bx = tV > 0 ? tMx / tV : 0, by = tV > 0 ? tMy / tV : 0, bz = tV > 0 ? tMz / tV : 0;
@@ -799,8 +816,8 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
   // R0: zero the array part of lamNext; thread 0: integration, loads totals and buoyancy in reverse, mark seeds
   for (int i = 6 + tid; i < 6 + 5 * NN; i += TPB) lamNext[i] = 0;
   if (tid == 0) {
-    const double lz = lamCur[0], lr = lamCur[2], lvz = lamCur[3], lwp = lamCur[5];
-    const double lp = lamCur[1] + seed_of(T, k, 0, 0), lwt = lamCur[4];
+    const double lz = lamCur[0], lvz = lamCur[3], lwp = lamCur[5];
+    const double lp = lamCur[1] + seed_of(T, k, 0, 0), lr = lamCur[2] + seed_of(T, k, 3, 0), lwt = lamCur[4];
     const bool cp_ = o[T_CLP] != 0, cr = o[T_CLR] != 0;
     const double gpRaw = cp_ ? 0.0 : lp, gwtRaw = (cp_ ? 0.0 : lwt) + gpRaw * dt;
     const double grRaw = cr ? 0.0 : lr, gwpRaw = (cr ? 0.0 : lwp) + grRaw * dt;
@@ -826,7 +843,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     FG.g[F_R00] += gbxw * bx; FG.g[F_R01] += gbxw * by; FG.g[F_R02] += gbxw * bz; FG.g[F_TX] += gbxw;
     FG.g[F_R10] += gbyw * bx; FG.g[F_R11] += gbyw * by; FG.g[F_R12] += gbyw * bz; FG.g[F_TY] += gbyw;
     const double gbx = gbxw * fr[F_R00] + gbyw * fr[F_R10], gby = gbxw * fr[F_R01] + gbyw * fr[F_R11], gbz = gbxw * fr[F_R02] + gbyw * fr[F_R12];
-    double gV = W * gFb, gMx = 0, gMy = 0, gMz = 0;
+    double gV = W * gFb + seed_of(T, k, 5, 0), gMx = 0, gMy = 0, gMz = 0;
     const double V = o[T_V];
     if (V > 0) {
       gMx = gbx / V; gMy = gby / V; gMz = gbz / V;
@@ -840,6 +857,11 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     FG.g[F_R20] += -iR * (gV * T01 + gMx * T11 + gMy * T12 + gMz * T13);
     FG.g[F_R21] += -iR * (gV * T02 + gMx * T12 + gMy * T22 + gMz * T23);
     FG.g[F_R22] += -iR * (gV * T03 + gMx * T13 + gMy * T23 + gMz * T33);
+    // hull-top seeds: the highest column top, base + R22 * top of its column
+    {
+      const double gt = seed_of(T, k, 4, 0);
+      if (gt != 0) add_world_z(FG, gt, T.bt[14], T.bt[15], T.bt[16]);
+    }
     // mark seeds: the step's world z of each mark
     for (int i = 0; i < P.M.nmark; ++i) {
       const double gz = seed_of(T, k, 2, i);

@@ -79,6 +79,8 @@ def test_one_step_values_and_dot_product(hulls, which, steps, relaxed):
     for k, (rec, nxt) in zip(steps, _oracle_states(ship, sim, steps, theta, pm)):
         lam = np.zeros_like(rec)
         lam[idx] = rng.standard_normal(idx.size)
+        lam[8] = rng.standard_normal() * 1e-3  # seeds on the step's buoyancy volume and hull top (founder margins)
+        lam[9] = rng.standard_normal()
         v = np.zeros_like(rec)
         v[idx] = rng.standard_normal(idx.size) * np.maximum(np.abs(rec[idx]), 1e-3) * 1e-3
         dth = rng.standard_normal(pm.count) * theta
@@ -90,7 +92,8 @@ def test_one_step_values_and_dot_product(hulls, which, steps, relaxed):
         bad = keep[ro[keep] != nxt[keep]]
         assert bad.size == 0, f"{where}: recomputed values differ from the oracle at record entries {bad[:8]}"
         r2, dro, sg, dsg = pyoracle.dual_step(model, en, area, tOn, pm.pa, pm.pc, theta, dth, rec, v, relaxed=relaxed)
-        terms_l = np.r_[lam[idx] * dro[idx], ms * dsg[:sim.monitors], mk * dsg[sim.monitors:sim.monitors + sim.marks]]
+        terms_l = np.r_[lam[idx] * dro[idx], lam[8:10] * dro[8:10], ms * dsg[:sim.monitors],
+                        mk * dsg[sim.monitors:sim.monitors + sim.marks]]
         terms_r = np.r_[lo[idx] * v[idx], gl * dth / theta]
         lhs, rhs = terms_l.sum(), terms_r.sum()
         scale = max(np.abs(terms_l).sum(), np.abs(terms_r).sum(), 1e-300)
@@ -98,15 +101,18 @@ def test_one_step_values_and_dot_product(hulls, which, steps, relaxed):
     b.close()
 
 
-def _loss_spec(trim_steps, events):
+def _loss_spec(read_steps, events, kinds=None):
+    """readouts (trim, or list where kinds says 1) after the given steps, and events (kind, idx)."""
     from sinksim_amd.calib import LossSpec
-    return LossSpec(np.asarray(trim_steps, np.int64), np.zeros(len(trim_steps)), np.full(len(trim_steps), 1 / 0.3 ** 2), 40.0,
-                    np.array([k for k, _ in events], np.int32), np.array([i for _, i in events], np.int32), np.zeros(len(events)),
-                    np.full(len(events), 1 / 300.0 ** 2))
+    n, m = len(read_steps), len(events)
+    kinds = np.zeros(n, np.int32) if kinds is None else np.asarray(kinds, np.int32)
+    return LossSpec(np.asarray(read_steps, np.int64), kinds, np.zeros(n), np.full(n, 1 / 0.3 ** 2), np.where(kinds == 0, 40.0, 0.0),
+                    np.array([k for k, _ in events], np.int32), np.array([i for _, i in events], np.int32), np.zeros(m),
+                    np.full(m, 1 / 300.0 ** 2), np.full(m, 18000.0))
 
 
 def _with_obs(spec, base, rng):
-    return spec.with_observations(base + rng.normal(0.0, 1.0, base.shape) * np.r_[np.full(len(spec.trim_steps), 0.2),
+    return spec.with_observations(base + rng.normal(0.0, 1.0, base.shape) * np.r_[np.full(len(spec.read_steps), 0.2),
                                                                                     np.full(len(spec.ev_kind), 120.0)])
 
 
@@ -119,7 +125,7 @@ def test_titanic_gradient_equals_dual(hulls, relaxed):
     rng = np.random.default_rng(41 + relaxed)
     n = 3
     thetas = np.exp(rng.uniform(-0.25, 0.25, (n, pm.count)))
-    spec0 = _loss_spec([3600, 7200, 10800], [(1, 4), (0, 2)])
+    spec0 = _loss_spec([3600, 7200, 10800, 9000], [(1, 4), (0, 2)], kinds=[0, 0, 0, 1])
     model = pyoracle.Model(ship, sim)
     en, area, tOn = sim.cEn.astype(np.uint8), sim.cArea.copy(), sim.cTOn.copy()
     base = pyoracle.dual_run(model, en, area, tOn, pm.pa, pm.pc, thetas[0], seed=np.zeros((1, pm.count)), max_steps=12000, spec=spec0)
@@ -234,5 +240,32 @@ def test_full_run_relaxed_twin_gradient_equals_dual(hulls):
         # over ~40-57k steps the linearised flooding has bursts of transient growth (spaces near full, weirs near
         # equal heads) that amplify the rounding differences of the two builds' summation orders: 1e-6, where every
         # horizon of 12k steps or less agrees to 1e-9
+        assert abs(gv - d["dloss"][0]) <= 1e-6 * max(abs(d["dloss"][0]), 1e-300), (i, gv, d["dloss"][0])
+    b.close()
+
+
+def test_reference_objective_gradient_equals_dual(hulls):
+    """The reference's calibration objective (trim and list readouts, the founder time and three event times against
+    the 1912 observations) over whole runs of four instances at random scales of breach, over, down and top: the loss
+    equals the dual build's, and the relaxed gradient its directional derivatives."""
+    import pyoracle
+    from sinksim_amd.calib import kind_params, reference_objective
+    ship, sim = hulls["titanic"]
+    pm = kind_params(sim, coef_kinds=())
+    spec = reference_objective(sim)
+    thetas = np.exp(np.random.default_rng(12).uniform(-0.3, 0.3, (4, pm.count)))
+    thetas[0] = 1.0
+    b = _batch([(ship, sim)], np.zeros(4, np.int32), t_max=260 * 60.0)
+    g = b.gradient(pm, thetas, spec, relaxed=True)
+    assert (g["mismatches"] == 0).all()
+    model = pyoracle.Model(ship, sim)
+    en, area, tOn = sim.cEn.astype(np.uint8), sim.cArea.copy(), sim.cTOn.copy()
+    rng = np.random.default_rng(13)
+    for i in range(4):
+        v = rng.standard_normal(pm.count) * thetas[i]
+        d = pyoracle.dual_run(model, en, area, tOn, pm.pa, pm.pc, thetas[i], seed=v[None], t_max=260 * 60.0, spec=spec, relaxed=True)
+        assert g["steps"][i] == d["steps"] and g["loss"][i] == d["loss"], (i, g["loss"][i], d["loss"])
+        assert np.array_equal(g["signals"][i], d["signals"])
+        gv = float(g["grad"][i] @ v)
         assert abs(gv - d["dloss"][0]) <= 1e-6 * max(abs(d["dloss"][0]), 1e-300), (i, gv, d["dloss"][0])
     b.close()

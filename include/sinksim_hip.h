@@ -252,7 +252,8 @@ typedef struct ss_loss {
 } ss_loss;
 
 /* checkpointEvery: K, the steps between stored states (the reverse sweep recomputes each segment of K steps from its
- * checkpoint); relaxed: 0 for the derivative of the discrete program, 1 for relaxed level derivatives (each level's
+ * checkpoint; K = 1 stores every state and recomputes nothing; 0 chooses 1 when every instance's states fit in a
+ * third of the free device memory, else 256); relaxed: 0 for the derivative of the discrete program, 1 for relaxed level derivatives (each level's
  * tangent taken at the exact root of its volume, as if the level solve had converged); tMax (0: the batch's) and
  * maxSteps (<= 0: none) end the run. */
 typedef struct ss_gradient_options {

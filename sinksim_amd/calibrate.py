@@ -176,7 +176,7 @@ def lbfgs_core(evaluate, x0, iterations: int = 12, memory: int = 6, max_step: fl
 
 
 def lbfgs(ship, sim, x0, iterations: int = 12, memory: int = 6, kinds=CALIBRATION_KINDS, max_step: float = 0.3, lo: float = -1.6,
-          hi: float = 1.6, target: float | None = None, checkpoint_every: int = 256, obs: dict | None = None, log=None) -> Trace:
+          hi: float = 1.6, target: float | None = None, checkpoint_every: int = 0, obs: dict | None = None, log=None) -> Trace:
     """lbfgs_core over x = log scale factors of `kinds` on the reference objective, every evaluation one call of the
     adjoint (relaxed gradients) through its torch.autograd.Function."""
     import torch

@@ -151,8 +151,9 @@ __device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double 
   return P.M.mkWhen[idx] == 0 ? -zpre : zpre;
 }
 
-constexpr int kTPB = kThreads;
+constexpr int kMaxTPB = 512;
 
+template <int kTPB>
 __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                      const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -234,6 +235,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
     __syncthreads();
   }
 
+  if (K == 1) save_state(P, B, ckpt + done * SS, tid, kTPB);  // the state after the last step
   // 2. loss, signals, seeds
   if (tid == 0) {
     const double tEnd = B.scal[S_T], dt = P.M.dt;
@@ -281,6 +283,10 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
         } else { margin_seed(k, r[8], gs); margin_seed(k - 1, r[9], gsp); }
       }
     }
+    // latest step first (insertion sort: a few dozen records), the order the reverse sweep consumes them in
+    for (int a = 1; a < ns; ++a)
+      for (int b = a; b > 0 && T.seeds[4 * (b - 1)] < T.seeds[4 * b]; --b)
+        for (int q = 0; q < 4; ++q) { const double t = T.seeds[4 * b + q]; T.seeds[4 * b + q] = T.seeds[4 * (b - 1) + q]; T.seeds[4 * (b - 1) + q] = t; }
     T.nseeds[0] = ns;
     D.loss[inst] = L;
     D.steps[inst] = done;
@@ -291,11 +297,24 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
   // 3. reverse
   for (int i = tid; i < 6 + 5 * NN; i += kTPB) { T.lamA[i] = 0; T.lamB[i] = 0; }
   for (int p = tid; p < AM.np; p += kTPB) T.G[p] = 0;
-  __syncthreads();
+  const size_t iC = static_cast<size_t>(Bt.connOff[inst]);
+  load_adjoint_tables<kTPB>(P, T, D.pa + iC, D.pc + iC, tid);
   SSL double* lamCur = T.lamA;
   SSL double* lamNext = T.lamB;
-  const size_t iC = static_cast<size_t>(Bt.connOff[inst]);
-  const long long nSeg = (done + K - 1) / K;
+  if (K == 1) {
+    // every state is stored: no recomputation
+    for (long long k = done; k >= 1; --k) {
+      load_tape_state(P, T, ckpt + (k - 1) * SS, tid, kTPB);
+      __syncthreads();
+      jac_step<kTPB>(P, AM, B, T, tid);
+      verify_step(P, T, ckpt + k * SS, tid, kTPB, (SSL int32_t*)&bad);
+      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext);
+      SSL double* sw = lamCur;
+      lamCur = lamNext;
+      lamNext = sw;
+    }
+  }
+  const long long nSeg = K == 1 ? 0 : (done + K - 1) / K;
   for (long long seg = nSeg - 1; seg >= 0; --seg) {
     restore_state(P, B, ckpt + seg * SS, tid, kTPB);
     if (tid == 0) *B.nextGroup = 0;
@@ -317,7 +336,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
       __syncthreads();
       jac_step<kTPB>(P, AM, B, T, tid);
       verify_step(P, T, ring + (i + 1) * SS, tid, kTPB, (SSL int32_t*)&bad);
-      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext, D.pa + iC, D.pc + iC);
+      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext);
       SSL double* sw = lamCur;
       lamCur = lamNext;
       lamNext = sw;
@@ -334,6 +353,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
 // One step of instance 0 from a given state record, with a given adjoint of the state after it and given seeds on
 // the step's monitor sums and mark world z: the adjoint of the state before it, the parameters' log-derivatives and
 // the recomputed state after the step (the test hook of the step's reverse).
+template <int kTPB>
 __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                           const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -374,8 +394,9 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
     T.nseeds[0] = ns;
   }
   __syncthreads();
+  load_adjoint_tables<kTPB>(P, T, D.pa, D.pc, tid);
   jac_step<kTPB>(P, AM, B, T, tid);
-  reverse_step<kTPB>(P, AM, B, T, tid, 1, T.lamA, T.lamB, D.pa, D.pc);
+  reverse_step<kTPB>(P, AM, B, T, tid, 1, T.lamA, T.lamB);
   for (int i = tid; i < 10 + 5 * NN; i += kTPB) {
     double lam = 0, rec = 0;
     if (i >= 1 && i <= 6) lam = T.lamB[i - 1];
@@ -400,7 +421,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
 namespace {
 
 // LDS layout of the adjoint's tape, in doubles (after the forward layout).
-AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig) {
+AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig) {
   AdjLayout L{};
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
@@ -408,12 +429,13 @@ AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t maxGroup, int32
   L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(kBw * kUnits); L.bt = take(20);
   L.oLvl = take(NN); L.oVol = take(NN); L.oArea = take(NN); L.oCx = take(NN); L.oCy = take(NN); L.oCz = take(NN); L.oZmax = take(NN);
   L.jac = take(NN * kSolveOut * kDof); L.carry = take(NN); L.gMer = take(NG); L.gH0 = take(NG);
-  L.mp = take(kUnits * maxGroup * kPassRec);
+  L.mp = take((kMaxTPB / kLanes) * maxGroup * kPassRec);
   L.hEff = take(NN); L.aEff = take(NN); L.aSrc = take(NN); L.deg = take(NN); L.dV = take(NC); L.cp = take(6 * NC);
   L.acc = take(NN); L.vol1 = take(NN); L.clip = take(NN); L.tot = take(T_COUNT);
   L.lamA = take(6 + 5 * NN); L.lamB = take(6 + 5 * NN); L.G = take(np); L.gOut = take(kSolveOut * NN); L.gVol1 = take(NN);
-  L.gAcc = take(NN); L.cg = take(5 * NC); L.red = take(3 * (kThreads / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
+  L.gAcc = take(NN); L.cg = take(5 * NC); L.red = take(3 * (kMaxTPB / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
   L.sig = take(nsig);
+  L.cpa = take(NC); L.cpc = take(NC); L.cmon = take(NC); L.monSeed = take(nmon); L.sflag = take(2);
   L.total = off;
   return L;
 }
@@ -432,6 +454,7 @@ struct AdjRun {
   Dev<int32_t> pa, pc, rdKind, evKind, evIdx, mismatch;
   Dev<long long> rdStep, steps, ckptOff, ringOff;
   size_t lds = 0;
+  int tpb = kThreads;
 };
 
 struct Setup {
@@ -442,12 +465,12 @@ struct Setup {
 };
 
 // Checks, uploads the batch and the adjoint's tables; returns the launch's LDS size.
-void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const double* obs, int32_t K, double tMax, long long maxSteps,
+void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const double* obs, int32_t& K, double tMax, long long maxSteps,
              bool oneStep) {
   const size_t count = static_cast<size_t>(b->opt.count);
   need(s.np >= 1 && s.np <= 64, "ss_batch_gradient: the parameter count must be 1..64");
   need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
-  need(b->tpb == kThreads, "ss_batch_gradient: the gradient runs on 256-thread workgroups (set_launch 256 or 0)");
+
   need(!b->mixed, "ss_batch_gradient: the gradient is FP64 (portable numerics)");
   const int32_t nT = loss ? loss->nRead : 0, nE = loss ? loss->nEvent : 0;
   need(nT >= 0 && nE >= 0 && nT + nE <= 256, "ss_batch_gradient: at most 256 loss terms");
@@ -477,46 +500,77 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
       ca[k] = coef * area;
     }
   }
-  upload(b);
+  // the workgroup size: 512 threads (buoyancy partials beside the level solve, 16 solver units) where the LDS of every
+  // model fits, else 256; the models are uploaded with that size's forward layout
+  const int32_t nseeds = nT + 2 * nE + 64 + 64 + 2;  // the one-step form seeds every monitor and mark, V and the top
+  auto need_lds = [&](int tpb) {
+    size_t l = 0;
+    for (const auto& m : b->models) {
+      const AdjLayout A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
+      l = std::max(l, (static_cast<size_t>(m->layout(tpb).total) + static_cast<size_t>(A.total)) * sizeof(double));
+    }
+    return l;
+  };
+  R.tpb = need_lds(kMaxTPB) <= lds_limit() ? kMaxTPB : kThreads;
+  const int32_t keepTpb = b->tpb;
+  b->tpb = R.tpb;
+  try {
+    upload(b);
+  } catch (...) {
+    b->tpb = keepTpb;
+    throw;
+  }
+  b->tpb = keepTpb;
   R.ca.upload(ca);
   R.theta.upload(s.theta, count * static_cast<size_t>(s.np));
   R.pa.upload(s.pa, nconn);
   R.pc.upload(s.pc, nconn);
   // per model: the tape layout
-  const int32_t nseeds = nT + 2 * nE + 64 + 64 + 2;  // the one-step form seeds every monitor and mark, V and the top
   std::vector<AdjModel> am;
   size_t lds = 0;
   for (const auto& m : b->models) {
     need(m->maxGroup <= 8, "ss_batch_gradient: free-surface groups of more than 8 spaces are not supported");
     need(m->nmon <= 64 && m->nmark <= 64, "ss_batch_gradient: at most 64 monitors and 64 marks");
     AdjModel a{};
-    a.A = adj_layout(m->NN, m->NC, m->NG, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
+    a.A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
     a.maxGroup = m->maxGroup;
     a.np = s.np;
     a.mode = s.relaxed ? 1 : 0;
     am.push_back(a);
-    lds = std::max(lds, (static_cast<size_t>(m->layout(kThreads).total) + static_cast<size_t>(a.A.total)) * sizeof(double));
+    lds = std::max(lds, (static_cast<size_t>(m->layout(R.tpb).total) + static_cast<size_t>(a.A.total)) * sizeof(double));
   }
   if (lds > lds_limit())
     fail(SS_RUNTIME, "ss_batch_gradient: " + std::to_string(lds) + " bytes of LDS per simulation exceed the device's limit");
   R.lds = lds;
   R.dAdjModels.upload(am);
-  // checkpoints and rings per instance
-  std::vector<long long> ckOff(count), rgOff(count);
-  long long ck = 0, rg = 0;
+  // each instance's step bound and state size; K = 0 chooses 1 (every state stored, no recomputation) when all of them
+  // fit in a third of the free device memory, else 256
+  std::vector<long long> caps(count), sizes(count);
+  double every = 0;
   for (size_t i = 0; i < count; ++i) {
     const Model& m = b->model(static_cast<int32_t>(i));
-    const long long SS = 7 + 5 * static_cast<long long>(m.NN);
+    sizes[i] = 7 + 5 * static_cast<long long>(m.NN);
     const double span = std::max(0.0, tMax - b->hSt[i * 7]);
     const double bound = span / m.net.dt + 2;
     long long cap = maxSteps > 0 ? maxSteps : LLONG_MAX;
     if (bound < 1e9) cap = std::min(cap, static_cast<long long>(std::ceil(bound)));
     need(cap <= 100000000LL, "ss_batch_gradient: the run is unbounded (set tMax or maxSteps to at most 1e8 steps)");
+    caps[i] = cap;
+    every += static_cast<double>(cap + 1) * static_cast<double>(sizes[i]) * sizeof(double);
+  }
+  if (K == 0) {
+    size_t freeB = 0, totalB = 0;
+    hip_check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
+    K = every <= static_cast<double>(freeB) / 3 ? 1 : 256;
+  }
+  std::vector<long long> ckOff(count), rgOff(count);
+  long long ck = 0, rg = 0;
+  for (size_t i = 0; i < count; ++i) {
     ckOff[i] = ck;
     rgOff[i] = rg;
     if (!oneStep) {
-      ck += (cap / K + 1) * SS;
-      rg += (static_cast<long long>(K) + 1) * SS;
+      ck += (caps[i] / K + 1) * sizes[i];
+      if (K > 1) rg += (static_cast<long long>(K) + 1) * sizes[i];
     }
   }
   R.ckpt.alloc(static_cast<size_t>(std::max(ck, 1LL)));
@@ -555,14 +609,18 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   R.dAdj.upload(&d, 1);
 }
 
-void launch(void (*kernel)(const ModelParams*, const AdjModel*, const DevBatch*, const DevAdj*), ss_batch* b, AdjRun& R, int grid, double* ms) {
+using AdjKernel = void (*)(const ModelParams*, const AdjModel*, const DevBatch*, const DevAdj*);
+
+void launch(bool oneStep, ss_batch* b, AdjRun& R, int grid, double* ms) {
+  const AdjKernel kernel = oneStep ? (R.tpb == kMaxTPB ? adjoint_one_step<kMaxTPB> : adjoint_one_step<kThreads>)
+                                   : (R.tpb == kMaxTPB ? adjoint_run<kMaxTPB> : adjoint_run<kThreads>);
   hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),
             "hipFuncSetAttribute");
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
   hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kThreads), R.lds, nullptr, static_cast<const ModelParams*>(b->dModels.p),
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(R.tpb), R.lds, nullptr, static_cast<const ModelParams*>(b->dModels.p),
                      static_cast<const AdjModel*>(R.dAdjModels.p), static_cast<const DevBatch*>(b->dBatch.p),
                      static_cast<const DevAdj*>(R.dAdj.p));
   hip_check(hipGetLastError(), "adjoint launch");
@@ -586,11 +644,12 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
   return guarded([&] {
     need(b != nullptr && opt != nullptr, "ss_batch_gradient: null argument");
     need(lossOut != nullptr, "ss_batch_gradient: null loss output");
-    need(opt->checkpointEvery >= 1 && opt->checkpointEvery <= 65536, "ss_batch_gradient: checkpointEvery must be 1..65536");
+    need(opt->checkpointEvery >= 0 && opt->checkpointEvery <= 65536, "ss_batch_gradient: checkpointEvery must be 0..65536");
     need(opt->relaxed == 0 || opt->relaxed == 1, "ss_batch_gradient: relaxed must be 0 or 1");
     const double tMax = opt->tMax > 0 ? opt->tMax : b->opt.tMax;
     AdjRun R;
-    prepare(b, R, Setup{nparams, pa, pc, theta, opt->relaxed}, loss, observations, opt->checkpointEvery, tMax, opt->maxSteps, false);
+    int32_t K = opt->checkpointEvery;
+    prepare(b, R, Setup{nparams, pa, pc, theta, opt->relaxed}, loss, observations, K, tMax, opt->maxSteps, false);
     if (!grad) {
       DevAdj d{};
       R.dAdj.download(&d, 1);
@@ -598,7 +657,7 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
       R.dAdj.upload(&d, 1);
     }
     double ms = 0;
-    launch(adjoint_run, b, R, b->opt.count, &ms);
+    launch(false, b, R, b->opt.count, &ms);
     const size_t count = static_cast<size_t>(b->opt.count);
     R.loss.download(lossOut, count);
     if (grad) R.grad.download(grad, count * static_cast<size_t>(nparams));
@@ -621,7 +680,8 @@ ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa,
     need(b != nullptr && record != nullptr && lambda != nullptr && lambdaOut != nullptr && gradLog != nullptr && recordOut != nullptr,
          "ss_batch_adjoint_step: null argument");
     AdjRun R;
-    prepare(b, R, Setup{nparams, pa, pc, theta, relaxed}, nullptr, nullptr, 1, b->opt.tMax, 1, true);
+    int32_t K = 1;
+    prepare(b, R, Setup{nparams, pa, pc, theta, relaxed}, nullptr, nullptr, K, b->opt.tMax, 1, true);
     const Model& m = b->model(0);
     const size_t rs = 10 + 5 * static_cast<size_t>(m.NN);
     R.stepRec.upload(record, rs);
@@ -638,7 +698,7 @@ ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa,
     d.stepRec = G(R.stepRec.p); d.stepLam = G(R.stepLam.p); d.stepMon = G(R.stepMon.p); d.stepMark = G(R.stepMark.p);
     d.stepLamOut = W(R.stepLamOut.p); d.stepRecOut = W(R.stepRecOut.p);
     R.dAdj.upload(&d, 1);
-    launch(adjoint_one_step, b, R, 1, nullptr);
+    launch(true, b, R, 1, nullptr);
     R.stepLamOut.download(lambdaOut, rs);
     R.stepRecOut.download(recordOut, rs);
     R.grad.download(gradLog, static_cast<size_t>(nparams));

@@ -40,6 +40,7 @@ struct AdjLayout {
   int32_t oLvl, oVol, oArea, oCx, oCy, oCz, oZmax, jac, carry, gMer, gH0, mp;
   int32_t hEff, aEff, aSrc, deg, dV, cp, acc, vol1, clip, tot;
   int32_t lamA, lamB, G, gOut, gVol1, gAcc, cg, red, seeds, nseeds, sig;
+  int32_t cpa, cpc, cmon, monSeed, sflag;
   int32_t total;
 };
 
@@ -84,6 +85,7 @@ struct Tape {
   SSL double *oLvl, *oVol, *oArea, *oCx, *oCy, *oCz, *oZmax, *jac, *carry, *gMer, *gH0, *mp;
   SSL double *hEff, *aEff, *aSrc, *deg, *dV, *cp, *acc, *vol1, *clip, *tot;
   SSL double *lamA, *lamB, *G, *gOut, *gVol1, *gAcc, *cg, *red, *seeds, *nseeds, *sig;
+  SSL double *cpa, *cpc, *cmon, *monSeed, *sflag;
 };
 
 __device__ inline Tape carve_tape(SSL double* base, const SSC AdjLayout& L) {
@@ -98,6 +100,7 @@ __device__ inline Tape carve_tape(SSL double* base, const SSC AdjLayout& L) {
   t.lamA = base + L.lamA; t.lamB = base + L.lamB; t.G = base + L.G; t.gOut = base + L.gOut; t.gVol1 = base + L.gVol1;
   t.gAcc = base + L.gAcc; t.cg = base + L.cg; t.red = base + L.red; t.seeds = base + L.seeds; t.nseeds = base + L.nseeds;
   t.sig = base + L.sig;
+  t.cpa = base + L.cpa; t.cpc = base + L.cpc; t.cmon = base + L.cmon; t.monSeed = base + L.monSeed; t.sflag = base + L.sflag;
   return t;
 }
 
@@ -107,8 +110,10 @@ enum : int { T_V = 0, T_MX, T_MY, T_MZ, T_IR, T_BX, T_BY, T_BZ, T_FB, T_BXW, T_B
              T_GMW, T_GIWP, T_GIWR, T_SEA, T_COUNT = 48 };
 // moment sums of `bt` (buoyancy) after the 4 values: T00 T01 T02 T03 T11 T12 T13 T22 T23 T33
 // moment sums are followed by the hull top's column (x, y, top) at bt[14..16]
-// seeds: records of 4 doubles (step, kind, index, value); kinds: 0 pitch after the step, 1 a monitor's flow sum,
-// 2 a mark's world z, 3 roll after the step, 4 the step's hull top, 5 the step's buoyancy volume
+// seeds: records of 4 doubles (step, kind, index, value), sorted by step, latest first, and consumed in that order by
+// the reverse sweep (sflag[0]: the next record; sflag[1]: whether this step has monitor seeds, in monSeed); kinds:
+// 0 pitch after the step, 1 a monitor's flow sum, 2 a mark's world z, 3 roll after the step, 4 the step's hull top,
+// 5 the step's buoyancy volume. cpa, cpc, cmon: per connection its parameter indices and its monitor (-1: none).
 
 // ------------------------------------------------------------------------------------------- recompute
 
@@ -464,12 +469,13 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const Tape& T, SSL do
   }
 }
 
-// levels.hpp:166-194: unit u takes groups u, u + 8, ... (the merge decision is the forward's: every group's
-// outputs are the same whichever unit computes them)
+// levels.hpp:166-194: unit u of the TPB / 32 units takes groups u, u + TPB / 32, ... (the merge decision is the
+// forward's: every group's outputs are the same whichever unit computes them)
+template <int TPB>
 __device__ inline void solve_levels_tv(PC& P, const Blk& B, const Tape& T, SSL double* mpBase, int maxGroup, int tid, bool relaxed) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
   SSL double* mp = mpBase + unit * maxGroup * kPassRec;
-  for (int g = unit; g < P.M.ng; g += kUnits) {
+  for (int g = unit; g < P.M.ng; g += TPB / kLanes) {
     const int N = B.gCount[g], first = B.gStart[g];
     const int mode = B.gMode[g];
     bool merged = false;
@@ -566,9 +572,10 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T,
     }
   }
   __syncthreads();
-  // (a) buoyancy with its moment sums, (b) the level solve with tangents
-  buoyancy_rec(P, T, tid);
-  solve_levels_tv(P, B, T, T.mp, AM.maxGroup, tid, relaxed);
+  // (a) buoyancy with its moment sums (the canonical 256 partials: the first 256 threads), (b) the level solve with
+  // tangents (every unit; with 512 threads the second 256 start it while the first compute the partials)
+  if (TPB == kThreads || tid < kThreads) buoyancy_rec(P, T, tid);
+  solve_levels_tv<TPB>(P, B, T, T.mp, AM.maxGroup, tid, relaxed);
   __syncthreads();
   // (c) heads, effective areas (and which of aov / amin / area gave it) and limiter degrees (flows.hpp:211-231)
   for (int n = tid; n < NN; n += TPB) {
@@ -779,6 +786,22 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T,
 
 // ------------------------------------------------------------------------------------------- reverse
 
+// The instance's per-connection parameter indices (pa, pc: its rows) and every connection's monitor into the tape;
+// clears the seed state. Ends with a barrier.
+template <int TPB>
+__device__ void load_adjoint_tables(PC& P, const Tape& T, const SSG int32_t* pa, const SSG int32_t* pc, int tid) {
+  const int NC = P.M.nc;
+  for (int c = tid; c < NC; c += TPB) { T.cpa[c] = pa[c]; T.cpc[c] = pc[c]; T.cmon[c] = -1; }
+  for (int m = tid; m < P.M.nmon; m += TPB) T.monSeed[m] = 0;
+  if (tid == 0) { T.sflag[0] = 0; T.sflag[1] = 0; }
+  __syncthreads();
+  for (int m = tid; m < P.M.nmon; m += TPB) {
+    const int s = P.M.monStart[m], e = s + P.M.monCount[m];
+    for (int j = s; j < e; ++j) T.cmon[P.M.monConn[j]] = m;
+  }
+  __syncthreads();
+}
+
 // Frame-slot gradients of one thread (F_R00..F_CTH), and the direct zO gradient of the loads' wz.
 struct FrameGrad {
   double g[kFrameDeriv];
@@ -789,23 +812,12 @@ __device__ __forceinline__ void add_world_z(FrameGrad& G, double gz, double x, d
   G.g[F_R20] += gz * x; G.g[F_R21] += gz * y; G.g[F_R22] += gz * z; G.g[F_TZ] += gz;
 }
 
-// Seeds of step k of kind `kind` (see the seeds record).
-__device__ __forceinline__ double seed_of(const Tape& T, long long k, int kind, int idx) {
-  const int ns = static_cast<int>(T.nseeds[0]);
-  double s = 0;
-  for (int i = 0; i < ns; ++i) {
-    const SSL double* r = T.seeds + 4 * i;
-    if (static_cast<long long>(r[0]) == k && static_cast<int>(r[1]) == kind && static_cast<int>(r[2]) == idx) s += r[3];
-  }
-  return s;
-}
-
 // The adjoint of step k: from lamCur (the adjoint of the state after the step, record order zO pitch roll vz wth
 // wph, vol, level, cx, cy, cz) and the loss's seeds at step k, the adjoint of the state before it in lamNext, and
-// the parameters' log-derivatives added to G. pa / pc: the instance's parameter indices per connection.
+// the parameters' log-derivatives added to G.
 template <int TPB>
 __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T, int tid, long long k, const SSL double* lamCur,
-                             SSL double* lamNext, const SSG int32_t* pa, const SSG int32_t* pc) {
+                             SSL double* lamNext) {
   const int NN = P.S.nn, NC = P.M.nc;
   const SSL double* fr = T.fr;
   const SSL double* o = T.tot;
@@ -816,8 +828,26 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
   // R0: zero the array part of lamNext; thread 0: integration, loads totals and buoyancy in reverse, mark seeds
   for (int i = 6 + tid; i < 6 + 5 * NN; i += TPB) lamNext[i] = 0;
   if (tid == 0) {
+    // this step's seeds (the records are sorted, latest step first)
+    double sPitch = 0, sRoll = 0, sTop = 0, sV = 0;
+    bool anyMon = false;
+    int pos = static_cast<int>(T.sflag[0]);
+    const int ns = static_cast<int>(T.nseeds[0]);
+    for (; pos < ns && static_cast<long long>(T.seeds[4 * pos]) == k; ++pos) {
+      const SSL double* r = T.seeds + 4 * pos;
+      const int kind = static_cast<int>(r[1]), idx = static_cast<int>(r[2]);
+      const double v = r[3];
+      if (kind == 0) sPitch += v;
+      else if (kind == 3) sRoll += v;
+      else if (kind == 4) sTop += v;
+      else if (kind == 5) sV += v;
+      else if (kind == 2) add_world_z(FG, v, P.M.mkx[idx], P.M.mky[idx], P.M.mkz[idx]);
+      else { T.monSeed[idx] += v; anyMon = true; }
+    }
+    T.sflag[0] = pos;
+    T.sflag[1] = anyMon ? 1.0 : 0.0;
     const double lz = lamCur[0], lvz = lamCur[3], lwp = lamCur[5];
-    const double lp = lamCur[1] + seed_of(T, k, 0, 0), lr = lamCur[2] + seed_of(T, k, 3, 0), lwt = lamCur[4];
+    const double lp = lamCur[1] + sPitch, lr = lamCur[2] + sRoll, lwt = lamCur[4];
     const bool cp_ = o[T_CLP] != 0, cr = o[T_CLR] != 0;
     const double gpRaw = cp_ ? 0.0 : lp, gwtRaw = (cp_ ? 0.0 : lwt) + gpRaw * dt;
     const double grRaw = cr ? 0.0 : lr, gwpRaw = (cr ? 0.0 : lwp) + grRaw * dt;
@@ -843,7 +873,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     FG.g[F_R00] += gbxw * bx; FG.g[F_R01] += gbxw * by; FG.g[F_R02] += gbxw * bz; FG.g[F_TX] += gbxw;
     FG.g[F_R10] += gbyw * bx; FG.g[F_R11] += gbyw * by; FG.g[F_R12] += gbyw * bz; FG.g[F_TY] += gbyw;
     const double gbx = gbxw * fr[F_R00] + gbyw * fr[F_R10], gby = gbxw * fr[F_R01] + gbyw * fr[F_R11], gbz = gbxw * fr[F_R02] + gbyw * fr[F_R12];
-    double gV = W * gFb + seed_of(T, k, 5, 0), gMx = 0, gMy = 0, gMz = 0;
+    double gV = W * gFb + sV, gMx = 0, gMy = 0, gMz = 0;
     const double V = o[T_V];
     if (V > 0) {
       gMx = gbx / V; gMy = gby / V; gMz = gbz / V;
@@ -858,15 +888,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     FG.g[F_R21] += -iR * (gV * T02 + gMx * T12 + gMy * T22 + gMz * T23);
     FG.g[F_R22] += -iR * (gV * T03 + gMx * T13 + gMy * T23 + gMz * T33);
     // hull-top seeds: the highest column top, base + R22 * top of its column
-    {
-      const double gt = seed_of(T, k, 4, 0);
-      if (gt != 0) add_world_z(FG, gt, T.bt[14], T.bt[15], T.bt[16]);
-    }
-    // mark seeds: the step's world z of each mark
-    for (int i = 0; i < P.M.nmark; ++i) {
-      const double gz = seed_of(T, k, 2, i);
-      if (gz != 0) add_world_z(FG, gz, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]);
-    }
+    if (sTop != 0) add_world_z(FG, sTop, T.bt[14], T.bt[15], T.bt[16]);
     SSL double* g = T.tot + T_GFZ;
     g[0] = gFz; g[1] = gTauX; g[2] = gTauY; g[3] = gmw; g[4] = gIwp; g[5] = gIwr;
   }
@@ -922,15 +944,10 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     double g = 0;
     if (dV != 0) {
       g = T.gAcc[ci.y] - (ci.x >= 0 ? T.gAcc[ci.x] : 0.0);
-      if (dV > 0) {
-        // monitor seeds: the step's flow sum of a monitor takes q = dV / dt of its connections with q > 0
-        for (int m = 0; m < P.M.nmon; ++m) {
-          const double s = seed_of(T, k, 1, m);
-          if (s == 0) continue;
-          const int s0 = P.M.monStart[m], s1 = s0 + P.M.monCount[m];
-          for (int j = s0; j < s1; ++j)
-            if (P.M.monConn[j] == c) g += s / dt;
-        }
+      // monitor seeds: the step's flow sum of a monitor takes q = dV / dt of its connections with q > 0
+      if (dV > 0 && T.sflag[1] != 0) {
+        const int m = static_cast<int>(T.cmon[c]);
+        if (m >= 0) g += T.monSeed[m] / dt;
       }
     }
     const SSL double* pd = T.cp + 6 * c;
@@ -955,11 +972,20 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     if (T.oVol[n] > kEmptyVolume) go[0] += gh;
     if (T.aSrc[n] == 2) go[2] += ga;
   }
-  for (int p = tid; p < AM.np; p += TPB) {
-    double s = 0;
-    for (int c = 0; c < NC; ++c)
-      if (pa[c] == p || pc[c] == p) s += T.cg[5 * c + 4];
-    T.G[p] += s;
+  {
+    // one wave per parameter: lane l sums connections l, l + 64, ... in order, then a fixed butterfly
+    const int wave = tid >> 6, lane = tid & 63;
+    for (int p = wave; p < AM.np; p += TPB / 64) {
+      double s = 0;
+      for (int c = lane; c < NC; c += 64)
+        if (T.cpa[c] == p || T.cpc[c] == p) s += T.cg[5 * c + 4];
+      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (lane == 0) T.G[p] += s;
+    }
+  }
+  if (tid == 0 && T.sflag[1] != 0) {  // the step's monitor seeds are used
+    for (int m = 0; m < P.M.nmon; ++m) T.monSeed[m] = 0;
+    T.sflag[1] = 0;
   }
   __syncthreads();
   // R4: per group: the level solve in reverse through its local Jacobians (rows along V, h0, zO, pitch, roll)

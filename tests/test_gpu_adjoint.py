@@ -176,7 +176,7 @@ def test_checkpoint_interval_does_not_change_the_gradient(hulls):
     thetas = np.exp(np.random.default_rng(2).uniform(-0.2, 0.2, (2, pm.count)))
     spec = _loss_spec([500, 1500, 3000], []).with_observations(np.array([0.05, 0.2, 0.4]))
     b = _batch([(ship, sim)], np.zeros(2, np.int32))
-    runs = [b.gradient(pm, thetas, spec, checkpoint_every=K, max_steps=3000) for K in (1, 37, 256, 5000)]
+    runs = [b.gradient(pm, thetas, spec, checkpoint_every=K, max_steps=3000) for K in (1, 37, 256, 5000, 0)]
     for r in runs[1:]:
         assert np.array_equal(r["grad"], runs[0]["grad"]) and np.array_equal(r["loss"], runs[0]["loss"])
         assert (r["mismatches"] == 0).all()

@@ -267,8 +267,10 @@ typedef struct ss_gradient_options {
  * theta [count * nparams] (null: forward only, no reverse sweep), the loss's signals [count * (nRead + nEvent)]: readout
  * values then event times (may be null), the steps run [count], the recomputed steps whose values differed from the
  * forward's [count] (always 0 for a correct kernel; may be null) and the kernel time (ms, may be null). FP64
- * (portable) batches on 256-thread workgroups only. The batch may be run any number of times this way; its own
- * run() is unaffected. */
+ * (portable) batches only. The launch takes 512 threads per simulation where the forward layout and the whole tape
+ * fit in LDS, else 256, else the same two with the tape's Jacobian rows and per-connection partials in global memory
+ * (ss_batch_gradient_launch reports which). The batch may be run any number of times this way; its own run() is
+ * unaffected. */
 ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta,
                             const ss_loss* loss, const double* observations, const ss_gradient_options* opt, double* lossOut,
                             double* grad, double* signals, int64_t* steps, int32_t* mismatches, double* kernelMs);
@@ -282,6 +284,11 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
 ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta, int32_t relaxed,
                                 const double* record, const double* lambda, const double* monitorSeed, const double* markSeed,
                                 double* lambdaOut, double* gradLog, double* recordOut);
+
+/* The launch shape of the last ss_batch_gradient or ss_batch_adjoint_step call: threads per simulation, 1 when the
+ * tape's Jacobian rows and per-connection partials were in global memory (else 0), LDS bytes per simulation. SS_LOGIC
+ * before any such call. No reference counterpart: a measurement and test hook. */
+ss_status ss_batch_gradient_launch(const ss_batch* b, int32_t* threads, int32_t* globalTape, int64_t* ldsBytes);
 
 #ifdef __cplusplus
 }

@@ -286,6 +286,13 @@ class Batch:
                                             steps.ctypes.data_as(C.POINTER(C.c_int64)), A.ptr(mism, C.c_int32), C.byref(ms)))
         return {"loss": loss, "grad": grad, "signals": sig[:, :nsig], "steps": steps, "mismatches": mism, "ms": ms.value}
 
+    def gradient_launch(self) -> dict:
+        """The last gradient call's launch shape: threads per simulation, whether the tape's large arrays were in
+        global memory, LDS bytes per simulation."""
+        t, g, l = C.c_int32(), C.c_int32(), C.c_int64()
+        A.check(self._lib.ss_batch_gradient_launch(self._h, C.byref(t), C.byref(g), C.byref(l)))
+        return {"threads": t.value, "global_tape": bool(g.value), "lds_bytes": l.value}
+
     def adjoint_step(self, params, theta, record, lam, monitor_seed=None, mark_seed=None, relaxed: bool = False):
         """One step of instance 0 in reverse (the adjoint's test hook): from a state record (trace layout) and the
         adjoint of the state after the step, the adjoint of the state before it, d loss / d log theta, and the

@@ -48,6 +48,8 @@ struct DevAdj {
   SSG int32_t* mismatch;           // per instance: recomputed steps whose values differ from the forward's
   SSG double *ckpt, *ring;
   const SSG long long *ckptOff, *ringOff;  // per instance, in doubles
+  SSG double* gtape;                       // the tape's arrays kept in global memory (GT kernels): gStride per instance
+  long long gStride;
   // the one-step form
   gcd *stepRec, *stepLam, *stepMon, *stepMark;
   SSG double *stepLamOut, *stepRecOut;
@@ -82,7 +84,8 @@ __device__ inline void restore_state(PC& P, const Blk& B, const SSG double* src,
     }
   }
 }
-__device__ inline void load_tape_state(PC& P, const Tape& T, const SSG double* src, int tid, int tpb) {
+template <bool GT>
+__device__ inline void load_tape_state(PC& P, const TapeT<GT>& T, const SSG double* src, int tid, int tpb) {
   const int NN = P.S.nn;
   for (int i = tid; i < 7 + 5 * NN; i += tpb) {
     const double v = src[i];
@@ -96,7 +99,8 @@ __device__ inline void load_tape_state(PC& P, const Tape& T, const SSG double* s
 }
 
 // the recomputed step's values against the stored next state; counts differences into *bad
-__device__ inline void verify_step(PC& P, const Tape& T, const SSG double* next, int tid, int tpb, SSL int32_t* bad) {
+template <bool GT>
+__device__ inline void verify_step(PC& P, const TapeT<GT>& T, const SSG double* next, int tid, int tpb, SSL int32_t* bad) {
   const int NN = P.S.nn;
   for (int i = tid; i < 7 + 5 * NN; i += tpb) {
     double v;
@@ -153,7 +157,7 @@ __device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double 
 
 constexpr int kMaxTPB = 512;
 
-template <int kTPB>
+template <int kTPB, bool GT>
 __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                      const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -169,7 +173,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
   AdjModel AM{};
   AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
   const Blk B = carve(lds, P);
-  const Tape T = carve_tape(lds + P.L.total, AMs.A);
+  const TapeT<GT> T = carve_tape<GT>(lds + P.L.total, D.gtape + static_cast<long long>(inst) * D.gStride, AMs.A);
   const int NN = P.S.nn;
   const int SS = 7 + 5 * NN;
   const int nT = D.nRead, nE = D.nEv, nSig = nT + nE;
@@ -353,7 +357,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
 // One step of instance 0 from a given state record, with a given adjoint of the state after it and given seeds on
 // the step's monitor sums and mark world z: the adjoint of the state before it, the parameters' log-derivatives and
 // the recomputed state after the step (the test hook of the step's reverse).
-template <int kTPB>
+template <int kTPB, bool GT>
 __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                           const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -368,7 +372,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
   AdjModel AM{};
   AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
   const Blk B = carve(lds, P);
-  const Tape T = carve_tape(lds + P.L.total, AMs.A);
+  const TapeT<GT> T = carve_tape<GT>(lds + P.L.total, D.gtape, AMs.A);
   const int NN = P.S.nn;
   load_instance(P, Bt, D, B, inst, tid, kTPB);
   // the record (t zO pitch roll vz wth wph inflow Vb hullTop, vol level cx cy cz) into the tape's input slots
@@ -421,22 +425,30 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
 namespace {
 
 // LDS layout of the adjoint's tape, in doubles (after the forward layout).
-AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig) {
+// global: the Jacobian rows and the connections' partials and gradient terms go to a per-instance global slice.
+AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig,
+                     bool global) {
   AdjLayout L{};
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
   L.sv = take(8); L.iVol = take(NN); L.iLvl = take(NN); L.iCx = take(NN); L.iCy = take(NN); L.iCz = take(NN);
   L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(kBw * kUnits); L.bt = take(20);
   L.oLvl = take(NN); L.oVol = take(NN); L.oArea = take(NN); L.oCx = take(NN); L.oCy = take(NN); L.oCz = take(NN); L.oZmax = take(NN);
-  L.jac = take(NN * kSolveOut * kDof); L.carry = take(NN); L.gMer = take(NG); L.gH0 = take(NG);
+  L.jac = global ? 0 : take(NN * kSolveOut * kDof); L.carry = take(NN); L.gMer = take(NG); L.gH0 = take(NG);
   L.mp = take((kMaxTPB / kLanes) * maxGroup * kPassRec);
-  L.hEff = take(NN); L.aEff = take(NN); L.aSrc = take(NN); L.deg = take(NN); L.dV = take(NC); L.cp = take(6 * NC);
+  L.hEff = take(NN); L.aEff = take(NN); L.aSrc = take(NN); L.deg = take(NN); L.dV = take(NC); L.cp = global ? 0 : take(6 * NC);
   L.acc = take(NN); L.vol1 = take(NN); L.clip = take(NN); L.tot = take(T_COUNT);
   L.lamA = take(6 + 5 * NN); L.lamB = take(6 + 5 * NN); L.G = take(np); L.gOut = take(kSolveOut * NN); L.gVol1 = take(NN);
-  L.gAcc = take(NN); L.cg = take(5 * NC); L.red = take(3 * (kMaxTPB / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
+  L.gAcc = take(NN); L.cg = global ? 0 : take(5 * NC); L.red = take(3 * (kMaxTPB / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
   L.sig = take(nsig);
   L.cpa = take(NC); L.cpc = take(NC); L.cmon = take(NC); L.monSeed = take(nmon); L.sflag = take(2);
   L.total = off;
+  if (global) {
+    L.gJac = 0;
+    L.gCp = NN * kSolveOut * kDof;
+    L.gCg = L.gCp + 6 * NC;
+    L.gTotal = L.gCg + 5 * NC;
+  }
   return L;
 }
 
@@ -453,8 +465,10 @@ struct AdjRun {
       stepRecOut;
   Dev<int32_t> pa, pc, rdKind, evKind, evIdx, mismatch;
   Dev<long long> rdStep, steps, ckptOff, ringOff;
+  Dev<double> gtape;
   size_t lds = 0;
   int tpb = kThreads;
+  bool global = false;  // the tape's three largest arrays in global memory
 };
 
 struct Setup {
@@ -500,18 +514,28 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
       ca[k] = coef * area;
     }
   }
-  // the workgroup size: 512 threads (buoyancy partials beside the level solve, 16 solver units) where the LDS of every
-  // model fits, else 256; the models are uploaded with that size's forward layout
+  // the launch shape, the first whose LDS fits every model: 512 threads (buoyancy partials beside the level solve, 16
+  // solver units) with the whole tape in LDS, 256 threads with the whole tape in LDS, then the same two with the
+  // tape's three largest arrays in global memory; the models are uploaded with that size's forward layout
   const int32_t nseeds = nT + 2 * nE + 64 + 64 + 2;  // the one-step form seeds every monitor and mark, V and the top
-  auto need_lds = [&](int tpb) {
+  auto need_lds = [&](int tpb, bool global) {
     size_t l = 0;
     for (const auto& m : b->models) {
-      const AdjLayout A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
+      const AdjLayout A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE, global);
       l = std::max(l, (static_cast<size_t>(m->layout(tpb).total) + static_cast<size_t>(A.total)) * sizeof(double));
     }
     return l;
   };
-  R.tpb = need_lds(kMaxTPB) <= lds_limit() ? kMaxTPB : kThreads;
+  {
+    const int shapes[4] = {kMaxTPB, kThreads, kMaxTPB, kThreads};
+    int pick = -1;
+    for (int k = 0; k < 4 && pick < 0; ++k)
+      if (need_lds(shapes[k], k >= 2) <= lds_limit()) pick = k;
+    if (pick < 0)
+      fail(SS_RUNTIME, "ss_batch_gradient: " + std::to_string(need_lds(kThreads, true)) + " bytes of LDS per simulation exceed the device's limit");
+    R.tpb = shapes[pick];
+    R.global = pick >= 2;
+  }
   const int32_t keepTpb = b->tpb;
   b->tpb = R.tpb;
   try {
@@ -528,11 +552,13 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   // per model: the tape layout
   std::vector<AdjModel> am;
   size_t lds = 0;
+  long long gStride = 0;
   for (const auto& m : b->models) {
     need(m->maxGroup <= 8, "ss_batch_gradient: free-surface groups of more than 8 spaces are not supported");
     need(m->nmon <= 64 && m->nmark <= 64, "ss_batch_gradient: at most 64 monitors and 64 marks");
     AdjModel a{};
-    a.A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE);
+    a.A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE, R.global);
+    gStride = std::max<long long>(gStride, a.A.gTotal);
     a.maxGroup = m->maxGroup;
     a.np = s.np;
     a.mode = s.relaxed ? 1 : 0;
@@ -542,6 +568,9 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   if (lds > lds_limit())
     fail(SS_RUNTIME, "ss_batch_gradient: " + std::to_string(lds) + " bytes of LDS per simulation exceed the device's limit");
   R.lds = lds;
+  b->adjTpb = R.tpb;
+  b->adjGlobal = R.global ? 1 : 0;
+  b->adjLds = lds;
   R.dAdjModels.upload(am);
   // each instance's step bound and state size; K = 0 chooses 1 (every state stored, no recomputation) when all of them
   // fit in a third of the free device memory, else 256
@@ -577,6 +606,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   R.ring.alloc(static_cast<size_t>(std::max(rg, 1LL)));
   R.ckptOff.upload(ckOff);
   R.ringOff.upload(rgOff);
+  R.gtape.alloc(R.global ? static_cast<size_t>(gStride) * (oneStep ? 1 : count) : 1);
   if (nT > 0) {
     R.rdStep.upload(reinterpret_cast<const long long*>(loss->rdStep), static_cast<size_t>(nT));
     R.rdKind.upload(loss->rdKind, static_cast<size_t>(nT));
@@ -606,14 +636,20 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   d.loss = W(R.loss.p); d.grad = W(R.grad.p); d.signals = W(R.signals.p); d.steps = W(R.steps.p); d.mismatch = W(R.mismatch.p);
   d.ckpt = W(R.ckpt.p); d.ring = W(R.ring.p);
   d.ckptOff = (const SSG long long*)R.ckptOff.p; d.ringOff = (const SSG long long*)R.ringOff.p;
+  d.gtape = W(R.gtape.p);
+  d.gStride = R.global ? gStride : 0;
   R.dAdj.upload(&d, 1);
 }
 
 using AdjKernel = void (*)(const ModelParams*, const AdjModel*, const DevBatch*, const DevAdj*);
 
 void launch(bool oneStep, ss_batch* b, AdjRun& R, int grid, double* ms) {
-  const AdjKernel kernel = oneStep ? (R.tpb == kMaxTPB ? adjoint_one_step<kMaxTPB> : adjoint_one_step<kThreads>)
-                                   : (R.tpb == kMaxTPB ? adjoint_run<kMaxTPB> : adjoint_run<kThreads>);
+  const bool wide = R.tpb == kMaxTPB;
+  const AdjKernel kernel =
+      oneStep ? (R.global ? (wide ? adjoint_one_step<kMaxTPB, true> : adjoint_one_step<kThreads, true>)
+                          : (wide ? adjoint_one_step<kMaxTPB, false> : adjoint_one_step<kThreads, false>))
+              : (R.global ? (wide ? adjoint_run<kMaxTPB, true> : adjoint_run<kThreads, true>)
+                          : (wide ? adjoint_run<kMaxTPB, false> : adjoint_run<kThreads, false>));
   hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),
             "hipFuncSetAttribute");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -702,6 +738,16 @@ ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa,
     R.stepLamOut.download(lambdaOut, rs);
     R.stepRecOut.download(recordOut, rs);
     R.grad.download(gradLog, static_cast<size_t>(nparams));
+  });
+}
+
+ss_status ss_batch_gradient_launch(const ss_batch* b, int32_t* threads, int32_t* globalTape, int64_t* ldsBytes) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_gradient_launch: null batch");
+    if (b->adjTpb == 0) fail(SS_LOGIC, "ss_batch_gradient_launch: no gradient call yet");
+    if (threads) *threads = b->adjTpb;
+    if (globalTape) *globalTape = b->adjGlobal;
+    if (ldsBytes) *ldsBytes = static_cast<int64_t>(b->adjLds);
   });
 }
 

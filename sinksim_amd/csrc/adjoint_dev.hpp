@@ -42,6 +42,7 @@ struct AdjLayout {
   int32_t lamA, lamB, G, gOut, gVol1, gAcc, cg, red, seeds, nseeds, sig;
   int32_t cpa, cpc, cmon, monSeed, sflag;
   int32_t total;
+  int32_t gJac, gCp, gCg, gTotal;  // the global slice (doubles per instance; 0 when everything is in LDS)
 };
 
 // The adjoint's per-model record beside ModelParams.
@@ -80,27 +81,41 @@ __device__ __forceinline__ TV tfabs(TV a) { return a.v < 0 ? -a : a; }
 
 // ------------------------------------------------------------------------------------------- the tape
 
-struct Tape {
+// The tape's arrays. The three largest -- the level solve's Jacobian rows (35 per space), the connections' partial
+// derivatives (6 each) and their gradient terms (5 each) -- are in LDS (GT = false) or, for models whose tape does not
+// fit beside the forward layout, in the instance's slice of a global buffer (GT = true); the rest is always in LDS.
+template <bool GT>
+struct TapeT {
+  using XP = std::conditional_t<GT, SSG double*, SSL double*>;
   SSL double *sv, *iVol, *iLvl, *iCx, *iCy, *iCz, *fr, *dfp, *bw, *bt;
-  SSL double *oLvl, *oVol, *oArea, *oCx, *oCy, *oCz, *oZmax, *jac, *carry, *gMer, *gH0, *mp;
-  SSL double *hEff, *aEff, *aSrc, *deg, *dV, *cp, *acc, *vol1, *clip, *tot;
-  SSL double *lamA, *lamB, *G, *gOut, *gVol1, *gAcc, *cg, *red, *seeds, *nseeds, *sig;
+  SSL double *oLvl, *oVol, *oArea, *oCx, *oCy, *oCz, *oZmax, *carry, *gMer, *gH0, *mp;
+  SSL double *hEff, *aEff, *aSrc, *deg, *dV, *acc, *vol1, *clip, *tot;
+  SSL double *lamA, *lamB, *G, *gOut, *gVol1, *gAcc, *red, *seeds, *nseeds, *sig;
   SSL double *cpa, *cpc, *cmon, *monSeed, *sflag;
+  XP jac, cp, cg;
 };
 
-__device__ inline Tape carve_tape(SSL double* base, const SSC AdjLayout& L) {
-  Tape t;
+// base: the tape's LDS; gbase: the instance's global slice (GT only; offsets gJac, gCp, gCg)
+template <bool GT>
+__device__ inline TapeT<GT> carve_tape(SSL double* base, SSG double* gbase, const SSC AdjLayout& L) {
+  TapeT<GT> t;
   t.sv = base + L.sv; t.iVol = base + L.iVol; t.iLvl = base + L.iLvl; t.iCx = base + L.iCx; t.iCy = base + L.iCy; t.iCz = base + L.iCz;
   t.fr = base + L.fr; t.dfp = base + L.dfp; t.bw = base + L.bw; t.bt = base + L.bt;
   t.oLvl = base + L.oLvl; t.oVol = base + L.oVol; t.oArea = base + L.oArea; t.oCx = base + L.oCx; t.oCy = base + L.oCy;
-  t.oCz = base + L.oCz; t.oZmax = base + L.oZmax; t.jac = base + L.jac; t.carry = base + L.carry; t.gMer = base + L.gMer;
+  t.oCz = base + L.oCz; t.oZmax = base + L.oZmax; t.carry = base + L.carry; t.gMer = base + L.gMer;
   t.gH0 = base + L.gH0; t.mp = base + L.mp;
-  t.hEff = base + L.hEff; t.aEff = base + L.aEff; t.aSrc = base + L.aSrc; t.deg = base + L.deg; t.dV = base + L.dV; t.cp = base + L.cp;
+  t.hEff = base + L.hEff; t.aEff = base + L.aEff; t.aSrc = base + L.aSrc; t.deg = base + L.deg; t.dV = base + L.dV;
   t.acc = base + L.acc; t.vol1 = base + L.vol1; t.clip = base + L.clip; t.tot = base + L.tot;
   t.lamA = base + L.lamA; t.lamB = base + L.lamB; t.G = base + L.G; t.gOut = base + L.gOut; t.gVol1 = base + L.gVol1;
-  t.gAcc = base + L.gAcc; t.cg = base + L.cg; t.red = base + L.red; t.seeds = base + L.seeds; t.nseeds = base + L.nseeds;
+  t.gAcc = base + L.gAcc; t.red = base + L.red; t.seeds = base + L.seeds; t.nseeds = base + L.nseeds;
   t.sig = base + L.sig;
   t.cpa = base + L.cpa; t.cpc = base + L.cpc; t.cmon = base + L.cmon; t.monSeed = base + L.monSeed; t.sflag = base + L.sflag;
+  if constexpr (GT) {
+    t.jac = gbase + L.gJac; t.cp = gbase + L.gCp; t.cg = gbase + L.gCg;
+  } else {
+    (void)gbase;
+    t.jac = base + L.jac; t.cp = base + L.cp; t.cg = base + L.cg;
+  }
   return t;
 }
 
@@ -254,13 +269,14 @@ __device__ __forceinline__ PassRec get_rec(const SSL double* m) {
 }
 
 // The solve's outputs of space n with this lane's tangent row (lanes 0..4 write their component).
-__device__ __forceinline__ void put_out(const Tape& T, int n, int lane, TV lvl, TV vol, TV area, TV cx, TV cy, TV cz, bool carry, TV zmax) {
+template <bool GT>
+__device__ __forceinline__ void put_out(const TapeT<GT>& T, int n, int lane, TV lvl, TV vol, TV area, TV cx, TV cy, TV cz, bool carry, TV zmax) {
   if (lane == 0) {
     T.oLvl[n] = lvl.v; T.oVol[n] = vol.v; T.oArea[n] = area.v; T.oCx[n] = cx.v; T.oCy[n] = cy.v; T.oCz[n] = cz.v; T.oZmax[n] = zmax.v;
     T.carry[n] = carry ? 1.0 : 0.0;
   }
   if (lane < kDof) {
-    SSL double* J = T.jac + n * (kSolveOut * kDof);
+    const auto J = T.jac + n * (kSolveOut * kDof);
     J[0 * kDof + lane] = lvl.d; J[1 * kDof + lane] = vol.d; J[2 * kDof + lane] = area.d;
     J[3 * kDof + lane] = carry ? 0.0 : cx.d; J[4 * kDof + lane] = carry ? 0.0 : cy.d; J[5 * kDof + lane] = carry ? 0.0 : cz.d;
     J[6 * kDof + lane] = zmax.d;
@@ -268,7 +284,8 @@ __device__ __forceinline__ void put_out(const Tape& T, int n, int lane, TV lvl, 
 }
 
 // levels.hpp:13-49 (full = false) for space n, with tangents along (V, h0, pose) = (vol[n], level[n], pose).
-__device__ inline void solve_single_tv(PC& P, const Blk& B, const Tape& T, int lane, int n, bool relaxed) {
+template <bool GT>
+__device__ inline void solve_single_tv(PC& P, const Blk& B, const TapeT<GT>& T, int lane, int n, bool relaxed) {
   const SSL double* fr = T.fr;
   const FrameTan f = frame_tan(T.dfp, lane);
   const double mu = B.nMu[n], iR = fr[F_IR];
@@ -322,7 +339,8 @@ __device__ inline void solve_single_tv(PC& P, const Blk& B, const Tape& T, int l
 // levels.hpp:54-162 (full = false) for a merged group, with tangents along (V, h0, pose), V the group's total
 // volume and h0 its starting level (the members' mean level when all are wet, else one member's level). The
 // members' pass records are kept in the unit's scratch (mp); every member's pass of a round is at the same level.
-__device__ inline void solve_group_tv(PC& P, const Blk& B, const Tape& T, SSL double* mp, int lane, int g, int first, int N, bool relaxed) {
+template <bool GT>
+__device__ inline void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mp, int lane, int g, int first, int N, bool relaxed) {
   const SSL double* fr = T.fr;
   const FrameTan f = frame_tan(T.dfp, lane);
   const double iR = fr[F_IR];
@@ -471,8 +489,8 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const Tape& T, SSL do
 
 // levels.hpp:166-194: unit u of the TPB / 32 units takes groups u, u + TPB / 32, ... (the merge decision is the
 // forward's: every group's outputs are the same whichever unit computes them)
-template <int TPB>
-__device__ inline void solve_levels_tv(PC& P, const Blk& B, const Tape& T, SSL double* mpBase, int maxGroup, int tid, bool relaxed) {
+template <int TPB, bool GT>
+__device__ inline void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mpBase, int maxGroup, int tid, bool relaxed) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
   SSL double* mp = mpBase + unit * maxGroup * kPassRec;
   for (int g = unit; g < P.M.ng; g += TPB / kLanes) {
@@ -504,7 +522,8 @@ __device__ inline void solve_levels_tv(PC& P, const Blk& B, const Tape& T, SSL d
 // buoyancy partials (hydro.hpp:242-258 values) with the moment sums of the partly wet columns and the column that
 // gives the hull top (carried through the canonical tree: a later lane replaces only a strictly higher top); each
 // 32-lane group's result goes to bw (5 values, 10 sums, the top column's x, y and top)
-__device__ inline void buoyancy_rec(PC& P, const Tape& T, int tid) {
+template <bool GT>
+__device__ inline void buoyancy_rec(PC& P, const TapeT<GT>& T, int tid) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
   const SSL double* fr = T.fr;
   const Fr64 F = fr64(fr);
@@ -555,8 +574,8 @@ __device__ inline void buoyancy_rec(PC& P, const Tape& T, int tid) {
 
 // The step's tape from the start state in the tape's input slots (sv, iVol..iCz). Ends with a barrier. The
 // values written (oLvl.., vol1, the integrated pose and velocities in tot) equal the forward step's.
-template <int TPB>
-__device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T, int tid) {
+template <int TPB, bool GT>
+__device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT>& T, int tid) {
   const int NN = P.S.nn, NC = P.M.nc;
   const bool relaxed = AM.mode == 1;
   // pose and its derivatives (frame.hpp:9-21)
@@ -605,7 +624,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T,
   // (c) flows with their partial derivatives (flows.hpp:234-274): cp = d dV / d(ha, hb, zo, aEff_a, aEff_b, ln ca)
   const double t = T.sv[0];
   for (int c = tid; c < NC; c += TPB) {
-    SSL double* pd = T.cp + 6 * c;
+    const auto pd = T.cp + 6 * c;
     for (int k = 0; k < 6; ++k) pd[k] = 0;
     T.dV[c] = 0;
     const i4v ci = P.M.conI[c];
@@ -788,8 +807,8 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T,
 
 // The instance's per-connection parameter indices (pa, pc: its rows) and every connection's monitor into the tape;
 // clears the seed state. Ends with a barrier.
-template <int TPB>
-__device__ void load_adjoint_tables(PC& P, const Tape& T, const SSG int32_t* pa, const SSG int32_t* pc, int tid) {
+template <int TPB, bool GT>
+__device__ void load_adjoint_tables(PC& P, const TapeT<GT>& T, const SSG int32_t* pa, const SSG int32_t* pc, int tid) {
   const int NC = P.M.nc;
   for (int c = tid; c < NC; c += TPB) { T.cpa[c] = pa[c]; T.cpc[c] = pc[c]; T.cmon[c] = -1; }
   for (int m = tid; m < P.M.nmon; m += TPB) T.monSeed[m] = 0;
@@ -815,8 +834,8 @@ __device__ __forceinline__ void add_world_z(FrameGrad& G, double gz, double x, d
 // The adjoint of step k: from lamCur (the adjoint of the state after the step, record order zO pitch roll vz wth
 // wph, vol, level, cx, cy, cz) and the loss's seeds at step k, the adjoint of the state before it in lamNext, and
 // the parameters' log-derivatives added to G.
-template <int TPB>
-__device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape& T, int tid, long long k, const SSL double* lamCur,
+template <int TPB, bool GT>
+__device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT>& T, int tid, long long k, const SSL double* lamCur,
                              SSL double* lamNext) {
   const int NN = P.S.nn, NC = P.M.nc;
   const SSL double* fr = T.fr;
@@ -938,7 +957,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
   __syncthreads();
   // R2: per connection: the gathers in reverse, monitor seeds, the flow's partial derivatives
   for (int c = tid; c < NC; c += TPB) {
-    SSL double* cg = T.cg + 5 * c;
+    const auto cg = T.cg + 5 * c;
     const double dV = T.dV[c];
     const i4v ci = P.M.conI[c];
     double g = 0;
@@ -950,7 +969,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
         if (m >= 0) g += T.monSeed[m] / dt;
       }
     }
-    const SSL double* pd = T.cp + 6 * c;
+    const auto pd = T.cp + 6 * c;
     cg[0] = g * pd[0]; cg[1] = g * pd[1]; cg[2] = g * pd[3]; cg[3] = g * pd[4]; cg[4] = g * pd[5];
     if (g != 0 && pd[2] != 0) {
       const d4v cd = P.M.conD[c];
@@ -965,7 +984,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     double gh = 0, ga = 0;
     for (int j = s; j < e; ++j) {
       const i4v q = P.M.inc[j];
-      const SSL double* cg = T.cg + 5 * q.x;
+      const auto cg = T.cg + 5 * q.x;
       if (q.w) { gh += cg[1]; ga += cg[3]; } else { gh += cg[0]; ga += cg[2]; }
     }
     SSL double* go = T.gOut + n * kSolveOut;
@@ -997,7 +1016,7 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     for (int kk = 0; kk < N; ++kk) {
       const int n = B.gNodes[first + kk];
       const SSL double* go = T.gOut + n * kSolveOut;
-      const SSL double* J = T.jac + n * (kSolveOut * kDof);
+      const auto J = T.jac + n * (kSolveOut * kDof);
       double gn[kDof] = {0, 0, 0, 0, 0};
       for (int oo = 0; oo < kSolveOut; ++oo)
         for (int l = 0; l < kDof; ++l) gn[l] += go[oo] * J[oo * kDof + l];

@@ -191,6 +191,9 @@ struct ss_batch {
   bool ran = false;
   double kernelMs = 0;
   int launches = 0;
+  // the last gradient call's launch shape: threads per simulation, the tape's large arrays in global memory, LDS bytes
+  int32_t adjTpb = 0, adjGlobal = 0;
+  size_t adjLds = 0;
 
   Dev<ModelParams> dModels;
   Dev<DevBatch> dBatch;

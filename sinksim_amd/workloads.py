@@ -4,6 +4,10 @@ titanic_batch_sweep: the 1,024-instance titanic64 batch the benchmark runs — p
 area of every connection kind present in the scenario drawn log-uniform in [0.5, 2], and each watertight door
 (kind "door", idx 2..13) overridden open with probability 0.2; tMax 4 h, curve every 30 s. It is written as a
 standard sinksim.sweep document, so the CPU tools run the identical instances.
+
+calibration_hull / calibration_problems: the gradient calibration workload on a synthetic hull of 16,000 columns,
+128 spaces and 600 connections over 20,000 steps, with 24 kind-zone area factors as parameters; each problem is a
+twin experiment with its own hidden scale factors.
 """
 from __future__ import annotations
 
@@ -79,3 +83,24 @@ def twin_experiment(count: int = 256, nparams: int = 12, seed: int = 20261008, s
     truth = np.exp(rng.uniform(-span, span, (count, nparams)))
     guess = np.exp(rng.uniform(-span, span, (count, nparams)))
     return truth, guess
+
+
+# the calibration hull: seed 5 of the synthetic generator at 16,000 columns, 128 spaces and 600 connections (16,680
+# columns after the plan's rounding, 61 free-surface groups of up to four spaces, 31 monitors, 3 marks); its scenario
+# stays afloat for the whole 20,000-step window (six events fire in it), so every run has the same length
+CALIBRATION_HULL = dict(seed=5, columns=16000, spaces=128, connections=600)
+CALIBRATION_STEPS = 20000
+CALIBRATION_TRIM_STEPS = (4800, 9600, 14400, 19200)  # 20, 40, 60 and 80 minutes at dt = 0.25 s
+
+
+def calibration_hull():
+    """(ship, sim) of the calibration workload."""
+    from .synth import synthetic_hull
+    return synthetic_hull(**CALIBRATION_HULL)
+
+
+def calibration_problems(count: int, seed: int = 20261008, span: float = 0.3, nparams: int = 24):
+    """Hidden true log scale factors of `count` twin experiments, uniform in [-span, span] (the reference's search
+    box around scales 1, tools/py/calibrate_gpu.py:90-91): [count, nparams]."""
+    rng = np.random.default_rng(seed)
+    return rng.uniform(-span, span, (count, nparams))

@@ -1,13 +1,14 @@
 """The adjoint kernel (sinksim_amd/csrc/adjoint_dev.hpp, adjoint.hip) through the C ABI, held to the CPU dual-number
 build (oracle/dual_step.hpp) and to finite differences.
 
-- One step: from states along titanic's run and two synthetic hulls' (one with a four-space group), the kernel's
-  recomputed step equals the oracle's step bit for bit, and its reverse passes the dot-product test against the dual
-  build's tangent of the same step, <lambda, J v> = <J^T lambda, v>, seeds on monitor sums and mark heights included,
-  in both derivative modes.
+- One step: from states along titanic's run, two synthetic hulls' (one with a four-space group) and the calibration
+  hull's (16,680 columns, 128 spaces, 600 connections: the tape's Jacobian rows and connection partials in global
+  memory), the kernel's recomputed step equals the oracle's step bit for bit, and its reverse passes the dot-product
+  test against the dual build's tangent of the same step, <lambda, J v> = <J^T lambda, v>, seeds on monitor sums and
+  mark heights included, in both derivative modes.
 - Whole runs: the loss and its signals equal the dual build's exactly; the gradient equals the dual's directional
   derivatives (exact and relaxed modes, titanic and a fleet of hulls); the gradient does not depend on the checkpoint
-  interval (bit for bit); the gradient matches central finite differences of the kernel's own loss (h = 1e-6
+  interval (bit for bit); on the calibration hull with 24 kind-zone parameters likewise; the gradient matches central finite differences of the kernel's own loss (h = 1e-6
   relative) to 1e-6 relative on a horizon where the perturbation flips no branch; every recomputed step matches the
   forward trajectory (no mismatches).
 """
@@ -34,8 +35,14 @@ def _batch(models, inst_model=None, t_max=4 * 3600.0):
 def hulls():
     from sinksim_amd.data_paths import titanic
     from sinksim_amd.synth import synthetic_hull
+    from sinksim_amd.workloads import calibration_hull
     return {"titanic": titanic("titanic"), "synthetic": synthetic_hull(11, columns=1500, spaces=16),
-            "synth4": synthetic_hull(12, columns=1500, spaces=16)}
+            "synth4": synthetic_hull(12, columns=1500, spaces=16), "large": calibration_hull()}
+
+
+def _params(which, sim):
+    from sinksim_amd.calib import kind_params, zone_params
+    return zone_params(sim) if which == "large" else kind_params(sim)
 
 
 def _oracle_states(ship, sim, steps, theta, pm):
@@ -63,13 +70,13 @@ def _oracle_states(ship, sim, steps, theta, pm):
 
 
 @pytest.mark.parametrize("which,steps", [("titanic", (0, 1, 700, 5000, 11574, 16300, 30000)),
-                                         ("synthetic", (0, 300, 2000, 6374, 12000)), ("synth4", (0, 150, 900, 4000))])
+                                         ("synthetic", (0, 300, 2000, 6374, 12000)), ("synth4", (0, 150, 900, 4000)),
+                                         ("large", (0, 2500, 7700, 19000))])
 @pytest.mark.parametrize("relaxed", [False, True])
 def test_one_step_values_and_dot_product(hulls, which, steps, relaxed):
     import pyoracle
-    from sinksim_amd.calib import kind_params
     ship, sim = hulls[which]
-    pm = kind_params(sim)
+    pm = _params(which, sim)
     rng = np.random.default_rng(len(steps) * 7 + relaxed)
     theta = np.exp(rng.uniform(-0.3, 0.3, pm.count))
     model = pyoracle.Model(ship, sim)
@@ -87,6 +94,7 @@ def test_one_step_values_and_dot_product(hulls, which, steps, relaxed):
         ms = rng.standard_normal(sim.monitors)
         mk = rng.standard_normal(sim.marks)
         lo, gl, ro = b.adjoint_step(pm, theta, rec, lam, ms, mk, relaxed=relaxed)
+        assert b.gradient_launch()["global_tape"] == (which == "large")
         where = f"{which} step {k} relaxed={relaxed}"
         keep = np.r_[0:7, 8, 10:10 + 5 * ship.nodes]
         bad = keep[ro[keep] != nxt[keep]]
@@ -144,12 +152,15 @@ def test_titanic_gradient_equals_dual(hulls, relaxed):
     b.close()
 
 
-def test_fleet_gradient_equals_dual(hulls):
+@pytest.mark.parametrize("fleet,inst", [(("titanic", "synthetic", "synth4"), (1, 0, 2, 1, 2)), (("titanic", "large"), (0, 1, 0))])
+def test_fleet_gradient_equals_dual(hulls, fleet, inst):
+    """A fleet's gradient; with the calibration hull in it every model runs with the tape's large arrays in global
+    memory, titanic included."""
     import pyoracle
     from sinksim_amd.calib import kind_params
-    models = [hulls["titanic"], hulls["synthetic"], hulls["synth4"]]
+    models = [hulls[w] for w in fleet]
     maps = [kind_params(sim) for _, sim in models]
-    inst = np.array([1, 0, 2, 1, 2], np.int32)
+    inst = np.array(inst, np.int32)
     rng = np.random.default_rng(5)
     thetas = np.exp(rng.uniform(-0.25, 0.25, (len(inst), maps[0].count)))
     horizon = 2500
@@ -157,6 +168,7 @@ def test_fleet_gradient_equals_dual(hulls):
     spec = spec.with_observations(np.array([0.1, 0.3, 0.5]))
     b = _batch(models, inst)
     g = b.gradient(maps, thetas, spec, checkpoint_every=100, relaxed=False, max_steps=horizon)
+    assert b.gradient_launch()["global_tape"] == ("large" in fleet)
     assert (g["mismatches"] == 0).all()
     for i, m in enumerate(inst):
         ship, sim = models[m]
@@ -166,6 +178,41 @@ def test_fleet_gradient_equals_dual(hulls):
         assert g["loss"][i] == d["loss"], (i, m)
         gv = float(g["grad"][i] @ v)
         assert abs(gv - d["dloss"][0]) <= 1e-9 * max(abs(d["dloss"][0]), 1e-300), (i, m, gv, d["dloss"][0])
+    b.close()
+
+
+@pytest.mark.parametrize("relaxed", [False, True])
+def test_calibration_hull_gradient_equals_dual(hulls, relaxed):
+    """24 kind-zone parameters on the calibration hull (the tape's large arrays in global memory), 8,000 steps with
+    trim readouts and every monitor and mark event (two fire)."""
+    import pyoracle
+    from sinksim_amd.calib import generic_loss
+    ship, sim = hulls["large"]
+    pm = _params("large", sim)
+    rng = np.random.default_rng(71 + relaxed)
+    n, horizon = 3, 8000
+    thetas = np.exp(rng.uniform(-0.3, 0.3, (n, pm.count)))
+    spec0 = generic_loss(sim, (2000, 5000, 8000))
+    model = pyoracle.Model(ship, sim)
+    en, area, tOn = sim.cEn.astype(np.uint8), sim.cArea.copy(), sim.cTOn.copy()
+    base = pyoracle.dual_run(model, en, area, tOn, pm.pa, pm.pc, thetas[0], seed=np.zeros((1, pm.count)), max_steps=horizon, spec=spec0)
+    spec = _with_obs(spec0, base["signals"], rng)
+    b = _batch([(ship, sim)], np.zeros(n, np.int32))
+    g = b.gradient(pm, thetas, spec, checkpoint_every=256, relaxed=relaxed, max_steps=horizon)
+    shape = b.gradient_launch()
+    assert shape["global_tape"] and shape["lds_bytes"] <= 160 * 1024, shape
+    assert (g["mismatches"] == 0).all(), g["mismatches"]
+    assert (g["steps"] == horizon).all()
+    assert (g["signals"][:, 3:] < 18000).any()  # event terms are exercised
+    for i in range(n):
+        v = rng.standard_normal(pm.count) * thetas[i]
+        d = pyoracle.dual_run(model, en, area, tOn, pm.pa, pm.pc, thetas[i], seed=v[None], max_steps=horizon, spec=spec, relaxed=relaxed)
+        assert g["loss"][i] == d["loss"], (i, g["loss"][i], d["loss"])
+        assert np.array_equal(g["signals"][i], d["signals"]), i
+        gv = float(g["grad"][i] @ v)
+        assert abs(gv - d["dloss"][0]) <= 1e-9 * max(abs(d["dloss"][0]), 1e-300), (i, gv, d["dloss"][0])
+        inactive = np.setdiff1d(np.arange(pm.count), pm.pa[pm.pa >= 0])
+        assert np.all(g["grad"][i][inactive] == 0)
     b.close()
 
 

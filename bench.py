@@ -42,13 +42,17 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint", "calibration"],
+    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint", "calibration", "calibration-titanic"],
                     help="titanic: the 1,024-instance titanic64 batch (the headline); uneven: the persistent batch of "
                          "seeded synthetic hulls of uneven size; adjoint: gradients of the twin-experiment loss of 256 "
-                         "titanic64 instances; calibration: the reference's 1,320-point search against gradient L-BFGS "
-                         "on the reference objective")
-    ap.add_argument("--starts", type=int, default=8, help="calibration: L-BFGS starts")
-    ap.add_argument("--iterations", type=int, default=12, help="calibration: adjoint calls of the L-BFGS at most")
+                         "titanic64 instances; calibration: twin experiments with 24 kind-zone factors on the 16,680-"
+                         "column calibration hull, the 1,320-run search per problem against batched L-BFGS with adjoint "
+                         "gradients, device time per problem; calibration-titanic: the reference's search against "
+                         "gradient L-BFGS on the reference objective (one problem)")
+    ap.add_argument("--starts", type=int, default=8, help="calibration-titanic: L-BFGS starts")
+    ap.add_argument("--problems", type=int, default=256, help="calibration: twin problems the L-BFGS batch calibrates")
+    ap.add_argument("--search-problems", type=int, default=8, help="calibration: problems the search also runs on")
+    ap.add_argument("--iterations", type=int, default=None, help="calibration: adjoint calls of the L-BFGS at most (default 16; 12 titanic)")
     ap.add_argument("--checkpoint", type=int, default=256, help="adjoint: steps between checkpoints")
     ap.add_argument("--exact", action="store_true", help="adjoint: the discrete program's derivative instead of the relaxed one")
     ap.add_argument("--count", type=int, default=None, help="instances (default 1024 titanic, 4096 uneven)")
@@ -230,6 +234,53 @@ def bench_adjoint(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_twin_calibration(args):
+    """Calibration on many problems (twin experiments on the calibration hull, 24 kind-zone area factors, 20,000
+    steps): the reference's search (5 rounds x 264 runs per problem) on `search-problems` of them against batched
+    L-BFGS with adjoint gradients on all `problems` from scales 1, then forward-only sampling around each problem's
+    best point, until every searched problem reaches its search's best objective or the rounds run out. value =
+    search device time per problem / gradient-method device time per problem when half of the searched problems
+    (the median problem) have reached their search's objective; the same ratio for all of them, and the share reached
+    within a tenth of the search's time, are reported beside it."""
+    import numpy as np
+    from sinksim_amd.calibrate import twin_compare
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    r = twin_compare(problems=args.problems, search_problems=args.search_problems, iterations=args.iterations or 16, log=log)
+    sw, lb, tp = r["search"], r["lbfgs"], r["problems"]
+    per_s, per_all, per_half = r["search_ms_per_problem"], r["ms_per_problem_all"], r["ms_per_problem_half"]
+    calls = len(lb.device_ms) - 1
+    tenth = [v for v, m in zip(r["share_reached"], r["device_ms"]) if m / args.problems <= per_s / 10]
+    out = {
+        "metric": "calibration device time per problem, search / adjoint L-BFGS at equal objective (median problem)",
+        "value": round(per_s / per_half, 2) if per_half else None, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * r["wall_s"], 1), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic: seeded hidden factors of twin experiments, observations from their runs",
+        "config": {"workload": f"calibration hull (16,680 columns, 128 spaces, 600 connections), {tp.steps} steps, 24 kind-zone area "
+                               f"factors, trim at 20/40/60/80 min + 34 event times, {args.problems} problems",
+                   "search": {"problems": len(r["target"]), "runs_per_problem": sw["evaluations"], "device_s": round(sw["device_ms"] / 1e3, 3),
+                              "device_s_per_problem": round(per_s / 1e3, 4), "sim_steps": sw["steps"],
+                              "median_J_by_round": [float(f"{v:.4g}") for v in np.median(sw["J_by_round"], axis=1)],
+                              "target_J": [float(f"{v:.4g}") for v in r["target"]]},
+                   "gradient": {"problems": args.problems, "lbfgs_calls": calls,
+                                "polish_rounds": r["phase"].count("polish"), "polish_runs_per_round": 16,
+                                "device_s": round(r["device_ms"][-1] / 1e3, 3),
+                                "device_s_per_problem_all_reached": round(per_all / 1e3, 4) if per_all else None,
+                                "device_s_per_problem_half_reached": round(per_half / 1e3, 4) if per_half else None,
+                                "lbfgs_sim_steps": lb.steps[-1],
+                                "share_reached": [round(v, 3) for v in r["share_reached"]],
+                                "phase": r["phase"],
+                                "device_s_cumulative": [round(v / 1e3, 2) for v in r["device_ms"]],
+                                "median_J": [float(f"{v:.4g}") for v in r["median_J"]]},
+                   "ratio_all_reached": round(per_s / per_all, 2) if per_all else None,
+                   "share_reached_within_a_tenth_of_search_time": tenth[-1] if tenth else 0.0,
+                   "device": _device()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_calibration(args):
     """The reference's calibration (tools/py/calibrate_gpu.py: 5 rounds of 264 log-uniform points around the best,
     span 0.3 shrinking by half, seed 20261008, objective J against the 1912 observations) on the GPU batch, then
@@ -243,7 +294,7 @@ def bench_calibration(args):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    r = compare(ship, sim, starts=args.starts, iterations=args.iterations, log=log)
+    r = compare(ship, sim, starts=args.starts, iterations=args.iterations or 12, log=log)
     sw, lb = r["search"], r["lbfgs"]
     reached = r["lbfgs_ms_to_search_J"]
     out = {
@@ -277,6 +328,9 @@ def main():
         bench_adjoint(args)
         return
     if args.workload == "calibration":
+        bench_twin_calibration(args)
+        return
+    if args.workload == "calibration-titanic":
         bench_calibration(args)
         return
     args.count = args.count or 1024

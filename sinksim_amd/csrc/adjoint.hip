@@ -157,7 +157,7 @@ __device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double 
 
 constexpr int kMaxTPB = 512;
 
-template <int kTPB, bool GT>
+template <int kTPB, bool GT, bool FWD>
 __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                      const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restric
     D.steps[inst] = done;
   }
   __syncthreads();
-  if (!D.wantGrad) return;
+  if (FWD || !D.wantGrad) return;
 
   // 3. reverse
   for (int i = tid; i < 6 + 5 * NN; i += kTPB) { T.lamA[i] = 0; T.lamB[i] = 0; }
@@ -426,11 +426,17 @@ namespace {
 
 // LDS layout of the adjoint's tape, in doubles (after the forward layout).
 // global: the Jacobian rows and the connections' partials and gradient terms go to a per-instance global slice.
+// forwardOnly: only what the forward and the loss use (the signals and the seed records).
 AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t maxGroup, int32_t np, int32_t nseeds, int32_t nsig,
-                     bool global) {
+                     bool global, bool forwardOnly = false) {
   AdjLayout L{};
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
+  if (forwardOnly) {
+    L.seeds = take(4 * nseeds); L.nseeds = take(1); L.sig = take(nsig);
+    L.total = off;
+    return L;
+  }
   L.sv = take(8); L.iVol = take(NN); L.iLvl = take(NN); L.iCx = take(NN); L.iCy = take(NN); L.iCz = take(NN);
   L.fr = take(F_COUNT); L.dfp = take(kFrameDeriv * 3); L.bw = take(kBw * kUnits); L.bt = take(20);
   L.oLvl = take(NN); L.oVol = take(NN); L.oArea = take(NN); L.oCx = take(NN); L.oCy = take(NN); L.oCz = take(NN); L.oZmax = take(NN);
@@ -469,6 +475,7 @@ struct AdjRun {
   size_t lds = 0;
   int tpb = kThreads;
   bool global = false;  // the tape's three largest arrays in global memory
+  bool forwardOnly = false;  // no reverse sweep: the forward-only kernel and layout
 };
 
 struct Setup {
@@ -479,8 +486,9 @@ struct Setup {
 };
 
 // Checks, uploads the batch and the adjoint's tables; returns the launch's LDS size.
+// wantGrad false: the forward alone (one checkpoint per instance, no ring).
 void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const double* obs, int32_t& K, double tMax, long long maxSteps,
-             bool oneStep) {
+             bool oneStep, bool wantGrad) {
   const size_t count = static_cast<size_t>(b->opt.count);
   need(s.np >= 1 && s.np <= 64, "ss_batch_gradient: the parameter count must be 1..64");
   need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
@@ -526,7 +534,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     }
     return l;
   };
-  {
+  if (wantGrad) {
     const int shapes[4] = {kMaxTPB, kThreads, kMaxTPB, kThreads};
     int pick = -1;
     for (int k = 0; k < 4 && pick < 0; ++k)
@@ -535,6 +543,11 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
       fail(SS_RUNTIME, "ss_batch_gradient: " + std::to_string(need_lds(kThreads, true)) + " bytes of LDS per simulation exceed the device's limit");
     R.tpb = shapes[pick];
     R.global = pick >= 2;
+  } else {
+    // the forward alone: the batch kernel's default shape and only the loss's records beside the forward layout
+    R.tpb = kThreads;
+    R.global = false;
+    R.forwardOnly = true;
   }
   const int32_t keepTpb = b->tpb;
   b->tpb = R.tpb;
@@ -557,7 +570,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     need(m->maxGroup <= 8, "ss_batch_gradient: free-surface groups of more than 8 spaces are not supported");
     need(m->nmon <= 64 && m->nmark <= 64, "ss_batch_gradient: at most 64 monitors and 64 marks");
     AdjModel a{};
-    a.A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE, R.global);
+    a.A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE, R.global, R.forwardOnly);
     gStride = std::max<long long>(gStride, a.A.gTotal);
     a.maxGroup = m->maxGroup;
     a.np = s.np;
@@ -587,7 +600,9 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     caps[i] = cap;
     every += static_cast<double>(cap + 1) * static_cast<double>(sizes[i]) * sizeof(double);
   }
-  if (K == 0) {
+  if (!wantGrad) {
+    K = INT32_MAX;
+  } else if (K == 0) {
     size_t freeB = 0, totalB = 0;
     hip_check(hipMemGetInfo(&freeB, &totalB), "hipMemGetInfo");
     K = every <= static_cast<double>(freeB) / 3 ? 1 : 256;
@@ -599,7 +614,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     rgOff[i] = rg;
     if (!oneStep) {
       ck += (caps[i] / K + 1) * sizes[i];
-      if (K > 1) rg += (static_cast<long long>(K) + 1) * sizes[i];
+      if (K > 1 && wantGrad) rg += (static_cast<long long>(K) + 1) * sizes[i];
     }
   }
   R.ckpt.alloc(static_cast<size_t>(std::max(ck, 1LL)));
@@ -628,7 +643,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   R.steps.fill(0, count);
   R.mismatch.fill(0, count);
   DevAdj d{};
-  d.K = K; d.wantGrad = 1; d.maxSteps = maxSteps > 0 ? maxSteps : LLONG_MAX; d.tMax = tMax;
+  d.K = K; d.wantGrad = wantGrad ? 1 : 0; d.maxSteps = maxSteps > 0 ? maxSteps : LLONG_MAX; d.tMax = tMax;
   d.theta = G(R.theta.p); d.pa = G(R.pa.p); d.pc = G(R.pc.p); d.ca = G(R.ca.p);
   d.nRead = nT; d.nEv = nE;
   d.rdStep = (const SSG long long*)R.rdStep.p; d.rdKind = G(R.rdKind.p); d.rdW = G(R.rdW.p); d.rdAfter = G(R.rdAfter.p);
@@ -648,8 +663,9 @@ void launch(bool oneStep, ss_batch* b, AdjRun& R, int grid, double* ms) {
   const AdjKernel kernel =
       oneStep ? (R.global ? (wide ? adjoint_one_step<kMaxTPB, true> : adjoint_one_step<kThreads, true>)
                           : (wide ? adjoint_one_step<kMaxTPB, false> : adjoint_one_step<kThreads, false>))
-              : (R.global ? (wide ? adjoint_run<kMaxTPB, true> : adjoint_run<kThreads, true>)
-                          : (wide ? adjoint_run<kMaxTPB, false> : adjoint_run<kThreads, false>));
+              : R.forwardOnly ? adjoint_run<kThreads, false, true>
+              : (R.global ? (wide ? adjoint_run<kMaxTPB, true, false> : adjoint_run<kThreads, true, false>)
+                          : (wide ? adjoint_run<kMaxTPB, false, false> : adjoint_run<kThreads, false, false>));
   hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),
             "hipFuncSetAttribute");
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -685,13 +701,7 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
     const double tMax = opt->tMax > 0 ? opt->tMax : b->opt.tMax;
     AdjRun R;
     int32_t K = opt->checkpointEvery;
-    prepare(b, R, Setup{nparams, pa, pc, theta, opt->relaxed}, loss, observations, K, tMax, opt->maxSteps, false);
-    if (!grad) {
-      DevAdj d{};
-      R.dAdj.download(&d, 1);
-      d.wantGrad = 0;
-      R.dAdj.upload(&d, 1);
-    }
+    prepare(b, R, Setup{nparams, pa, pc, theta, opt->relaxed}, loss, observations, K, tMax, opt->maxSteps, false, grad != nullptr);
     double ms = 0;
     launch(false, b, R, b->opt.count, &ms);
     const size_t count = static_cast<size_t>(b->opt.count);
@@ -717,7 +727,7 @@ ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa,
          "ss_batch_adjoint_step: null argument");
     AdjRun R;
     int32_t K = 1;
-    prepare(b, R, Setup{nparams, pa, pc, theta, relaxed}, nullptr, nullptr, K, b->opt.tMax, 1, true);
+    prepare(b, R, Setup{nparams, pa, pc, theta, relaxed}, nullptr, nullptr, K, b->opt.tMax, 1, true, true);
     const Model& m = b->model(0);
     const size_t rs = 10 + 5 * static_cast<size_t>(m.NN);
     R.stepRec.upload(record, rs);

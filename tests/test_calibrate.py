@@ -101,3 +101,53 @@ def test_search_candidates_follow_the_reference_draws(monkeypatch):
             for k in kinds] for i in range(6)]
     assert np.allclose(got, first, atol=1e-12)
     assert len(tr.best_J) == 2 and tr.evaluations == [6, 12]
+
+
+def test_lbfgs_core_per_start_targets():
+    """With one target per start the run stops once every start has reached its own; row_best tracks each start."""
+    from sinksim_amd.calibrate import lbfgs_core
+    c = np.array([[0.5, -0.2], [-0.1, 0.3], [0.2, 0.2]])
+
+    def evaluate(x):
+        d = x - c
+        return np.sum(d * d, axis=1), 2 * d
+
+    target = np.array([1e-3, 1e-3, np.inf])
+    tr = lbfgs_core(evaluate, np.zeros((3, 2)), iterations=30, target=target)
+    rows = np.array(tr.row_best)
+    assert rows.shape[1] == 3 and len(rows) == len(tr.best_J)
+    assert np.all(np.diff(rows, axis=0) <= 0)
+    assert np.all(rows[-1][:2] <= 1e-3)
+    assert not np.all(rows[-2][:2] <= 1e-3)  # it stopped at the first call where both had reached
+
+
+def test_twin_J_tracks_the_differentiable_loss():
+    """twin_J (curve and event log, event times at their firing step) against the dual build's loss (event times
+    interpolated within the step) on the calibration hull, 20,000 steps: equal but for the interpolation, which moves
+    each event term w r^2 by at most w (2 |r| dt + dt^2)."""
+    import pyoracle
+    from sinksim_amd.calib import generic_loss, zone_params
+    from sinksim_amd.calibrate import _area_rows, twin_J
+    from sinksim_amd.workloads import CALIBRATION_STEPS, CALIBRATION_TRIM_STEPS, calibration_hull
+    ship, sim = calibration_hull()
+    pm = zone_params(sim)
+    spec = generic_loss(sim, CALIBRATION_TRIM_STEPS)
+    rng = np.random.default_rng(9)
+    x_true, x = rng.uniform(-0.3, 0.3, (2, pm.count))
+    model = pyoracle.Model(ship, sim)
+    en, tOn = sim.cEn.astype(np.uint8), sim.cTOn.copy()
+    truth = pyoracle.dual_run(model, en, sim.cArea.copy(), tOn, pm.pa, pm.pc, np.exp(x_true), seed=np.zeros((1, pm.count)),
+                              max_steps=CALIBRATION_STEPS, spec=spec)
+    obs = truth["signals"]
+    assert truth["steps"] == CALIBRATION_STEPS and np.sum(obs[4:] < 18000) >= 3
+    d = pyoracle.dual_run(model, en, sim.cArea.copy(), tOn, pm.pa, pm.pc, np.exp(x), seed=np.zeros((1, pm.count)),
+                          max_steps=CALIBRATION_STEPS, spec=spec.with_observations(obs))
+    area = _area_rows(sim, pm, x[None])
+    out = pyoracle.sweep(ship, sim, "portable", en[None], area, tOn[None], CALIBRATION_STEPS * sim.dt, 30.0, threads=1)[0]
+    assert out["steps"] == CALIBRATION_STEPS
+    J = twin_J(out["curve"], out["events"], out["foundered"], out["founderT"], spec, obs, sim.dt)
+    n = len(spec.read_steps)
+    r = np.abs(d["signals"][n:] - obs[n:])
+    bound = float(np.sum(spec.ev_w * (2 * r * sim.dt + sim.dt ** 2)))
+    assert abs(J - d["loss"]) <= bound, (J, d["loss"], bound)
+    assert np.allclose(d["signals"][:n], [np.interp(s * sim.dt, out["curve"][:, 0], out["curve"][:, 1]) for s in spec.read_steps], rtol=0, atol=0)

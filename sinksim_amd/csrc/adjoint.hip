@@ -448,6 +448,7 @@ AdjLayout adj_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t m
   L.gAcc = take(NN); L.cg = global ? 0 : take(5 * NC); L.red = take(3 * (kMaxTPB / 64)); L.seeds = take(4 * nseeds); L.nseeds = take(1);
   L.sig = take(nsig);
   L.cpa = take(NC); L.cpc = take(NC); L.cmon = take(NC); L.monSeed = take(nmon); L.sflag = take(2);
+  L.gnext = take(1);
   L.total = off;
   if (global) {
     L.gJac = 0;

@@ -40,7 +40,7 @@ struct AdjLayout {
   int32_t oLvl, oVol, oArea, oCx, oCy, oCz, oZmax, jac, carry, gMer, gH0, mp;
   int32_t hEff, aEff, aSrc, deg, dV, cp, acc, vol1, clip, tot;
   int32_t lamA, lamB, G, gOut, gVol1, gAcc, cg, red, seeds, nseeds, sig;
-  int32_t cpa, cpc, cmon, monSeed, sflag;
+  int32_t cpa, cpc, cmon, monSeed, sflag, gnext;
   int32_t total;
   int32_t gJac, gCp, gCg, gTotal;  // the global slice (doubles per instance; 0 when everything is in LDS)
 };
@@ -92,6 +92,7 @@ struct TapeT {
   SSL double *hEff, *aEff, *aSrc, *deg, *dV, *acc, *vol1, *clip, *tot;
   SSL double *lamA, *lamB, *G, *gOut, *gVol1, *gAcc, *red, *seeds, *nseeds, *sig;
   SSL double *cpa, *cpc, *cmon, *monSeed, *sflag;
+  SSL double* gnext;  // the Jacobian solve's group counter (an int32)
   XP jac, cp, cg;
 };
 
@@ -110,6 +111,7 @@ __device__ inline TapeT<GT> carve_tape(SSL double* base, SSG double* gbase, cons
   t.gAcc = base + L.gAcc; t.red = base + L.red; t.seeds = base + L.seeds; t.nseeds = base + L.nseeds;
   t.sig = base + L.sig;
   t.cpa = base + L.cpa; t.cpc = base + L.cpc; t.cmon = base + L.cmon; t.monSeed = base + L.monSeed; t.sflag = base + L.sflag;
+  t.gnext = base + L.gnext;
   if constexpr (GT) {
     t.jac = gbase + L.gJac; t.cp = gbase + L.gCp; t.cg = gbase + L.gCg;
   } else {
@@ -493,7 +495,13 @@ template <int TPB, bool GT>
 __device__ inline void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mpBase, int maxGroup, int tid, bool relaxed) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
   SSL double* mp = mpBase + unit * maxGroup * kPassRec;
-  for (int g = unit; g < P.M.ng; g += TPB / kLanes) {
+  // each unit takes the next group from the counter (its lane 0's LDS atomic, broadcast to the unit), so units that
+  // start late (after the buoyancy partials) or draw cheap groups take more
+  for (;;) {
+    int g = 0;
+    if (lane == 0) g = atomicAdd((int*)(SSL int32_t*)T.gnext, 1);
+    g = static_cast<int>(lead32(static_cast<uint32_t>(g)));
+    if (g >= P.M.ng) break;
     const int N = B.gCount[g], first = B.gStart[g];
     const int mode = B.gMode[g];
     bool merged = false;
@@ -588,6 +596,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
     if (lane == 0) {
       write_frame(P, T.fr, T.sv[1], cth, sth, cph, sph);
       write_frame_deriv(P, T.dfp, cth, sth, cph, sph);
+      *(SSL int32_t*)T.gnext = 0;
     }
   }
   __syncthreads();

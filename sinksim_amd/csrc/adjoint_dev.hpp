@@ -747,10 +747,32 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
     }
     T.vol1[n] = v;
     T.clip[n] = clip;
+    // the load terms of the space (body.hpp:21-34), stored gated and signed as the forward's node update stores
+    // them (a space with vol <= 0 adds -0.0; tauY's term negated), for the six node-order chains of the tail
+    double tx = -0.0, ty = -0.0, tf = -0.0, tm = -0.0, tip = -0.0, tir = -0.0;
+    if (!(v <= 0)) {
+      const SSL double* fr = T.fr;
+      const double W = P.M.rho * P.M.g;
+      const double x = T.oCx[n], y = T.oCy[n], z = T.oCz[n];
+      const double wx = fr[F_R00] * x + fr[F_R01] * y + fr[F_R02] * z + fr[F_TX] - P.M.XO;
+      const double wy = fr[F_R10] * x + fr[F_R11] * y + fr[F_R12] * z + fr[F_TY] - P.M.YO;
+      const double wz = fr[F_R20] * x + fr[F_R21] * y + fr[F_R22] * z + fr[F_TZ] - T.sv[1];
+      const double f = -W * v;
+      const double m = P.M.rho * v;
+      tx = wy * f;
+      ty = -(wx * f);
+      tf = f;
+      tm = m;
+      tip = m * (wx * wx + wz * wz);
+      tir = m * (wy * wy + wz * wz);
+    }
+    B.tX[n] = tx; B.tY[n] = ty; B.tF[n] = tf; B.tM[n] = tm; B.tIp[n] = tip; B.tIr[n] = tir;
   }
   __syncthreads();
-  // (a) buoyancy totals, (d) loads and integration (body.hpp:11-57), one thread, in node order
-  if (tid == 0) {
+  // (a) buoyancy totals, (d) loads and integration (body.hpp:11-57) on wave 0: lanes 0..5 run the six node-order
+  // chains over the stored terms (the forward's step tail), lane 0 integrates
+  if (tid < 64) {
+    const int lane = tid;
     const SSL double* fr = T.fr;
     double tV = T.bw[0], tMx = T.bw[1], tMy = T.bw[2], tMz = T.bw[3], tTop = T.bw[4];
     double tc3[3] = {T.bw[15], T.bw[16], T.bw[17]};
@@ -762,33 +784,33 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
       if (q[4] > tTop) { tTop = q[4]; tc3[0] = q[15]; tc3[1] = q[16]; tc3[2] = q[17]; }
       for (int k = 0; k < 10; ++k) S[k] += q[5 + k];
     }
-    for (int k = 0; k < 10; ++k) T.bt[4 + k] = S[k];
-    T.bt[14] = tc3[0]; T.bt[15] = tc3[1]; T.bt[16] = tc3[2];
-    T.bt[0] = tV; T.bt[1] = tMx; T.bt[2] = tMy; T.bt[3] = tMz;
+    if (lane == 0) {
+      for (int k = 0; k < 10; ++k) T.bt[4 + k] = S[k];
+      T.bt[14] = tc3[0]; T.bt[15] = tc3[1]; T.bt[16] = tc3[2];
+      T.bt[0] = tV; T.bt[1] = tMx; T.bt[2] = tMy; T.bt[3] = tMz;
+    }
     const double V = tV;
     const double bx = tV > 0 ? tMx / tV : 0, by = tV > 0 ? tMy / tV : 0, bz = tV > 0 ? tMz / tV : 0;
     const double W = P.M.rho * P.M.g;
     const double Fb = W * V;
     const double bxw = fr[F_R00] * bx + fr[F_R01] * by + fr[F_R02] * bz + fr[F_TX];
     const double byw = fr[F_R10] * bx + fr[F_R11] * by + fr[F_R12] * bz + fr[F_TY];
-    double tauX = (byw - P.M.YO) * Fb, tauY = -(bxw - P.M.XO) * Fb;
-    double Fz = Fb - P.M.g * P.M.ms;
-    double mw = 0, Iwp = 0, Iwr = 0;
     const double zO = T.sv[1];
-    for (int n = 0; n < NN; ++n) {
-      const double Vn = T.vol1[n];
-      if (Vn <= 0) continue;
-      const double x = T.oCx[n], y = T.oCy[n], z = T.oCz[n];
-      const double wx = fr[F_R00] * x + fr[F_R01] * y + fr[F_R02] * z + fr[F_TX] - P.M.XO;
-      const double wy = fr[F_R10] * x + fr[F_R11] * y + fr[F_R12] * z + fr[F_TY] - P.M.YO;
-      const double wz = fr[F_R20] * x + fr[F_R21] * y + fr[F_R22] * z + fr[F_TZ] - zO;
-      const double fn = -W * Vn;
-      tauX += wy * fn; tauY -= wx * fn; Fz += fn;
-      const double m = P.M.rho * Vn;
-      mw += m;
-      Iwp += m * (wx * wx + wz * wz);
-      Iwr += m * (wy * wy + wz * wz);
+    double acc = 0;
+    if (lane < 6) {
+      const SSL double* src = lane == 0 ? B.tX : lane == 1 ? B.tY : lane == 2 ? B.tF : lane == 3 ? B.tM : lane == 4 ? B.tIp : B.tIr;
+      acc = lane == 0 ? (byw - P.M.YO) * Fb : lane == 1 ? -(bxw - P.M.XO) * Fb : lane == 2 ? Fb - P.M.g * P.M.ms : 0.0;
+      for (int n0 = 0; n0 < NN; n0 += 8) {
+        double sv8[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sv8[k] = src[n0 + k < NN ? n0 + k : NN - 1];
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+          if (n0 + k < NN) acc += sv8[k];
+      }
     }
+    const double tauX = bcast(acc, 0), tauY = bcast(acc, 1), Fz = bcast(acc, 2), mw = bcast(acc, 3);
+    const double Iwp = bcast(acc, 4), Iwr = bcast(acc, 5);
     const double Mth = tauY, Mph = tauX * fr[F_CTH];
     const double dt = P.M.dt;
     double vz = T.sv[4], wth = T.sv[5], wph = T.sv[6];
@@ -803,11 +825,13 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
     if (pitch < -kPitchClamp) { pitch = -kPitchClamp; wth = 0; clp = 1; }
     if (roll > kRollClamp) { roll = kRollClamp; wph = 0; clr = 1; }
     if (roll < -kRollClamp) { roll = -kRollClamp; wph = 0; clr = 1; }
-    SSL double* o = T.tot;
-    o[T_V] = V; o[T_MX] = tMx; o[T_MY] = tMy; o[T_MZ] = tMz; o[T_IR] = fr[F_IR]; o[T_BX] = bx; o[T_BY] = by; o[T_BZ] = bz;
-    o[T_FB] = Fb; o[T_BXW] = bxw; o[T_BYW] = byw; o[T_TAUX] = tauX; o[T_TAUY] = tauY; o[T_FZ] = Fz; o[T_MW] = mw;
-    o[T_IWP] = Iwp; o[T_IWR] = Iwr; o[T_MH] = mh; o[T_ITH] = Ith; o[T_IPH] = Iph; o[T_AZ] = az; o[T_ATH] = ath; o[T_APH] = aph;
-    o[T_CLP] = clp; o[T_CLR] = clr; o[T_VZ1] = vz; o[T_WTH1] = wth; o[T_WPH1] = wph; o[T_ZO1] = zO1; o[T_P1] = pitch; o[T_R1] = roll;
+    if (lane == 0) {
+      SSL double* o = T.tot;
+      o[T_V] = V; o[T_MX] = tMx; o[T_MY] = tMy; o[T_MZ] = tMz; o[T_IR] = fr[F_IR]; o[T_BX] = bx; o[T_BY] = by; o[T_BZ] = bz;
+      o[T_FB] = Fb; o[T_BXW] = bxw; o[T_BYW] = byw; o[T_TAUX] = tauX; o[T_TAUY] = tauY; o[T_FZ] = Fz; o[T_MW] = mw;
+      o[T_IWP] = Iwp; o[T_IWR] = Iwr; o[T_MH] = mh; o[T_ITH] = Ith; o[T_IPH] = Iph; o[T_AZ] = az; o[T_ATH] = ath; o[T_APH] = aph;
+      o[T_CLP] = clp; o[T_CLR] = clr; o[T_VZ1] = vz; o[T_WTH1] = wth; o[T_WPH1] = wph; o[T_ZO1] = zO1; o[T_P1] = pitch; o[T_R1] = roll;
+    }
   }
   __syncthreads();
 }

@@ -491,7 +491,7 @@ struct Setup {
 void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const double* obs, int32_t& K, double tMax, long long maxSteps,
              bool oneStep, bool wantGrad) {
   const size_t count = static_cast<size_t>(b->opt.count);
-  need(s.np >= 1 && s.np <= 64, "ss_batch_gradient: the parameter count must be 1..64");
+  need(s.np >= 1 && s.np <= kMaxParams, "ss_batch_gradient: the parameter count must be 1..64");
   need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
 
   need(!b->mixed, "ss_batch_gradient: the gradient is FP64 (portable numerics)");

@@ -32,6 +32,7 @@ constexpr int kSolveOut = 7;  // per space: level, volume (after group sharing),
 constexpr int kPassRec = 23;  // a node pass's reduced record: 7 values, 10 moment sums, 6 extent coordinates
 constexpr int kFrameDeriv = 13;  // frame slots F_R00..F_CTH with derivatives along the pose
 constexpr int kBw = 18;          // a 32-lane group's buoyancy record: V Mx My Mz top, 10 moment sums, top column x y top
+constexpr int kMaxParams = 64;   // parameters of a gradient call at most
 
 // LDS layout of the adjoint's step tape, offsets in doubles from the end of the forward layout (the host computes
 // it: adj_layout in adjoint.hip).
@@ -1015,24 +1016,50 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
   for (int n = tid; n < NN; n += TPB) {
     const int s = P.M.incStart[n], e = s + P.M.incCount[n];
     double gh = 0, ga = 0;
-    for (int j = s; j < e; ++j) {
-      const i4v q = P.M.inc[j];
-      const auto cg = T.cg + 5 * q.x;
-      if (q.w) { gh += cg[1]; ga += cg[3]; } else { gh += cg[0]; ga += cg[2]; }
+    // four entries' loads issued together, then added in incidence order
+    for (int j0 = s; j0 < e; j0 += 4) {
+      i4v q[4];
+      double a[4], b[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) q[k] = P.M.inc[j0 + k < e ? j0 + k : e - 1];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const auto cg = T.cg + 5 * q[k].x;
+        a[k] = q[k].w ? cg[1] : cg[0];
+        b[k] = q[k].w ? cg[3] : cg[2];
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (j0 + k < e) { gh += a[k]; ga += b[k]; }
     }
     SSL double* go = T.gOut + n * kSolveOut;
     if (T.oVol[n] > kEmptyVolume) go[0] += gh;
     if (T.aSrc[n] == 2) go[2] += ga;
   }
   {
-    // one wave per parameter: lane l sums connections l, l + 64, ... in order, then a fixed butterfly
+    // wave w takes parameters w, w + TPB/64, ...: lane l reads connections l, l + 64, ... once and adds each to the
+    // sums of the wave's parameters it belongs to (per parameter in connection order), then a fixed butterfly
+    constexpr int kWaves = TPB / 64, kPer = (kMaxParams + kWaves - 1) / kWaves;
     const int wave = tid >> 6, lane = tid & 63;
-    for (int p = wave; p < AM.np; p += TPB / 64) {
-      double s = 0;
-      for (int c = lane; c < NC; c += 64)
-        if (T.cpa[c] == p || T.cpc[c] == p) s += T.cg[5 * c + 4];
-      for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-      if (lane == 0) T.G[p] += s;
+    double sp[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) sp[k] = 0;
+    for (int c = lane; c < NC; c += 64) {
+      const double ia = T.cpa[c], ic = T.cpc[c];
+      const double v = T.cg[5 * c + 4];
+#pragma unroll
+      for (int k = 0; k < kPer; ++k) {
+        const int p = wave + k * kWaves;
+        if (p < AM.np && (ia == p || ic == p)) sp[k] += v;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int p = wave + k * kWaves;
+      if (p >= AM.np) break;
+      double sv = sp[k];
+      for (int off = 32; off > 0; off >>= 1) sv += __shfl_xor(sv, off, 64);
+      if (lane == 0) T.G[p] += sv;
     }
   }
   if (tid == 0 && T.sflag[1] != 0) {  // the step's monitor seeds are used

@@ -53,7 +53,11 @@ def parse():
     ap.add_argument("--problems", type=int, default=256, help="calibration: twin problems the L-BFGS batch calibrates")
     ap.add_argument("--search-problems", type=int, default=8, help="calibration: problems the search also runs on")
     ap.add_argument("--iterations", type=int, default=None, help="calibration: adjoint calls of the L-BFGS at most (default 16; 12 titanic)")
-    ap.add_argument("--checkpoint", type=int, default=256, help="adjoint: steps between checkpoints")
+    ap.add_argument("--checkpoint", type=int, default=256, help="adjoint: steps between checkpoints (0: automatic)")
+    ap.add_argument("--hull", default="titanic", choices=["titanic", "calibration"],
+                    help="adjoint: titanic64 twin experiments (12 parameters), or the calibration hull's twin problems "
+                         "(24 kind-zone factors, 20,000 steps)")
+    ap.add_argument("--polish-rounds", type=int, default=8, help="calibration: forward-only sampling rounds after the L-BFGS")
     ap.add_argument("--exact", action="store_true", help="adjoint: the discrete program's derivative instead of the relaxed one")
     ap.add_argument("--count", type=int, default=None, help="instances (default 1024 titanic, 4096 uneven)")
     ap.add_argument("--tmax", type=float, default=None, help="s (default 4 h titanic, 2 h uneven)")
@@ -188,11 +192,24 @@ def bench_adjoint(args):
     from sinksim_amd.data_paths import titanic
     from sinksim_amd.workloads import twin_experiment
     count = args.count or 256
-    t_max = args.tmax or 260 * 60.0
-    ship, sim = titanic("titanic")
-    pm = kind_params(sim)
-    spec = titanic_loss(sim)
-    truth, guess = twin_experiment(count, pm.count)
+    if args.hull == "calibration":
+        from sinksim_amd.calibrate import twin_problems
+        from sinksim_amd.workloads import calibration_problems
+        tp = twin_problems(count)
+        ship, sim, pm, spec = tp.ship, tp.sim, tp.params, tp.spec
+        t_max = tp.steps * sim.dt
+        truth = np.exp(tp.truth)
+        guess = np.exp(calibration_problems(count, seed=7, nparams=pm.count))
+        label = (f"{count} calibration-hull twin problems (16,680 columns, 128 spaces, 600 connections), 24 kind-zone factors, "
+                 f"trim at 20/40/60/80 min + 34 event times, {tp.steps} steps")
+    else:
+        t_max = args.tmax or 260 * 60.0
+        ship, sim = titanic("titanic")
+        pm = kind_params(sim)
+        spec = titanic_loss(sim)
+        truth, guess = twin_experiment(count, pm.count)
+        label = (f"{count} titanic64 instances, twin experiment, 12 scale factors, trim at 15/30/45/75 min + overF/well/bridge "
+                 f"times, to founder or 260 min")
     b = Batch(ship, sim, BatchOptions(count=count, mixed_precision=False, traced_instances=0, curve_every=0.0, t_max=t_max))
     twin = b.gradient(pm, truth, spec, t_max=t_max, want_grad=False)
     obs = twin["signals"]
@@ -217,17 +234,17 @@ def bench_adjoint(args):
             raise RuntimeError(f"{int(r['mismatches'].sum())} recomputed steps differ from the forward trajectory")
         print(f"step {k}: {count} gradients in {r['ms']:.0f} ms kernel, {wall:.2f} s wall; mean loss {r['loss'].mean():.3f}",
               file=sys.stderr, flush=True)
+    b_launch = b.gradient_launch()
     b.close()
     steps = int(r["steps"].sum())
     secs = tot_ms * 1e-3 / args.steps
     out = {
-        "metric": "adjoint gradients/s (twin-experiment loss, 12 parameters)", "value": round(count / secs, 3),
+        "metric": f"adjoint gradients/s (twin-experiment loss, {pm.count} parameters)", "value": round(count / secs, 3),
         "unit": "gradients/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(1e3 * tot_wall / args.steps, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (seeded hidden parameters of the titanic64 model)",
-        "config": {"workload": f"{count} titanic64 instances, twin experiment, 12 scale factors, trim at 15/30/45/75 min + "
-                               f"overF/well/bridge times, relaxed={not args.exact}, checkpoint every {args.checkpoint} steps, "
-                               f"to founder or {t_max / 60:g} min", "instances": count, "sim_steps": steps,
+        "vs_baseline": None, "dtype": "f64", "data": f"synthetic (seeded hidden parameters of the {args.hull} model)",
+        "config": {"workload": f"{label}, relaxed={not args.exact}, checkpoint every {args.checkpoint} steps (0: automatic)",
+                   "instances": count, "sim_steps": steps, "launch": b_launch,
                    "forward_only_ms": round(fwd["ms"], 1), "gradient_over_forward": round(tot_ms / args.steps / fwd["ms"], 2),
                    "device": _device()},
     }
@@ -248,7 +265,8 @@ def bench_twin_calibration(args):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    r = twin_compare(problems=args.problems, search_problems=args.search_problems, iterations=args.iterations or 16, log=log)
+    r = twin_compare(problems=args.problems, search_problems=args.search_problems, iterations=args.iterations or 16,
+                     polish_rounds=args.polish_rounds, log=log)
     sw, lb, tp = r["search"], r["lbfgs"], r["problems"]
     per_s, per_all, per_half = r["search_ms_per_problem"], r["ms_per_problem_all"], r["ms_per_problem_half"]
     calls = len(lb.device_ms) - 1

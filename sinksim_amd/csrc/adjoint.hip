@@ -545,8 +545,16 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     R.tpb = shapes[pick];
     R.global = pick >= 2;
   } else {
-    // the forward alone: the batch kernel's default shape and only the loss's records beside the forward layout
-    R.tpb = kThreads;
+    // the forward alone, with only the loss's records beside the forward layout: 512 threads while the batch leaves
+    // compute units without an instance (each instance's latency is the run's), else 256 (more resident instances)
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+    size_t wideLds = 0;
+    for (const auto& m : b->models) {
+      const AdjLayout A = adj_layout(m->NN, m->NC, m->NG, m->nmon, m->maxGroup, s.np, nseeds, 2 * nT + 10 * nE, false, true);
+      wideLds = std::max(wideLds, (static_cast<size_t>(m->layout(kMaxTPB).total) + static_cast<size_t>(A.total)) * sizeof(double));
+    }
+    R.tpb = count <= static_cast<size_t>(cus) && wideLds <= lds_limit() ? kMaxTPB : kThreads;
     R.global = false;
     R.forwardOnly = true;
   }
@@ -664,7 +672,7 @@ void launch(bool oneStep, ss_batch* b, AdjRun& R, int grid, double* ms) {
   const AdjKernel kernel =
       oneStep ? (R.global ? (wide ? adjoint_one_step<kMaxTPB, true> : adjoint_one_step<kThreads, true>)
                           : (wide ? adjoint_one_step<kMaxTPB, false> : adjoint_one_step<kThreads, false>))
-              : R.forwardOnly ? adjoint_run<kThreads, false, true>
+              : R.forwardOnly ? (wide ? adjoint_run<kMaxTPB, false, true> : adjoint_run<kThreads, false, true>)
               : (R.global ? (wide ? adjoint_run<kMaxTPB, true, false> : adjoint_run<kThreads, true, false>)
                           : (wide ? adjoint_run<kMaxTPB, false, false> : adjoint_run<kThreads, false, false>));
   hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),

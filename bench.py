@@ -5,7 +5,7 @@ scale on every connection kind's area and each watertight door opened with p = 0
 cap, with the 30 s readout curves recorded (sinksim_amd.workloads.titanic_batch_sweep, seed 20261008).
 
 One bench step = one whole batch run (Batch.run(): every instance to its end, inputs already in HBM). value =
-sum over instances of steps / wall time of the timed runs. The roofline is that of the dominant kernel (run_chunk,
+sum over instances of steps / wall time of the timed runs. The roofline is that of the dominant kernel (run_batch,
 FP64): algorithmic FP64 operations per simulation step (measured with the oracle's operation counter on a sample of
 the same workload) x steps, over the kernel's HIP-event duration. The CPU baseline is the reference's own CPU
 engine (oracle/_ref/sinksim_sweep, compiled from the reference's sources) on a bounded sample of the same sweep;
@@ -32,9 +32,9 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two o
 # and 5,135 dry segment evaluations, 72.8 node passes, 51.6 active connections, 34.5 pow calls) and priced by
 # oracle/pyoracle.py:flops_per_step: 2.44e5. tests/test_bench_constants.py re-counts a sample and pins this figure.
 FLOPS_PER_STEP = 2.44e5
-# HBM bytes per launch of run_chunk<false> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a plain
-# copy kernel: profiles/run_chunk_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 90.5e6
+# HBM bytes per launch of run_batch<false, false, 256> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
+# plain copy kernel: profiles/run_batch_fp64_titanic64_batch1024.md).
+TRAFFIC_BYTES_PER_LAUNCH = 233.8e6
 
 
 def parse():
@@ -419,7 +419,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH if (args.count == 1024 and not mixed) else None,
-                     "kernel": "run_chunk<false>" if not mixed else "run_chunk<true>", "kernel_ms": round(per_launch_ms, 3)},
+                     "kernel": "run_batch<false>" if not mixed else "run_batch<true>", "kernel_ms": round(per_launch_ms, 3)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample)

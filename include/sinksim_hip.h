@@ -172,6 +172,14 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
  * the kernel refill them. SS_LOGIC after run. */
 ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgroups);
 
+/* The oracle's stopWhenStable rule on the device (RunOptions::stopWhenStable, engine/include/sinksim/simulation.hpp:34;
+ * engine/src/simulation.cpp:179-182): past 30 minutes an instance accumulates calm time (dt per step) while the
+ * step's sea inflow stays under 0.02, the pitch rate under 1e-5 and the heave velocity under 1e-3 in magnitude,
+ * resets it otherwise, and stops once it exceeds 20 minutes. The reference's GPU batch refuses such sweeps
+ * (apps/sinksim_cuda_run.cpp:28; its sweep.hpp:41 says CPU only); here every instance applies it when on is 1.
+ * SS_LOGIC after run. */
+ss_status ss_batch_set_stop_when_stable(ss_batch* b, int32_t on);
+
 /* No counterpart in the reference: what ss_batch_autotune measured and chose. rate[k]: simulation steps per second
  * of the trial at 256 (k = 0) and 512 (k = 1) threads per simulation (0 if that shape does not fit). */
 typedef struct ss_tune_result {

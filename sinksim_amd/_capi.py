@@ -105,6 +105,7 @@ SIGNATURES = {
                                     _dp, _dp, C.POINTER(C.c_int64), _ip, _dp]),
     "ss_batch_adjoint_step": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "ss_batch_gradient_launch": (C.c_int, [C.c_void_p, _ip, _ip, C.POINTER(C.c_int64)]),
+    "ss_batch_set_stop_when_stable": (C.c_int, [C.c_void_p, C.c_int32]),
 }
 
 _lib = None

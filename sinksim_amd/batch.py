@@ -158,6 +158,11 @@ class Batch:
         resident at once; fewer makes the kernel refill them)."""
         A.check(self._lib.ss_batch_set_launch(self._h, int(threads_per_sim), int(workgroups)))
 
+    def set_stop_when_stable(self, on: bool = True) -> None:
+        """The oracle's stopWhenStable rule (engine/src/simulation.cpp:179-182) for every instance: past 30 min, stop
+        after 20 min of calm (small inflow, pitch rate and heave velocity). The reference's GPU batch refuses it."""
+        A.check(self._lib.ss_batch_set_stop_when_stable(self._h, 1 if on else 0))
+
     def set_workgroups(self, n: int) -> None:
         self.set_launch(0, n)
 

@@ -66,7 +66,7 @@ def build_tools(force: bool = True) -> list[str]:
     deps = HOST_SOURCES + [os.path.join(HOST, f) for f in ("json.hpp", "model.hpp", "sweep.hpp", "batch.hpp")] + [
         os.path.join(APPS, "cli.hpp"), os.path.join(ROOT, "include", "sinksim_hip.h")]
     out = []
-    for app in ("sinksim_hip_run", "sinksim_sweep"):
+    for app in ("sinksim_hip_run", "sinksim_sweep", "sinksim_hip_validate"):
         exe = os.path.join(BIN, app)
         src = os.path.join(APPS, app + ".cpp")
         if not force and os.path.exists(exe) and all(os.path.getmtime(exe) >= os.path.getmtime(d) for d in deps + [src, LIB]):

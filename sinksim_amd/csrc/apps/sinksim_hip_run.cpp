@@ -29,7 +29,7 @@ sweep::Readouts to_readouts(const hip::CurveSample& c, Real rho) {
 
 int run_sweep(const cli::Args& args, std::shared_ptr<const CompiledShip> ship, const CompiledSim& sim, bool mixed, const hip::DeviceInfo& info) {
   const sweep::Sweep sw = sweep::load_sweep(args.get("sweep", ""), sim);
-  if (sw.stopWhenStable) throw std::runtime_error("stopWhenStable is not supported on the GPU; run the sweep with the CPU reference engine");
+  // stopWhenStable runs on the device here (the reference's sinksim_cuda_run refuses it, apps/sinksim_cuda_run.cpp:28)
   if (sw.instances.empty()) throw std::runtime_error("the sweep has no instances");
   hip::BatchOptions opt;
   opt.count = static_cast<Index>(sw.instances.size());
@@ -41,6 +41,7 @@ int run_sweep(const cli::Args& args, std::shared_ptr<const CompiledShip> ship, c
   std::printf("sweep of %zu instances on %s / %s, numerics %s, tMax %s, curve every %g s\n", sw.instances.size(), ship->id.c_str(), sim.id.c_str(),
               mixed ? "portable32" : "portable", cli::minutes(sw.tMax).c_str(), sw.every);
   hip::Batch batch(ship, sim, opt);
+  batch.set_stop_when_stable(sw.stopWhenStable);
   // the instances' connection tables, built on the host and handed over in one call
   const size_t nc = static_cast<size_t>(sim.connections()), count = sw.instances.size();
   std::vector<Byte> en(count * nc);

@@ -52,7 +52,7 @@ int main(int argc, char** argv) {
     const CompiledSim sim = io::load_sim(args.positional[1]);
     io::check_pairing(*ship, sim);
     const sweep::Sweep sw = sweep::load_sweep(args.positional[2], sim);
-    if (sw.stopWhenStable) throw std::runtime_error("stopWhenStable is not supported on the GPU; run the sweep with the CPU reference engine");
+    // stopWhenStable (the oracle's calm rule) runs on the device
     if (sw.instances.empty()) throw std::runtime_error("the sweep has no instances");
     std::printf("sweep of %zu instances on %s / %s, numerics %s, tMax %s, curve every %g s, on the GPU\n", sw.instances.size(), ship->id.c_str(),
                 sim.id.c_str(), mixed ? "portable32" : "portable", cli::minutes(sw.tMax).c_str(), sw.every);
@@ -64,6 +64,7 @@ int main(int argc, char** argv) {
     opt.tMax = sw.tMax;
     opt.curveEvery = sw.every;
     hip::Batch batch(ship, sim, opt);
+    batch.set_stop_when_stable(sw.stopWhenStable);
     for (Index i = 0; i < opt.count; ++i)
       sweep::apply(sw.instances[static_cast<size_t>(i)], sim, [&](Index c, bool en) { batch.set_connection_enabled(i, c, en); },
                    [&](Index c, Real a) { batch.set_connection_area(i, c, a); }, [&](Index c, Real t) { batch.set_connection_activation(i, c, t); });

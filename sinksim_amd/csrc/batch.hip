@@ -68,6 +68,7 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     B.scal[S_FOUNDERED] = Bt.foundered[inst];
     B.scal[S_FOUNDERT] = Bt.founderT[inst];
     B.scal[S_SEA] = 0;
+    B.scal[S_CALM] = Bt.stopWhenStable ? Bt.calm[inst] : 0.0;
     if (Bt.curveEvery > 0) {
       B.scal[S_NEXTCURVE] = Bt.nextCurve[inst];
       B.scal[S_CURVECOUNT] = Bt.curveCount[inst];
@@ -124,7 +125,8 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
   for (long long s = 0; s < maxSteps; ++s) {
     PC& Q = model_at(pa);
     const Blk Bq = carve(lds, Q);
-    if (Bq.scal[S_FOUNDERED] != 0 || Bq.scal[S_T] >= tMax) break;  // uniform: LDS values after a barrier
+    // uniform: LDS values after a barrier
+    if (Bq.scal[S_FOUNDERED] != 0 || Bq.scal[S_T] >= tMax || Bq.scal[S_CALM] > kCalmStop) break;
     // a readout-curve record when due (simulation.cpp:175-176). Every wave evaluates the condition on the same
     // LDS values; the barrier inside orders all of those reads (and the record's reads of the volumes) before
     // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
@@ -142,6 +144,15 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     }
     device_step<Mixed, TPB>(pa, lds, tid);
     ++done;
+    if (Bt.stopWhenStable) {
+      // simulation.cpp:179-182: past 30 minutes, calm accumulates while the inflow, the pitch rate and the heave
+      // velocity after the step stay small, and resets otherwise; the loop ends once it exceeds 20 minutes
+      if (tid == 0 && Bq.scal[S_T] > kCalmStart) {
+        const bool still = __builtin_fabs(Bq.scal[O_INFLOW]) < 0.02 && __builtin_fabs(Bq.scal[S_WTH]) < 1e-5 && __builtin_fabs(Bq.scal[S_VZ]) < 1e-3;
+        Bq.scal[S_CALM] = still ? Bq.scal[S_CALM] + Q.M.dt : 0.0;
+      }
+      __syncthreads();
+    }
     if (traced) {
       if (tid == 0) {
         const size_t rs = static_cast<size_t>(Bt.recSize);
@@ -169,6 +180,7 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     Bt.foundered[inst] = static_cast<uint8_t>(B.scal[S_FOUNDERED]);
     Bt.founderT[inst] = B.scal[S_FOUNDERT];
     Bt.steps[inst] = done;
+    if (Bt.stopWhenStable) Bt.calm[inst] = B.scal[S_CALM];
     if (Bt.curveEvery > 0) {
       Bt.nextCurve[inst] = B.scal[S_NEXTCURVE];
       Bt.curveCount[inst] = static_cast<int32_t>(B.scal[S_CURVECOUNT]);
@@ -608,6 +620,7 @@ void ssa::upload(ss_batch* b) {
   b->overT.fill(-1.0, b->monTotal);
   b->markT.fill(-1.0, b->markTotal);
   b->foundered.fill(0, count); b->founderT.fill(-1.0, count); b->steps.fill(0, count); b->final7.fill(0.0, count * 7);
+  b->calm.fill(0.0, b->stopWhenStable ? count : 1);
   const size_t traced = static_cast<size_t>(b->opt.traced), rs = static_cast<size_t>(b->recSize);
   b->denseRec.alloc(traced * static_cast<size_t>(b->opt.denseSteps + 1) * rs);
   b->sampleRec.alloc(traced * static_cast<size_t>(b->maxSamples) * rs);
@@ -698,7 +711,7 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
   b->kernelMs = 0;
   b->launches = 0;
   std::vector<uint8_t> f(count);
-  std::vector<double> st(count * 7);
+  std::vector<double> st(count * 7), calm(count, 0.0);
   for (;;) {
     hip_check(hipEventRecord(e0, stream), "hipEventRecord");
     hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
@@ -711,9 +724,10 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
     if (maxSteps == LLONG_MAX || b->launches >= maxLaunches) break;
     b->foundered.download(f.data(), count);
     b->st.download(st.data(), count * 7);
+    if (b->stopWhenStable) b->calm.download(calm.data(), count);
     bool active = false;
     for (size_t i = 0; i < count; ++i)
-      if (!f[i] && st[i * 7] < b->opt.tMax) { active = true; break; }
+      if (!f[i] && st[i * 7] < b->opt.tMax && !(calm[i] > kCalmStop)) { active = true; break; }
     if (!active) break;
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -733,6 +747,15 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
     need(workgroups >= 0, "ss_batch_set_launch: negative workgroup count");
     b->tpb = threadsPerSim == 0 ? kThreads : threadsPerSim;
     b->blocks = workgroups;
+  });
+}
+
+ss_status ss_batch_set_stop_when_stable(ss_batch* b, int32_t on) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_set_stop_when_stable: null batch");
+    if (b->ran) fail(SS_LOGIC, "ss_batch_set_stop_when_stable: set after run");
+    need(on == 0 || on == 1, "ss_batch_set_stop_when_stable: on must be 0 or 1");
+    b->stopWhenStable = on == 1;
   });
 }
 

@@ -191,6 +191,8 @@ struct ss_batch {
   bool ran = false;
   double kernelMs = 0;
   int launches = 0;
+  bool stopWhenStable = false;  // the oracle's calm rule ends instances that settle (simulation.cpp:179-182)
+  Dev<double> calm;             // per instance, with stopWhenStable
   // the last gradient call's launch shape: threads per simulation, the tape's large arrays in global memory, LDS bytes
   int32_t adjTpb = 0, adjGlobal = 0;
   size_t adjLds = 0;
@@ -232,6 +234,7 @@ struct ss_batch {
     b.nextSample = W(nextSample.p);
     b.curveEvery = opt.curveEvery; b.curveMax = curveMax; b.curveRec = W(curveRec.p); b.curveCount = W(curveCount.p);
     b.nextCurve = W(nextCurve.p);
+    b.stopWhenStable = stopWhenStable ? 1 : 0; b.calm = W(calm.p);
     return b;
   }
 };

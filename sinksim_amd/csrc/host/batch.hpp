@@ -114,6 +114,8 @@ class Batch {
   void set_connection_area(Index i, Index c, Real area) { check(ss_batch_set_connection_area(h_.get(), i, c, area)); }
   void set_connection_activation(Index i, Index c, Real tOn) { check(ss_batch_set_connection_activation(h_.get(), i, c, tOn)); }
   void set_connections(const Byte* en, const Real* area, const Real* tOn) { check(ss_batch_set_connections(h_.get(), en, area, tOn)); }
+  // the oracle's stopWhenStable rule on the device (no counterpart in the reference's GPU batch)
+  void set_stop_when_stable(bool on) { check(ss_batch_set_stop_when_stable(h_.get(), on ? 1 : 0)); }
 
   double run() {
     double s = 0;

@@ -49,6 +49,7 @@ constexpr double kSlopeMin = 1e-9;
 constexpr double kCentroidMinVolume = 1e-9;
 constexpr double kNoHead = -1e9;
 constexpr double kNoHeadTest = -1e8;
+constexpr double kCalmStart = 1800, kCalmStop = 1200;  // stopWhenStable: calm counts after 30 min, stops past 20 min (simulation.cpp:179-182)
 constexpr double kPitchClamp = 1.35;
 constexpr double kRollClamp = 1.4;
 constexpr double kPi = 3.141592653589793;
@@ -143,6 +144,8 @@ struct DevBatch {
   SSG double* curveRec;
   SSG int32_t* curveCount;
   SSG double* nextCurve;
+  int32_t stopWhenStable;  // stop an instance after 20 min of calm, once past 30 min (simulation.cpp:179-182)
+  SSG double* calm;        // [count] the calm time so far (s), kept across launches; > 1200 once stopped
 };
 
 // LDS layout of one simulation, as offsets in doubles (the host computes it: lds_layout in batch.hip).
@@ -171,7 +174,7 @@ enum : int { F_R00 = 0, F_R01, F_R02, F_R10, F_R11, F_R12, F_R20, F_R21, F_R22, 
 
 // scalar slots of the `scal` block
 enum : int { S_T = 0, S_ZO, S_PITCH, S_ROLL, S_VZ, S_WTH, S_WPH, O_INFLOW, O_VB, O_TOP, O_MW, O_BX, O_BY, O_BZ, S_SEA, S_FOUNDERT, S_EVCOUNT,
-             S_FOUNDERED, S_NEXTCURVE, S_CURVECOUNT, S_COUNT };
+             S_FOUNDERED, S_NEXTCURVE, S_CURVECOUNT, S_CALM, S_COUNT };
 
 struct Blk {
   SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;

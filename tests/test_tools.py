@@ -103,3 +103,10 @@ def test_oracle_sweep_equals_reference_sweep(titanic, tmp_path, numerics):
     d = S.compare(ref, mine)
     assert d.differing == 0, d.first
     assert d.compared > 1000
+
+
+def test_validate_tool_usage_and_numerics():
+    """sinksim_hip_validate without arguments exits 2; the CPU-only numerics are refused with exit 2."""
+    assert _run(_tool("sinksim_hip_validate")).returncode == 2
+    from sinksim_amd.data_paths import CATALOG
+    assert _run(_tool("sinksim_hip_validate"), CATALOG, "--numerics", "oracle").returncode == 2

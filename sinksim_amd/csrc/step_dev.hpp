@@ -286,6 +286,45 @@ __device__ __forceinline__ double lead(double v) {
 }
 __device__ __forceinline__ float lead(float v) { return __builtin_bit_cast(float, lead32(__builtin_bit_cast(uint32_t, v))); }
 
+// Lane 16 of each logical 32-lane group (wave lanes 16 and 48) to every lane of its group: source lane
+// ((l & 0) | 16) ^ 0 = 16 within each group of 32.
+__device__ __forceinline__ uint32_t lead16_32(uint32_t v) { return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x0200)); }
+__device__ __forceinline__ double lead16(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(lead16_32(static_cast<uint32_t>(b >> 32))) << 32) | lead16_32(static_cast<uint32_t>(b)));
+}
+__device__ __forceinline__ float lead16(float v) { return __builtin_bit_cast(float, lead16_32(__builtin_bit_cast(uint32_t, v))); }
+
+// Two values of a logical 32-lane group, a and b, after the first level of the halving tree (l absorbs l + 16)
+// packed into one register: lanes 0..15 of the group end with a's sums and lanes 16..31 with b's. The gfx950
+// v_permlane16_swap exchanges rows 1 and 3 (16 lanes each) of its first operand with rows 0 and 2 of its second:
+// afterwards the first holds a[l] in lanes 0..15 and b[l - 16] in lanes 16..31, the second a[l + 16] and b[l], so
+// first + second is the tree's own sum with its own operand order in both halves. The later levels (+8 ... +1) stay
+// inside a row of 16 lanes and work on the packed register unchanged; a's total ends in lane 0, b's in lane 16.
+__device__ __forceinline__ void swap16(uint32_t& x, uint32_t& y) {
+  const auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+  x = r[0]; y = r[1];
+}
+__device__ __forceinline__ void swap16(double& a, double& b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  uint32_t alo = static_cast<uint32_t>(ua), ahi = static_cast<uint32_t>(ua >> 32);
+  uint32_t blo = static_cast<uint32_t>(ub), bhi = static_cast<uint32_t>(ub >> 32);
+  swap16(alo, blo); swap16(ahi, bhi);
+  a = __builtin_bit_cast(double, (static_cast<uint64_t>(ahi) << 32) | alo);
+  b = __builtin_bit_cast(double, (static_cast<uint64_t>(bhi) << 32) | blo);
+}
+__device__ __forceinline__ void swap16(float& a, float& b) {
+  uint32_t x = __builtin_bit_cast(uint32_t, a), y = __builtin_bit_cast(uint32_t, b);
+  swap16(x, y);
+  a = __builtin_bit_cast(float, x); b = __builtin_bit_cast(float, y);
+}
+// first level of two sum trees, packed
+template <class T>
+__device__ __forceinline__ T pack_sum16(T a, T b) { swap16(a, b); return a + b; }
+// first level of two minimum trees (`if (other < own) own = other`), packed; own is the first after the swap
+template <class T>
+__device__ __forceinline__ T pack_min16(T a, T b) { swap16(a, b); return b < a ? b : a; }
+
 // ------------------------------------------------------------------------------------------- partials
 
 template <class T>
@@ -297,34 +336,51 @@ struct PP {
   T vol, dv, mx, my, mz, zmin, zmax;
 };
 
-// One level of the halving tree. Lanes l < OFF absorb lane l + OFF; the other lanes are never read again, so they
-// add too (their sums are discarded) and no lane mask is needed for the sums.
+// One level of the halving tree (reduce.hpp:102-107: l absorbs l + 16, 8, 4, 2, 1; the total in lane 0 of the
+// group). Lanes l < OFF absorb lane l + OFF; the other lanes are never read again, so they add too (their sums are
+// discarded) and no lane mask is needed.
 template <int OFF, class T>
-__device__ __forceinline__ void tree_level(HP<T>& p) {
-  const T V = down<OFF>(p.V), Mx = down<OFF>(p.Mx), My = down<OFF>(p.My), Mz = down<OFF>(p.Mz), top = down<OFF>(p.top);
-  p.V += V; p.Mx += Mx; p.My += My; p.Mz += Mz;
-  if (top > p.top) p.top = top;
-}
-template <int OFF, bool Ext, class T>
-__device__ __forceinline__ void tree_level(PP<T>& p) {
-  const T vol = down<OFF>(p.vol), dv = down<OFF>(p.dv), mx = down<OFF>(p.mx), my = down<OFF>(p.my), mz = down<OFF>(p.mz);
-  p.vol += vol; p.dv += dv; p.mx += mx; p.my += my; p.mz += mz;
-  if constexpr (Ext) {
-    const T zmin = down<OFF>(p.zmin), zmax = down<OFF>(p.zmax);
-    if (zmin < p.zmin) p.zmin = zmin;
-    if (zmax > p.zmax) p.zmax = zmax;
-  }
+__device__ __forceinline__ void sum_level(T& v) { v += down<OFF>(v); }
+template <int OFF, class T>
+__device__ __forceinline__ void min_level(T& v) {
+  const T o = down<OFF>(v);
+  if (o < v) v = o;
 }
 
-// The halving tree of a logical 32-lane group (reduce.hpp:102-107): l absorbs l + 16, 8, 4, 2, 1; the total is in
-// lane 0 of the group.
-template <class T>
-__device__ __forceinline__ void tree32(HP<T>& p) {
-  tree_level<16>(p); tree_level<8>(p); tree_level<4>(p); tree_level<2>(p); tree_level<1>(p);
-}
+// The node pass's halving trees (the reference's sums and comparisons, operand for operand), with (vol, dv) and
+// (mx, my) packed in pairs, and with Ext (zmin, -zmax) as a pair of minimum trees: -max(-a, -b) is max(a, b)
+// bit for bit, ties and NaN keeping the lane's own value in both. The totals go to every lane of the group.
 template <bool Ext, class T>
-__device__ __forceinline__ void tree32(PP<T>& p) {
-  tree_level<16, Ext>(p); tree_level<8, Ext>(p); tree_level<4, Ext>(p); tree_level<2, Ext>(p); tree_level<1, Ext>(p);
+__device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
+  T vd = pack_sum16(p.vol, p.dv), xy = pack_sum16(p.mx, p.my), z = p.mz;
+  sum_level<16>(z);
+  T mm = 0;
+  if constexpr (Ext) mm = pack_min16(p.zmin, -p.zmax);
+  sum_level<8>(vd); sum_level<8>(xy); sum_level<8>(z);
+  if constexpr (Ext) min_level<8>(mm);
+  sum_level<4>(vd); sum_level<4>(xy); sum_level<4>(z);
+  if constexpr (Ext) min_level<4>(mm);
+  sum_level<2>(vd); sum_level<2>(xy); sum_level<2>(z);
+  if constexpr (Ext) min_level<2>(mm);
+  sum_level<1>(vd); sum_level<1>(xy); sum_level<1>(z);
+  if constexpr (Ext) min_level<1>(mm);
+  PP<T> r;
+  r.vol = lead(vd); r.dv = lead16(vd); r.mx = lead(xy); r.my = lead16(xy); r.mz = lead(z);
+  r.zmin = 0; r.zmax = 0;
+  if constexpr (Ext) { r.zmin = lead(mm); r.zmax = -lead16(mm); }
+  return r;
+}
+
+// The buoyancy's halving trees (sums, and `if (other > own) own = other` for top), with (V, Mx) and (My, Mz)
+// packed: V, My and top end in lane 0 of the group, Mx and Mz in lane 16.
+template <class T>
+__device__ __forceinline__ void tree32_packed(const HP<T>& p, T& vx, T& yz, T& top) {
+  vx = pack_sum16(p.V, p.Mx); yz = pack_sum16(p.My, p.Mz); top = p.top;
+  T o = down<16>(top);
+  if (o > top) top = o;
+#define SS_HP_LEVEL(OFF) sum_level<OFF>(vx); sum_level<OFF>(yz); o = down<OFF>(top); if (o > top) top = o;
+  SS_HP_LEVEL(8) SS_HP_LEVEL(4) SS_HP_LEVEL(2) SS_HP_LEVEL(1)
+#undef SS_HP_LEVEL
 }
 
 struct Fr64 {
@@ -347,16 +403,21 @@ template <class T>
 struct Geo {
   T x, y, lo, hi, A;
 };
-__device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) {
-  const SSG d2v* r = (const SSG d2v*)((const SSG char*)table + static_cast<uint32_t>(i) * 48u);
+// record sizes: five doubles padded to 48 bytes, five floats padded to 32
+template <class T>
+constexpr uint32_t kRecBytes = sizeof(T) == sizeof(double) ? 48u : 32u;
+__device__ __forceinline__ Geo<double> geo_at(const SSG d2v* table, uint32_t off) {
+  const SSG d2v* r = (const SSG d2v*)((const SSG char*)table + off);
   const d2v a = r[0], b = r[1], c = r[2];
   return Geo<double>{a.x, a.y, b.x, b.y, c.x};
 }
-__device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) {
-  const SSG f4v* r = (const SSG f4v*)((const SSG char*)table + static_cast<uint32_t>(i) * 32u);
+__device__ __forceinline__ Geo<float> geo_at(const SSG f4v* table, uint32_t off) {
+  const SSG f4v* r = (const SSG f4v*)((const SSG char*)table + off);
   const f4v a = r[0], b = r[1];
   return Geo<float>{a.x, a.y, a.z, a.w, b.x};
 }
+__device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<double>); }
+__device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<float>); }
 
 // one column into a buoyancy partial (reduce.hpp:67-77 / 159-170)
 __device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, HP<double>& p) {
@@ -431,20 +492,31 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
 }
 
 // A lane's partial over the records s0 + lane, s0 + lane + step, ... below s1, in order, each record's load
-// issued while the previous record is evaluated.
+// issued while the previous record is evaluated. Unrolled by two so that the two records in flight keep their
+// registers (no copies between iterations); records are addressed by byte offsets advanced by step records (no
+// multiplication per record). A lane past its last record reloads its current one (in bounds) and stops.
 template <class T, class Tab, class Fn>
 __device__ __forceinline__ void stream(const Tab* table, int first, int end, int step, Fn&& eval) {
   int j = first;
   if (j >= end) return;
-  Geo<T> g = geo(table, j);
+  constexpr uint32_t rec = kRecBytes<T>;
+  const uint32_t stride = static_cast<uint32_t>(step) * rec;
+  uint32_t off = static_cast<uint32_t>(j) * rec;
+  Geo<T> g0 = geo_at(table, off);
   while (true) {
-    const int jn = j + step;
-    const bool more = jn < end;
-    const Geo<T> gn = geo(table, more ? jn : j);
-    eval(g);
-    if (!more) break;
-    g = gn;
-    j = jn;
+    const int j1 = j + step;
+    const bool m1 = j1 < end;
+    const uint32_t off1 = m1 ? off + stride : off;
+    const Geo<T> g1 = geo_at(table, off1);
+    eval(g0);
+    if (!m1) break;
+    const int j2 = j1 + step;
+    const bool m2 = j2 < end;
+    off = m2 ? off1 + stride : off1;
+    g0 = geo_at(table, off);
+    eval(g1);
+    if (!m2) break;
+    j = j2;
   }
 }
 
@@ -470,17 +542,17 @@ __device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, 
     const float hf = static_cast<float>(h);
     PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
     stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g) { segment<Ext>(f, hf, g, p); });
-    tree32<Ext>(p);
-    vol = static_cast<double>(lead(p.vol)); dv = static_cast<double>(lead(p.dv));
-    mx = static_cast<double>(lead(p.mx)); my = static_cast<double>(lead(p.my)); mz = static_cast<double>(lead(p.mz));
-    if constexpr (Ext) { zmin = static_cast<double>(lead(p.zmin)); zmax = static_cast<double>(lead(p.zmax)); }
+    const PP<float> t = tree32_all<Ext>(p);
+    vol = static_cast<double>(t.vol); dv = static_cast<double>(t.dv);
+    mx = static_cast<double>(t.mx); my = static_cast<double>(t.my); mz = static_cast<double>(t.mz);
+    if constexpr (Ext) { zmin = static_cast<double>(t.zmin); zmax = static_cast<double>(t.zmax); }
   } else {
     const Fr64 F = fr64(fr);
     PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
     stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g) { segment<Ext>(F, h, g, p); });
-    tree32<Ext>(p);
-    vol = lead(p.vol); dv = lead(p.dv); mx = lead(p.mx); my = lead(p.my); mz = lead(p.mz);
-    if constexpr (Ext) { zmin = lead(p.zmin); zmax = lead(p.zmax); }
+    const PP<double> t = tree32_all<Ext>(p);
+    vol = t.vol; dv = t.dv; mx = t.mx; my = t.my; mz = t.mz;
+    if constexpr (Ext) { zmin = t.zmin; zmax = t.zmax; }
   }
   const double mu = B.nMu[n >= 0 ? n : 0];
   const double iR = fr[F_IR];
@@ -966,7 +1038,7 @@ __device__ inline void push_event(PC& P, const Blk& B, double t, int id, int kin
 
 // ------------------------------------------------------------------------------------------- the step
 
-// Buoyancy partials of this thread's 32-lane group; the lead lane stores the group's tree result.
+// Buoyancy partials of this thread's 32-lane group; lanes 0 and 16 store the group's tree results.
 template <bool Mixed>
 __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
@@ -975,14 +1047,18 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
     const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
     stream<float>(P.S.colF, tid, P.S.ncol, kThreads, [&](const Geo<float>& g) { column(f, g, p); });
-    tree32(p);
-    if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
+    float vx, yz, top;
+    tree32_packed(p, vx, yz, top);
+    if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }
+    if (lane == 16) { w[1] = vx; w[3] = yz; }
   } else {
     const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
     stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g) { column(F, g, p); });
-    tree32(p);
-    if (lane == 0) { w[0] = p.V; w[1] = p.Mx; w[2] = p.My; w[3] = p.Mz; w[4] = p.top; }
+    double vx, yz, top;
+    tree32_packed(p, vx, yz, top);
+    if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }
+    if (lane == 16) { w[1] = vx; w[3] = yz; }
   }
 }
 

@@ -157,8 +157,9 @@ __device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double 
 
 constexpr int kMaxTPB = 512;
 
+// The forward-only instantiation (FWD) is held to 128 VGPRs, so that two 512-thread workgroups share a compute unit.
 template <int kTPB, bool GT, bool FWD>
-__global__ __launch_bounds__(kTPB) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
+__global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FWD ? 4 : 1, 8))) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                      const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
   DC& D = *(DC*)adj;

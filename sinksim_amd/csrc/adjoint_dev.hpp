@@ -164,22 +164,13 @@ struct PassRec {
   double xmn, ymn, amn, xmx, ymx, bmx;
 };
 
+// A minimum tree that carries three coordinates with the extreme value: `if (other < own)` the lane takes the
+// other's value and coordinates (ties and NaN keep its own).
 template <int OFF>
-__device__ __forceinline__ void pass_level(PassRec& p) {
-  const double vol = down<OFF>(p.vol), dv = down<OFF>(p.dv), mx = down<OFF>(p.mx), my = down<OFF>(p.my), mz = down<OFF>(p.mz);
-  const double zmin = down<OFF>(p.zmin), zmax = down<OFF>(p.zmax);
-  const double xmn = down<OFF>(p.xmn), ymn = down<OFF>(p.ymn), amn = down<OFF>(p.amn);
-  const double xmx = down<OFF>(p.xmx), ymx = down<OFF>(p.ymx), bmx = down<OFF>(p.bmx);
-  double S[10];
-#pragma unroll
-  for (int k = 0; k < 10; ++k) S[k] = down<OFF>(p.S[k]);
-  p.vol += vol; p.dv += dv; p.mx += mx; p.my += my; p.mz += mz;
-  if (zmin < p.zmin) { p.zmin = zmin; p.xmn = xmn; p.ymn = ymn; p.amn = amn; }
-  if (zmax > p.zmax) { p.zmax = zmax; p.xmx = xmx; p.ymx = ymx; p.bmx = bmx; }
-#pragma unroll
-  for (int k = 0; k < 10; ++k) p.S[k] += S[k];
+__device__ __forceinline__ void argmin_level(double& z, double& u, double& v, double& w) {
+  const double oz = down<OFF>(z), ou = down<OFF>(u), ov = down<OFF>(v), ow = down<OFF>(w);
+  if (oz < z) { z = oz; u = ou; v = ov; w = ow; }
 }
-
 // The pass of node n at level h by a 32-lane unit (hydro.hpp:306-316 values, in the canonical tree), with its
 // moment sums; the record is in every lane of the unit.
 __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, int lane, int n, double h) {
@@ -190,15 +181,14 @@ __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, in
 #pragma unroll
   for (int k = 0; k < 10; ++k) p.S[k] = 0;
   const int s0 = B.nSegStart[n], s1 = s0 + B.nSegCount[n];
-  for (int j = s0 + lane; j < s1; j += kLanes) {
-    const Geo<double> g = geo(P.S.seg, j);
+  stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g) {
     const double x = g.x, y = g.y, a = g.lo, b = g.hi;
     const double base = R20 * x + R21 * y + tz;
     const double wa = base + R22 * a, wb = base + R22 * b;
     if (wa < p.zmin) { p.zmin = wa; p.xmn = x; p.ymn = y; p.amn = a; }
     if (wb > p.zmax) { p.zmax = wb; p.xmx = x; p.ymx = y; p.bmx = b; }
     const double zp = (h - base) * iR;
-    if (zp <= a) continue;
+    if (zp <= a) return;
     const double A = g.A;
     double L;
     if (zp >= b) {
@@ -212,14 +202,37 @@ __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, in
     }
     const double v = A * L;
     p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
-  }
-  pass_level<16>(p); pass_level<8>(p); pass_level<4>(p); pass_level<2>(p); pass_level<1>(p);
-  PassRec r;
-  r.vol = lead(p.vol); r.dv = lead(p.dv); r.mx = lead(p.mx); r.my = lead(p.my); r.mz = lead(p.mz);
-  r.zmin = lead(p.zmin); r.zmax = lead(p.zmax);
+  });
+  // The halving tree (l absorbs l + 16, 8, 4, 2, 1) of the 15 sums and of the extent with its coordinates, two
+  // values packed per register (pack_sum16 / pack_min16): the sums are the forward pass's to the bit, and the
+  // extent's maximum runs as a minimum over -zmax (-max(-a, -b) is max(a, b), ties and NaN keeping the own lane).
+  double q[8];
+  q[0] = pack_sum16(p.vol, p.dv); q[1] = pack_sum16(p.mx, p.my); q[2] = pack_sum16(p.mz, p.S[0]);
+  q[3] = pack_sum16(p.S[1], p.S[2]); q[4] = pack_sum16(p.S[3], p.S[4]); q[5] = pack_sum16(p.S[5], p.S[6]);
+  q[6] = pack_sum16(p.S[7], p.S[8]); q[7] = p.S[9];
+  sum_level<16>(q[7]);
+  double ez = -p.zmax, ex = p.xmx, ey = p.ymx, ea = p.bmx;
+  double mz = p.zmin, mxx = p.xmn, myy = p.ymn, maa = p.amn;
+  swap16(mz, ez); swap16(mxx, ex); swap16(myy, ey); swap16(maa, ea);
+  if (ez < mz) { mz = ez; mxx = ex; myy = ey; maa = ea; }  // own is the first after the swap
 #pragma unroll
-  for (int k = 0; k < 10; ++k) r.S[k] = lead(p.S[k]);
-  r.xmn = lead(p.xmn); r.ymn = lead(p.ymn); r.amn = lead(p.amn); r.xmx = lead(p.xmx); r.ymx = lead(p.ymx); r.bmx = lead(p.bmx);
+  for (int k = 0; k < 8; ++k) sum_level<8>(q[k]);
+  argmin_level<8>(mz, mxx, myy, maa);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sum_level<4>(q[k]);
+  argmin_level<4>(mz, mxx, myy, maa);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sum_level<2>(q[k]);
+  argmin_level<2>(mz, mxx, myy, maa);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sum_level<1>(q[k]);
+  argmin_level<1>(mz, mxx, myy, maa);
+  PassRec r;
+  r.vol = lead(q[0]); r.dv = lead16(q[0]); r.mx = lead(q[1]); r.my = lead16(q[1]); r.mz = lead(q[2]);
+  r.S[0] = lead16(q[2]); r.S[1] = lead(q[3]); r.S[2] = lead16(q[3]); r.S[3] = lead(q[4]); r.S[4] = lead16(q[4]);
+  r.S[5] = lead(q[5]); r.S[6] = lead16(q[5]); r.S[7] = lead(q[6]); r.S[8] = lead16(q[6]); r.S[9] = lead(q[7]);
+  r.zmin = lead(mz); r.xmn = lead(mxx); r.ymn = lead(myy); r.amn = lead(maa);
+  r.zmax = -lead16(mz); r.xmx = lead16(mxx); r.ymx = lead16(myy); r.bmx = lead16(maa);
   const double mu = B.nMu[n];
   r.vol = r.vol * mu;
   r.dv = r.dv * mu * iR;

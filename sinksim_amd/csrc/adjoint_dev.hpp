@@ -646,7 +646,8 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
   __syncthreads();
   // (c) flows with their partial derivatives (flows.hpp:234-274): cp = d dV / d(ha, hb, zo, aEff_a, aEff_b, ln ca)
   const double t = T.sv[0];
-  for (int c = tid; c < NC; c += TPB) {
+  for (int k = tid; k < NC; k += TPB) {
+    const int c = P.M.flowOrder[k];  // weirs, then orifices (the forward's flow slots)
     const auto pd = T.cp + 6 * c;
     for (int k = 0; k < 6; ++k) pd[k] = 0;
     T.dV[c] = 0;
@@ -681,10 +682,10 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
         double fv = 1, dfr = 0, r = 0;
         if (H2 > 0) {
           r = H2 / H1;
-          const double pw = dpow(r, kWeirExponent);
+          const double pw = pow_weir(r);
           const double u = 1 - pw;
           const double gg = jmax(0.0, u);
-          fv = dpow(gg, kVillemonteExponent);
+          fv = pow_villemonte(gg);
           const double dpw = kWeirExponent * pw / r;
           const double dfg = gg != 0 ? kVillemonteExponent * fv / gg : 0.0;
           dfr = (u > 0 ? dfg : 0.0) * (-dpw);

@@ -416,6 +416,11 @@ std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
     conD[c] = make_double4(M.cx[c], M.cy[c], M.cz[c], M.coef[c]);
   }
   b.conI.upload(conI); b.conD.upload(conD);
+  std::vector<int32_t> flowOrder;  // weirs (law 1) first, then orifices, each in increasing connection order
+  for (int law = 1; law >= 0; --law)
+    for (size_t c = 0; c < ncn; ++c)
+      if ((M.law[c] != 0 ? 1 : 0) == law) flowOrder.push_back(static_cast<int32_t>(c));
+  b.flowOrder.upload(flowOrder);
   std::vector<int32_t> incCount(nn, 0), incStart(nn, 0), fill(nn, 0), sea;
   for (size_t c = 0; c < ncn; ++c) {
     if (M.ca[c] >= 0) incCount[M.ca[c]]++;
@@ -450,7 +455,7 @@ std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
   d.rho = M.rho; d.g = M.g; d.sq2g = std::sqrt(2 * M.g); d.dt = M.dt;
   d.ng = M.ngroup; d.nMembers = b.members; d.gStart = G(b.gStart.p); d.gCount = G(b.gCount.p); d.gNodes = G(b.gNodes.p); d.gMode = G(b.gMode.p);
   d.gpx = G(b.gpx.p); d.gpy = G(b.gpy.p); d.gpz = G(b.gpz.p); d.gMargin = M.gMargin;
-  d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p;
+  d.nc = M.nconn; d.conI = (const SSG i4v*)b.conI.p; d.conD = (const SSG d4v*)b.conD.p; d.flowOrder = G(b.flowOrder.p);
   d.incStart = G(b.incStart.p); d.incCount = G(b.incCount.p); d.inc = (const SSG i4v*)b.inc.p;
   d.seaCount = static_cast<int32_t>(sea.size()); d.sea = G(b.sea.p);
   d.nmon = b.nmon; d.monThreshold = M.monitorThreshold; d.monStart = G(b.monStart.p); d.monCount = G(b.monCount.p); d.monConn = G(b.monConn.p);

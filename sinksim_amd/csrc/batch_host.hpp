@@ -146,7 +146,7 @@ struct Model {
   Dev<float> colF, segF;                      // 8 floats per record
   Dev<int32_t> segStart, segCount;
   // scenario
-  Dev<int32_t> gStart, gCount, gNodes, incStart, incCount, sea, monStart, monCount, monConn;
+  Dev<int32_t> gStart, gCount, gNodes, incStart, incCount, sea, monStart, monCount, monConn, flowOrder;
   Dev<int4> inc, conI;
   Dev<double4> conD;
   Dev<uint8_t> gMode, mkWhen;

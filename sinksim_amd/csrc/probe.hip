@@ -13,7 +13,8 @@
 namespace {
 
 // op: 0 sin, 1 cos, 2 pow(a, b), 3 scalbn(a, (int)b), 4 a / b, 5 sqrt(a), 6 (double)((float)a / (float)b),
-//     7 (double)(float)a, 8 the step's pow call pair (dpow), 9 pose trig as the step calls it (trig(a, b != 0))
+//     7 (double)(float)a, 8 the weir powers as the step and the adjoint call them (b = 1.5: pow_weir(a), b = 0.385:
+//     pow_villemonte(a), else the Villemonte factor villemonte(a)), 9 pose trig as the step calls it (trig(a, b != 0))
 __global__ void probe_kernel(int op, const double* __restrict__ a, const double* __restrict__ b, double* __restrict__ out, long long n) {
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -28,7 +29,7 @@ __global__ void probe_kernel(int op, const double* __restrict__ a, const double*
     case 5: r = __builtin_sqrt(x); break;
     case 6: r = static_cast<double>(static_cast<float>(x) / static_cast<float>(y)); break;
     case 7: r = static_cast<double>(static_cast<float>(x)); break;
-    case 8: r = ssa::dpow(x, y); break;
+    case 8: r = y == 1.5 ? ssa::pow_weir(x) : (y == 0.385 ? ssa::pow_villemonte(x) : ssa::villemonte(x)); break;
     default: r = ssa::trig(x, y != 0 ? 1 : 0); break;
   }
   out[i] = r;

@@ -98,6 +98,7 @@ struct DevNet {
   const SSG i4v* inc;   // per incidence entry (node order, increasing connection): connection, its a, its b, side (1 = b)
   const SSG i4v* conI;  // per connection: a, b, skipGroup, law
   const SSG d4v* conD;  // per connection: x, y, z, coef
+  gci* flowOrder;       // the connection of each flow slot: weirs, then orifices, each in increasing order
   int32_t seaCount;
   gci* sea;
   int32_t nmon;
@@ -217,7 +218,6 @@ __device__ inline Blk carve(SSL double* base, PC& P) {
 
 // Out of line: called four lanes at a time once per step, and for the 30 s readouts.
 __device__ __noinline__ double trig(double x, int wantCos) { return fdl::sin_or_cos(x, wantCos); }
-__device__ __noinline__ double dpow(double x, double y) { return fdl::pow(x, y); }
 
 // reference: kernel/frame.hpp:10-21, written to the frame slots, with iR = 1 / R22 (hydro.hpp:209,245,264,309)
 // and the f32 frame of reduce.hpp:118-126
@@ -914,6 +914,16 @@ __device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
   B.deg[n] = d;
 }
 
+// The Villemonte factor of a submerged weir, (1 - r^1.5)^0.385 with the base clipped at 0 (flows.hpp:59): both
+// powers are fdlibm's, inlined into one call with their constant exponents folded in (same arithmetic, fewer
+// instructions and one call per weir instead of two).
+__device__ __noinline__ double villemonte(double r) {
+  return fdl::pow(jmax(0.0, 1 - fdl::pow(r, kWeirExponent)), kVillemonteExponent);
+}
+// the two powers one at a time (fdlibm's, exponent folded in), for callers that also need r^1.5
+__device__ __noinline__ double pow_weir(double r) { return fdl::pow(r, kWeirExponent); }
+__device__ __noinline__ double pow_villemonte(double g) { return fdl::pow(g, kVillemonteExponent); }
+
 // flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter. carea
 // holds the instance's products coef * area, the first multiplication of both flow laws, formed on the host.
 __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, const Blk& B, int c) {
@@ -937,7 +947,7 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
       double f = 1;
       if (H2 > 0) {
         const double r = H2 / H1;
-        f = dpow(jmax(0.0, 1 - dpow(r, kWeirExponent)), kVillemonteExponent);
+        f = villemonte(r);
       }
       q = B.carea[c] * kWeirCoefficient * sq2g * H1 * __builtin_sqrt(H1) * f * (Ha >= Hb ? 1.0 : -1.0);
     }
@@ -1226,7 +1236,12 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
     const double t = B.scal[S_T];
-    for (int c = tid; c < P.M.nc; c += TPB) B.dV[c] = connection_flow(P, B.frame, t, B, c);
+    // slots in flowOrder: a wave's lanes mostly follow one flow law, and the submerged weirs' powers run in the
+    // first waves only (each connection's flow depends on itself alone, so the order changes no value)
+    for (int k = tid; k < P.M.nc; k += TPB) {
+      const int c = P.M.flowOrder[k];
+      B.dV[c] = connection_flow(P, B.frame, t, B, c);
+    }
   }
   __syncthreads();
 

@@ -59,6 +59,11 @@ def test_fdlibm_against_oracle_on_the_kernel_domains():
     y = np.where(np.arange(r.size) % 2 == 0, 1.5, 0.385)
     p = _eval(8, r, y)
     assert _same(p, [pyoracle.fdl_pow(float(a), float(b)) for a, b in zip(r, y)]).all()
+    # the Villemonte factor (1 - r^1.5)^0.385, base clipped at 0, as one call with both exponents folded in
+    r = np.concatenate([rng.uniform(0, 1, 20000), [0.0, 1.0, 1e-300, 0.999999999]])
+    v = _eval(8, r, np.full(r.size, -1.0))
+    want = [pyoracle.fdl_pow(max(0.0, 1 - pyoracle.fdl_pow(float(a), 1.5)), 0.385) for a in r]
+    assert _same(v, want).all()
 
 
 def test_f64_division_and_sqrt_are_correctly_rounded():

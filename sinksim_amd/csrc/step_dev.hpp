@@ -14,10 +14,12 @@
 // compiled with -ffp-contract=off so that nothing is fused.
 //
 // Work layout of a step (| = workgroup barrier):
-//   buoyancy partials + the level solve, one logical 32-lane group per free-surface group | per node: effective
-//   heads and limiter degrees | per connection: flows | per node: gather, volume update, load terms; per monitor:
-//   flow sums; one lane: sea inflow | wave 0: the tail (loads as seven parallel chains, integration, events, and
-//   the pose of the next step) |
+//   buoyancy partials + the level solve, one logical 32-lane group per free-surface group | per node (a quad of
+//   lanes): effective heads and limiter degrees | per connection (weirs first): flows | per node: gather, volume
+//   update, load terms; per monitor: flow sums; one lane: sea inflow | wave 0: the tail (loads as seven parallel
+//   chains, integration, events, and the pose of the next step) |
+// Wave 0 issues at a raised priority from the flows to the end of the step, and a wave still solving levels after
+// many rounds likewise: the critical waves of an instance go ahead of the other instances' waves on their SIMDs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -872,6 +874,7 @@ __device__ void solve_levels(PC& P, const Blk& B, int tid) {
   LevelSolve S{};
   S.st = L_START;
   S.g = -1;
+  int round = 0;
   while (true) {
     sv.resolve(S, NG);
     int n;
@@ -881,7 +884,13 @@ __device__ void solve_levels(PC& P, const Blk& B, int tid) {
     if (__ballot(n >= 0) == 0) break;
     const Pass p = __ballot(ext) != 0 ? node_pass<Mixed, true>(P, B, lane, n, h) : node_pass<Mixed, false>(P, B, lane, n, h);
     if (n >= 0) sv.advance(S, p);
+    // the step waits for the instance's slowest wave: a wave that has run more rounds than most issues ahead of
+    // the other waves on its SIMD (issue priority only; no value changes)
+    ++round;
+    if (round == 8) __builtin_amdgcn_s_setprio(1);
+    if (round == 12) __builtin_amdgcn_s_setprio(2);
   }
+  if (round >= 8) __builtin_amdgcn_s_setprio(0);
 }
 
 // ------------------------------------------------------------------------------------------- (c) network
@@ -892,26 +901,32 @@ __device__ __forceinline__ double head_of(const Blk& B, int n) { return B.vol[n]
 
 // flows.hpp:211-231: effective head and area of node n, and the number of connections that count toward the
 // limiter there (an enabled connection not between two empty spaces). The heads of the other ends are
-// recomputed from vol and level, so that one pass over the nodes suffices.
-__device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n) {
-  const double v = B.vol[n];
-  B.hEff[n] = v > kEmptyVolume ? B.level[n] : kNoHead;
-  B.aEff[n] = v >= B.nVmax[n] ? B.nAov[n] : jmax(B.area[n], B.nAmin[n]);
+// recomputed from vol and level, so that one pass over the nodes suffices. Four lanes (a quad) per node: lane j
+// counts the node's incidence entries j, j + 4, ... (integer counts: their sum is the same in any order).
+__device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n, int j) {
   const int s = P.M.incStart[n], e = s + P.M.incCount[n];
   int d = 0;
-  for (int k0 = s; k0 < e; k0 += kFetch) {
+  for (int k0 = s + j; k0 < e; k0 += 4 * kFetch) {
     i4v q[kFetch];
 #pragma unroll
-    for (int u = 0; u < kFetch; ++u) q[u] = P.M.inc[k0 + u < e ? k0 + u : k0];
+    for (int u = 0; u < kFetch; ++u) q[u] = P.M.inc[k0 + 4 * u < e ? k0 + 4 * u : k0];
 #pragma unroll
     for (int u = 0; u < kFetch; ++u) {
-      if (k0 + u >= e || !B.cen[q[u].x]) continue;
+      if (k0 + 4 * u >= e || !B.cen[q[u].x]) continue;
       const int a = q[u].y, b = q[u].z;
       if (a >= 0 && head_of(B, a) < kNoHeadTest && head_of(B, b) < kNoHeadTest) continue;
       ++d;
     }
   }
-  B.deg[n] = d;
+  // the quad's four counts (quad_perm 1,0,3,2 then 2,3,0,1): every lane ends with the node's degree
+  d += __builtin_amdgcn_update_dpp(0, d, 0xB1, 0xF, 0xF, false);
+  d += __builtin_amdgcn_update_dpp(0, d, 0x4E, 0xF, 0xF, false);
+  if (j == 0) {
+    const double v = B.vol[n];
+    B.hEff[n] = v > kEmptyVolume ? B.level[n] : kNoHead;
+    B.aEff[n] = v >= B.nVmax[n] ? B.nAov[n] : jmax(B.area[n], B.nAmin[n]);
+    B.deg[n] = d;
+  }
 }
 
 // The Villemonte factor of a submerged weir, (1 - r^1.5)^0.385 with the base clipped at 0 (flows.hpp:59): both
@@ -1227,11 +1242,14 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
-    for (int n = tid; n < P.S.nn; n += TPB) node_heads(P, B, n);
+    for (int n = tid >> 2; n < P.S.nn; n += TPB >> 2) node_heads(P, B, n, tid & 3);
   }
   __syncthreads();
 
   // (c) flows per connection
+  // wave 0 carries the most work from here to the end of the step (the second flow slots, the node gathers and
+  // monitor sums, the tail while the other waves wait): it issues ahead of the other instances' waves on its SIMD
+  if (tid < 64) __builtin_amdgcn_s_setprio(2);
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
@@ -1270,7 +1288,11 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
 
   {
     PC& P = model_at(pa);  // outside the branch: the address stays uniform
-    if (tid < 64) step_tail(P, carve(lds, P), tid);
+    // wave 0 alone runs the tail, still at the raised priority, then returns to the default one
+    if (tid < 64) {
+      step_tail(P, carve(lds, P), tid);
+      __builtin_amdgcn_s_setprio(0);
+    }
   }
   __syncthreads();
 }

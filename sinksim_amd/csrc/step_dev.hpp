@@ -1272,9 +1272,16 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
     for (int m = tid; m < P.M.nmon; m += TPB) {
       double o = 0;
       const int s = P.M.monStart[m], e = s + P.M.monCount[m];
-      for (int k = s; k < e; ++k) {
-        const double qc = B.q[P.M.monConn[k]];
-        if (qc > 0) o += qc;
+      for (int k0 = s; k0 < e; k0 += kFetch) {
+        int c[kFetch];
+        double q[kFetch];
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u) c[u] = P.M.monConn[k0 + u < e ? k0 + u : k0];
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u) q[u] = B.q[c[u]];
+#pragma unroll
+        for (int u = 0; u < kFetch; ++u)
+          if (k0 + u < e && q[u] > 0) o += q[u];
       }
       B.over[m] = o;
     }

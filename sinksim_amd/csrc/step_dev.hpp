@@ -18,8 +18,9 @@
 //   lanes): effective heads and limiter degrees | per connection (weirs first): flows | per node: gather, volume
 //   update, load terms; per monitor: flow sums; one lane: sea inflow | wave 0: the tail (loads as seven parallel
 //   chains, integration, events, and the pose of the next step) |
-// Wave 0 issues at a raised priority from the flows to the end of the step, and a wave still solving levels after
-// many rounds likewise: the critical waves of an instance go ahead of the other instances' waves on their SIMDs.
+// Wave 0 issues at a raised priority from the flows to the end of the step, and the waves of the level solve from
+// their first round on (higher after five rounds): the critical waves of an instance go ahead of the other
+// instances' waves on their SIMDs.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -884,13 +885,14 @@ __device__ void solve_levels(PC& P, const Blk& B, int tid) {
     if (__ballot(n >= 0) == 0) break;
     const Pass p = __ballot(ext) != 0 ? node_pass<Mixed, true>(P, B, lane, n, h) : node_pass<Mixed, false>(P, B, lane, n, h);
     if (n >= 0) sv.advance(S, p);
-    // the step waits for the instance's slowest wave: a wave that has run more rounds than most issues ahead of
-    // the other waves on its SIMD (issue priority only; no value changes)
+    // issue priority only (no value changes): after its first round a solving wave goes ahead of the other
+    // instances' waves in their other phases on its SIMD, and after five rounds (a straggler, which the step's
+    // barrier waits for) further ahead
     ++round;
-    if (round == 8) __builtin_amdgcn_s_setprio(1);
-    if (round == 12) __builtin_amdgcn_s_setprio(2);
+    if (round == 1) __builtin_amdgcn_s_setprio(1);
+    if (round == 5) __builtin_amdgcn_s_setprio(2);
   }
-  if (round >= 8) __builtin_amdgcn_s_setprio(0);
+  if (round >= 1) __builtin_amdgcn_s_setprio(0);
 }
 
 // ------------------------------------------------------------------------------------------- (c) network

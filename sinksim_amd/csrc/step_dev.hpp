@@ -1297,8 +1297,10 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
 
   {
     PC& P = model_at(pa);  // outside the branch: the address stays uniform
-    // wave 0 alone runs the tail, still at the raised priority, then returns to the default one
+    // wave 0 alone runs the tail, at the highest priority (ahead of other instances' solving waves), then returns
+    // to the default one
     if (tid < 64) {
+      __builtin_amdgcn_s_setprio(3);
       step_tail(P, carve(lds, P), tid);
       __builtin_amdgcn_s_setprio(0);
     }

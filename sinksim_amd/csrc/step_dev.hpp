@@ -1250,8 +1250,9 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
 
   // (c) flows per connection
   // wave 0 carries the most work from here to the end of the step (the second flow slots, the node gathers and
-  // monitor sums, the tail while the other waves wait): it issues ahead of the other instances' waves on its SIMD
-  if (tid < 64) __builtin_amdgcn_s_setprio(2);
+  // monitor sums, the tail while the other waves wait): it issues at the highest priority, ahead of the other
+  // instances' waves on its SIMD (solving waves included)
+  if (tid < 64) __builtin_amdgcn_s_setprio(3);
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
@@ -1297,10 +1298,8 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
 
   {
     PC& P = model_at(pa);  // outside the branch: the address stays uniform
-    // wave 0 alone runs the tail, at the highest priority (ahead of other instances' solving waves), then returns
-    // to the default one
+    // wave 0 alone runs the tail, still at the highest priority, then returns to the default one
     if (tid < 64) {
-      __builtin_amdgcn_s_setprio(3);
       step_tail(P, carve(lds, P), tid);
       __builtin_amdgcn_s_setprio(0);
     }

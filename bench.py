@@ -34,7 +34,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two o
 FLOPS_PER_STEP = 2.44e5
 # HBM bytes per launch of run_batch<false, false, 256> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
 # plain copy kernel: profiles/run_batch_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 235.0e6
+TRAFFIC_BYTES_PER_LAUNCH = 235.1e6
 
 
 def parse():

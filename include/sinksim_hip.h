@@ -31,7 +31,7 @@ typedef enum ss_status {
 } ss_status;
 
 /* numerics of the batch: the reference's Numerics::Portable and Numerics::PortableMixed
- * (engine/include/sinksim/simulation.hpp:195); BatchOptions::mixedPrecision false / true
+ * (engine/include/sinksim/simulation.hpp:26); BatchOptions::mixedPrecision false / true
  * (engine/include/sinksim/cuda/batch.hpp:29) */
 enum { SS_NUMERICS_PORTABLE = 1, SS_NUMERICS_PORTABLE32 = 2 };
 
@@ -111,7 +111,7 @@ typedef struct ss_instance_result {
   int32_t events;    /* number of events recorded */
 } ss_instance_result;
 
-/* Event (engine/include/sinksim/kernel/types.hpp:252-258): kind 0 overflow (id = monitor), 1 mark (id = mark),
+/* Event (engine/include/sinksim/kernel/types.hpp:224-228): kind 0 overflow (id = monitor), 1 mark (id = mark),
  * 2 founder (id = -1). */
 typedef struct ss_event {
   double t;
@@ -196,7 +196,24 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
  * wall time of the device work (uploads excluded, as in the reference). SS_LOGIC when called twice. */
 ss_status ss_batch_run(ss_batch* b, double* seconds);
 
-/* Batch::total_steps / shared_memory_bytes (engine/include/sinksim/cuda/batch.hpp:73-74). */
+/* Layer 1 of the boundary: kernel::step (engine/include/sinksim/kernel/step.hpp:41-54) over the batch, as the
+ * reference's Simulation::step (engine/src/simulation.cpp:147-154) calls it after a restore in the acceptance procedure
+ * (apps/sinksim_check.cpp:109-127). Every instance takes exactly `steps` steps (perInstance[i] for instance i when
+ * perInstance is not null) from its staged or restored state, whether or not it founders or passes tMax, as repeated
+ * calls of step() do. The first call uploads the staged instances; later calls continue from the device state. Results,
+ * traces and state records read as after ss_batch_run; restore, the setters and ss_batch_run are refused afterwards
+ * (SS_LOGIC), and ss_batch_step is refused after ss_batch_run. *seconds (may be null) receives the wall time of the
+ * launch. */
+ss_status ss_batch_step(ss_batch* b, int64_t steps, const int64_t* perInstance, double* seconds);
+
+/* Every instance's state after ss_batch_run or ss_batch_step as a record of the trace layout (10 + 5 * nnode values:
+ * t zO pitch roll vz wth wph inflow Vb hullTop, then vol level cx cy cz per node; the columns of io::Trace,
+ * engine/include/sinksim/io/trace.hpp:15-20), instance after instance (a fleet's records have their own model's node
+ * count). No reference counterpart: one download in place of a Batch::result call per instance. */
+ss_status ss_batch_state_records(const ss_batch* b, double* records);
+
+/* Batch::total_steps / shared_memory_bytes (engine/include/sinksim/cuda/batch.hpp:73-74). The LDS size is the forward
+ * kernel's per simulation at the batch's launch shape, known from creation on. */
 ss_status ss_batch_total_steps(const ss_batch* b, int64_t* steps);
 ss_status ss_batch_shared_memory_bytes(const ss_batch* b, size_t* bytes);
 

@@ -180,6 +180,28 @@ class Batch:
         A.check(self._lib.ss_batch_run(self._h, C.byref(s)))
         return s.value
 
+    def step(self, steps: int = 1, per_instance=None) -> float:
+        """kernel::step over the batch (engine/include/sinksim/kernel/step.hpp:41-54): every instance takes exactly
+        `steps` steps (per_instance[i] for instance i), whether or not it founders or passes tMax. The first call starts
+        from the staged or restored states, later calls continue. Returns the launch's wall time (s)."""
+        s = C.c_double()
+        if per_instance is None:
+            A.check(self._lib.ss_batch_step(self._h, int(steps), None, C.byref(s)))
+        else:
+            p = np.ascontiguousarray(per_instance, dtype=np.int64)
+            if p.shape != (self.count,):
+                raise ValueError("per_instance must hold one step count per instance")
+            A.check(self._lib.ss_batch_step(self._h, 0, p.ctypes.data_as(C.POINTER(C.c_int64)), C.byref(s)))
+        return s.value
+
+    def state_records(self) -> np.ndarray:
+        """Every instance's state in the trace layout (10 + 5 * nodes values), [count, record] for a single-model batch
+        (a flat array of each instance's record for a fleet)."""
+        sizes = [10 + 5 * self.model_of(i)[0].nodes for i in range(self.count)]
+        out = np.zeros(int(sum(sizes)))
+        A.check(self._lib.ss_batch_state_records(self._h, A.ptr(out, C.c_double)))
+        return out.reshape(self.count, sizes[0]) if len(set(sizes)) == 1 else out
+
     def total_steps(self) -> int:
         v = C.c_int64()
         A.check(self._lib.ss_batch_total_steps(self._h, C.byref(v)))

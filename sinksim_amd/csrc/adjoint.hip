@@ -568,6 +568,8 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     throw;
   }
   b->tpb = keepTpb;
+  b->ldsBytes = forward_lds_bytes(*b);  // the forward kernel's layout, as the accessor reports it
+  b->stepped = false;                   // the device now holds the staged instances again
   R.ca.upload(ca);
   R.theta.upload(s.theta, count * static_cast<size_t>(s.np));
   R.pa.upload(s.pa, nconn);

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cfloat>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -121,12 +122,14 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
   }
 
   const bool curves = Bt.curveEvery > 0;
+  const bool exact = Bt.exact != 0;
+  const long long limit = exact && Bt.stepLimit ? Bt.stepLimit[inst] : maxSteps;
   uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P));  // the same in every lane, held in scalar registers
-  for (long long s = 0; s < maxSteps; ++s) {
+  for (long long s = 0; s < limit; ++s) {
     PC& Q = model_at(pa);
     const Blk Bq = carve(lds, Q);
     // uniform: LDS values after a barrier
-    if (Bq.scal[S_FOUNDERED] != 0 || Bq.scal[S_T] >= tMax || Bq.scal[S_CALM] > kCalmStop) break;
+    if (!exact && (Bq.scal[S_FOUNDERED] != 0 || Bq.scal[S_T] >= tMax || Bq.scal[S_CALM] > kCalmStop)) break;
     // a readout-curve record when due (simulation.cpp:175-176). Every wave evaluates the condition on the same
     // LDS values; the barrier inside orders all of those reads (and the record's reads of the volumes) before
     // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
@@ -511,6 +514,7 @@ std::unique_ptr<ss_batch> make_batch(const ss_batch_options& opt, std::vector<st
   for (size_t i = 0; i < count; ++i) b.order[i] = static_cast<int32_t>(i);
   std::stable_sort(b.order.begin(), b.order.end(), [&](int32_t x, int32_t y) { return b.model(x).cost > b.model(y).cost; });
   b.recSize = 10 + 5 * b.models[0]->NN;
+  b.ldsBytes = forward_lds_bytes(b);
   b.maxSamples = b.opt.traced > 0 ? static_cast<int32_t>(std::floor(b.opt.tMax / b.opt.sampleEvery)) + 2 : 0;
   return B;
 }
@@ -660,13 +664,15 @@ BatchKernel batch_kernel(bool mixed, bool persistent, int tpb) {
   return mixed ? run_batch<true, false, kThreads> : run_batch<false, false, kThreads>;
 }
 
-// Uploads the staged instances, runs them to their end (one graph launch, or one per chunk of stepsPerLaunch
-// steps; at most maxLaunches) and reads the results back. The launch shape is the batch's (tpb, blocks).
-void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
-  const size_t count = static_cast<size_t>(b->opt.count);
-  upload(b);
+// The launch shape of a batch after its upload: the kernel (numerics, refilling or not, workgroup size) and the
+// grid. Every kernel instantiation that may run gets the batch's dynamic LDS size allowed first.
+struct LaunchPlan {
+  BatchKernel kernel;
+  int grid;
+};
 
-  const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
+LaunchPlan plan_launch(ss_batch* b) {
+  const size_t count = static_cast<size_t>(b->opt.count);
   for (int tpb : {kThreads, 512})
     for (bool pers : {false, true})
       for (bool mixed : {false, true})
@@ -684,7 +690,17 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
   const int grid = static_cast<int>(std::min<long long>(static_cast<long long>(count), b->blocks > 0 ? b->blocks : resident));
   // one workgroup per instance when all of them are resident at once; the refilling loop otherwise
   b->refill = grid < static_cast<int>(count);
-  const BatchKernel kernel = batch_kernel(b->mixed, b->refill, b->tpb);
+  return LaunchPlan{batch_kernel(b->mixed, b->refill, b->tpb), grid};
+}
+
+// Uploads the staged instances, runs them to their end (one graph launch, or one per chunk of stepsPerLaunch
+// steps; at most maxLaunches) and reads the results back. The launch shape is the batch's (tpb, blocks).
+void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
+  const size_t count = static_cast<size_t>(b->opt.count);
+  upload(b);
+
+  const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
+  const LaunchPlan lp = plan_launch(b);
 
   // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
   // replayed for every chunk: identical launches, one submission each.
@@ -707,7 +723,7 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
   hip_check(hipEventCreate(&e1), "hipEventCreate");
   hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, b->counter.p);
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
+  hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
                      static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
   hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
@@ -740,6 +756,54 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
   if (seconds) *seconds = secs;
 }
 
+// One launch in exact-step mode: instance i takes exactly limits[i] steps (or k each when limits is null) from the
+// device state (uploaded from the staged instances at the first call), founder and tMax notwithstanding.
+void step_exact(ss_batch* b, long long k, const int64_t* limits, double* seconds) {
+  const size_t count = static_cast<size_t>(b->opt.count);
+  if (!b->stepped) upload(b);
+  b->stepped = true;
+  long long most = k;
+  if (limits) {
+    std::vector<long long> l(limits, limits + count);
+    most = *std::max_element(l.begin(), l.end());
+    b->stepLimit.upload(l);
+  } else {
+    b->stepLimit.release();
+  }
+  b->exact = true;
+  const DevBatch bv = b->batchView();
+  b->exact = false;
+  b->dBatch.upload(&bv, 1);
+  const LaunchPlan lp = plan_launch(b);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  struct Release {
+    hipEvent_t& a; hipEvent_t& c;
+    ~Release() {
+      if (a) (void)hipEventDestroy(a);
+      if (c) (void)hipEventDestroy(c);
+    }
+  } release{e0, e1};
+  hip_check(hipEventCreate(&e0), "hipEventCreate");
+  hip_check(hipEventCreate(&e1), "hipEventCreate");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+  const auto t0 = std::chrono::steady_clock::now();
+  hip_check(hipEventRecord(e0, nullptr), "hipEventRecord");
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, nullptr, b->counter.p);
+  if (most > 0)
+    hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, nullptr, static_cast<const ModelParams*>(b->dModels.p),
+                       static_cast<const DevBatch*>(b->dBatch.p), most, DBL_MAX);
+  hip_check(hipGetLastError(), "kernel launch");
+  hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
+  hip_check(hipEventSynchronize(e1), "kernel execution");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  b->kernelMs += ms;
+  b->launches++;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  read_back(b);
+  if (seconds) *seconds = secs;
+}
+
 }  // namespace
 
 extern "C" {
@@ -752,6 +816,7 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
     need(workgroups >= 0, "ss_batch_set_launch: negative workgroup count");
     b->tpb = threadsPerSim == 0 ? kThreads : threadsPerSim;
     b->blocks = workgroups;
+    b->ldsBytes = forward_lds_bytes(*b);
   });
 }
 
@@ -809,6 +874,7 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
     }
     b->tpb = bestTpb;
     b->blocks = bestBlocks;
+    b->ldsBytes = forward_lds_bytes(*b);
     res.threadsPerSim = bestTpb;
     res.workgroups = bestBlocks;
     if (out) *out = res;
@@ -818,9 +884,43 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
 ss_status ss_batch_run(ss_batch* b, double* seconds) {
   return guarded([&] {
     need(b != nullptr, "ss_batch_run: null batch");
+    if (b->stepped) fail(SS_LOGIC, "ss_batch_run: run after ss_batch_step");
     if (b->ran) fail(SS_LOGIC, "ss_batch_run: run twice");
     b->ran = true;
     execute(b, seconds);
+  });
+}
+
+ss_status ss_batch_step(ss_batch* b, int64_t steps, const int64_t* perInstance, double* seconds) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_step: null batch");
+    if (b->ran && !b->stepped) fail(SS_LOGIC, "ss_batch_step: step after run");
+    if (perInstance) {
+      for (int32_t i = 0; i < b->opt.count; ++i) need(perInstance[i] >= 0, "ss_batch_step: negative step count");
+    } else {
+      need(steps >= 0, "ss_batch_step: negative step count");
+    }
+    b->ran = true;
+    step_exact(b, steps, perInstance, seconds);
+  });
+}
+
+ss_status ss_batch_state_records(const ss_batch* b, double* records) {
+  return guarded([&] {
+    need(b != nullptr && records != nullptr, "ss_batch_state_records: null argument");
+    if (!b->ran) fail(SS_LOGIC, "ss_batch_state_records: records before run");
+    const std::vector<double> vol = b->vol.all(), level = b->level.all(), cx = b->ccx.all(), cy = b->ccy.all(), cz = b->ccz.all();
+    double* r = records;
+    for (int32_t i = 0; i < b->opt.count; ++i) {
+      const size_t k = static_cast<size_t>(i), nn = static_cast<size_t>(b->model(i).NN), o = static_cast<size_t>(b->nodeOff[k]);
+      for (int j = 0; j < 7; ++j) r[j] = b->rSt[k * 7 + j];
+      for (int j = 0; j < 3; ++j) r[7 + j] = b->rOut[k * 7 + j];
+      r += 10;
+      for (const std::vector<double>* a : {&vol, &level, &cx, &cy, &cz}) {
+        std::copy(a->begin() + static_cast<std::ptrdiff_t>(o), a->begin() + static_cast<std::ptrdiff_t>(o + nn), r);
+        r += nn;
+      }
+    }
   });
 }
 
@@ -891,6 +991,7 @@ ss_status ss_batch_trace(const ss_batch* b, int32_t instance, double* dense, int
     need(b != nullptr, "ss_batch_trace: null batch");
     if (!b->ran) fail(SS_LOGIC, "ss_batch_trace: trace before run");
     if (instance < 0 || instance >= b->opt.traced) fail(SS_OUT_OF_RANGE, "ss_batch_trace: instance was not traced");
+    need(denseCap >= 0 && samplesCap >= 0, "ss_batch_trace: negative capacity");
     const size_t i = static_cast<size_t>(instance), rs = static_cast<size_t>(b->recSize);
     int32_t nd = 0, ns = 0;
     b->denseCount.download(&nd, 1, i);
@@ -949,3 +1050,4 @@ static ss_status set_one(ss_batch* b, int32_t instance, int32_t conn, const char
     f(static_cast<size_t>(b->connOff[static_cast<size_t>(instance)]) + static_cast<size_t>(conn));
   });
 }
+

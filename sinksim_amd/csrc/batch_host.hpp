@@ -193,6 +193,10 @@ struct ss_batch {
   int launches = 0;
   bool stopWhenStable = false;  // the oracle's calm rule ends instances that settle (simulation.cpp:179-182)
   Dev<double> calm;             // per instance, with stopWhenStable
+  // exact stepping (ss_batch_step): the launch runs every instance for exactly its step count
+  bool exact = false;
+  bool stepped = false;          // the device holds the instances (ss_batch_step has uploaded them)
+  Dev<long long> stepLimit;      // per instance, when the counts differ
   // the last gradient call's launch shape: threads per simulation, the tape's large arrays in global memory, LDS bytes
   int32_t adjTpb = 0, adjGlobal = 0;
   size_t adjLds = 0;
@@ -235,6 +239,7 @@ struct ss_batch {
     b.curveEvery = opt.curveEvery; b.curveMax = curveMax; b.curveRec = W(curveRec.p); b.curveCount = W(curveCount.p);
     b.nextCurve = W(nextCurve.p);
     b.stopWhenStable = stopWhenStable ? 1 : 0; b.calm = W(calm.p);
+    b.exact = exact ? 1 : 0; b.stepLimit = exact && stepLimit.p ? G(stepLimit.p) : nullptr;
     return b;
   }
 };
@@ -247,6 +252,13 @@ inline void need(bool ok, const std::string& what) {
 
 inline void check_instance(const ss_batch* b, int32_t i) {
   if (i < 0 || i >= b->opt.count) fail(SS_OUT_OF_RANGE, "instance " + std::to_string(i) + " out of range");
+}
+
+// The forward kernel's LDS bytes per simulation: the largest model's layout at the batch's workgroup size.
+inline size_t forward_lds_bytes(const ss_batch& b) {
+  size_t n = 0;
+  for (const auto& m : b.models) n = std::max(n, m->lds_bytes(b.tpb));
+  return n;
 }
 
 // The largest dynamic LDS allocation a workgroup may request on the current device.

@@ -150,6 +150,11 @@ struct DevBatch {
   SSG double* nextCurve;
   int32_t stopWhenStable;  // stop an instance after 20 min of calm, once past 30 min (simulation.cpp:179-182)
   SSG double* calm;        // [count] the calm time so far (s), kept across launches; > 1200 once stopped
+  // Exact stepping (the single-step API): every instance takes exactly its step count in the launch, whether or not it
+  // founders or passes tMax (kernel::step does not look at either); stepLimit[inst] is that count (null: the
+  // launch's maxSteps for every instance).
+  int32_t exact;
+  const SSG long long* stepLimit;
 };
 
 // LDS layout of one simulation, as offsets in doubles (the host computes it: lds_layout in batch.hip).

@@ -212,6 +212,20 @@ ss_status ss_batch_step(ss_batch* b, int64_t steps, const int64_t* perInstance, 
  * count). No reference counterpart: one download in place of a Batch::result call per instance. */
 ss_status ss_batch_state_records(const ss_batch* b, double* records);
 
+/* No counterpart in the reference: ss_batch_step for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). It steps `count` instances of the batch's model (a
+ * single-model batch; its numerics and launch shape) exactly `steps` steps each, from the device records recIn
+ * [count * (10 + 5 * nnode)] (trace layout, as ss_batch_state_records returns them) with the device connection tables
+ * en [count * nconn] (0/1), area [count * nconn] and tOn [count * nconn], and writes the states after the steps to
+ * recOut (device, same layout; may equal recIn). The batch's own instances and staged tables are not used or changed.
+ * Everything else the launch needs lives in one fixed device array of 16 MiB per device (about 5.5 KB per titanic64
+ * instance): a count that needs more is SS_INVALID_ARGUMENT. No device or host memory is allocated on success. The
+ * work is queued on `stream` (a hipStream_t; null for the default stream) and not waited for: recOut is ready when
+ * the stream's work is. Every call on a device shares that device's array, so calls that overlap on one device race
+ * (the caller serialises them, e.g. on one stream). */
+ss_status ss_batch_step_noalloc(const ss_batch* b, int32_t count, int64_t steps, const double* recIn, const uint8_t* en, const double* area,
+                                const double* tOn, double* recOut, void* stream);
+
 /* Batch::total_steps / shared_memory_bytes (engine/include/sinksim/cuda/batch.hpp:73-74). The LDS size is the forward
  * kernel's per simulation at the batch's launch shape, known from creation on. */
 ss_status ss_batch_total_steps(const ss_batch* b, int64_t* steps);

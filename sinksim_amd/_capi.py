@@ -94,6 +94,8 @@ SIGNATURES = {
     "ss_batch_run": (C.c_int, [C.c_void_p, _dp]),
     "ss_batch_step": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), _dp]),
     "ss_batch_state_records": (C.c_int, [C.c_void_p, _dp]),
+    "ss_batch_step_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
     "ss_batch_total_steps": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "ss_batch_shared_memory_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
     "ss_batch_result": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(SsInstanceResult), _dp, _dp, _dp, _dp, _dp,

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -1051,3 +1052,205 @@ static ss_status set_one(ss_batch* b, int32_t instance, int32_t conn, const char
   });
 }
 
+// ------------------------------------------------------------------------------------------- allocation-free stepping
+//
+// ss_batch_step_noalloc: ss_batch_step for callers that may not allocate device memory. The caller passes device
+// inputs and outputs; everything else the launch needs (the batch and model views the kernel reads, the work counter,
+// the per-instance offsets, state, event bookkeeping and connection products) lives in one file-scope device array of
+// fixed size, `scratch`. Host code never passes the array's name to the device: its device address is looked up once
+// per device (hipGetSymbolAddress, with that device current) and kept per device.
+
+namespace ssa {
+
+constexpr size_t kNoAllocScratchBytes = 16u << 20;  // per device, held until the process ends
+constexpr int kMaxDevices = 64;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// the staging kernel takes both views by value (kernel arguments are limited to 4 KiB)
+static_assert(sizeof(ModelParams) + sizeof(DevBatch) + 6 * sizeof(void*) <= 4096, "views too large for kernel arguments");
+
+// Where each per-call array sits in the scratch array (byte offsets, 256-byte aligned).
+struct NoAllocPlan {
+  size_t models, batch, counter, instModel, order, nodeOff, connOff, evOff, monOff, markOff, carea, st, out, final7, vol, level, cx, cy, cz,
+      evT, evId, evKind, evCount, overT, markT, foundered, founderT, steps, calm, total;
+};
+
+inline NoAllocPlan noalloc_plan(size_t count, const Model& m) {
+  NoAllocPlan p{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + std::max<size_t>(bytes, 1) + 255) & ~static_cast<size_t>(255);
+    return at;
+  };
+  const size_t NN = static_cast<size_t>(m.NN), NC = static_cast<size_t>(m.NC), ev = static_cast<size_t>(m.evCap);
+  p.models = take(sizeof(ModelParams)); p.batch = take(sizeof(DevBatch)); p.counter = take(sizeof(int32_t));
+  p.instModel = take(count * 4); p.order = take(count * 4); p.nodeOff = take(count * 4); p.connOff = take(count * 4);
+  p.evOff = take(count * 4); p.monOff = take(count * 4); p.markOff = take(count * 4);
+  p.carea = take(count * NC * 8);
+  p.st = take(count * 7 * 8); p.out = take(count * 7 * 8); p.final7 = take(count * 7 * 8);
+  p.vol = take(count * NN * 8); p.level = take(count * NN * 8); p.cx = take(count * NN * 8); p.cy = take(count * NN * 8);
+  p.cz = take(count * NN * 8);
+  p.evT = take(count * ev * 8); p.evId = take(count * ev * 4); p.evKind = take(count * ev); p.evCount = take(count * 4);
+  p.overT = take(count * static_cast<size_t>(m.nmon) * 8); p.markT = take(count * static_cast<size_t>(m.nmark) * 8);
+  p.foundered = take(count); p.founderT = take(count * 8); p.steps = take(count * 8); p.calm = take(count * 8);
+  p.total = off;
+  return p;
+}
+
+// One workgroup per instance: the views (workgroup 0), the instance's offsets, its state unpacked from its record
+// (trace layout), its event bookkeeping cleared and its connection products coef * area (one rounded multiplication,
+// as the host forms them for ss_batch_run and ss_batch_step).
+__global__ __launch_bounds__(256) void noalloc_stage(ModelParams mp, DevBatch bv, ModelParams* dstM, DevBatch* dstB, int32_t* counter,
+                                                      const double* __restrict__ recIn, const double* __restrict__ area, double* carea) {
+  const int i = blockIdx.x, t = threadIdx.x;
+  const int NN = mp.S.nn, NC = mp.M.nc, ev = mp.M.evCap, nmon = mp.M.nmon, nmark = mp.M.nmark;
+  if (i == 0 && t == 0) {
+    *dstM = mp;
+    *dstB = bv;
+    *counter = 0;
+  }
+  SSG int32_t* im = const_cast<SSG int32_t*>(bv.instModel);
+  SSG int32_t* ord = const_cast<SSG int32_t*>(bv.order);
+  if (t == 0) {
+    im[i] = 0; ord[i] = i;
+    const_cast<SSG int32_t*>(bv.nodeOff)[i] = i * NN; const_cast<SSG int32_t*>(bv.connOff)[i] = i * NC;
+    const_cast<SSG int32_t*>(bv.evOff)[i] = i * ev; const_cast<SSG int32_t*>(bv.monOff)[i] = i * nmon;
+    const_cast<SSG int32_t*>(bv.markOff)[i] = i * nmark;
+    bv.evCount[i] = 0; bv.foundered[i] = 0; bv.founderT[i] = -1.0; bv.steps[i] = 0; bv.calm[i] = 0.0;
+  }
+  const size_t rs = static_cast<size_t>(10 + 5 * NN);
+  const double* r = recIn + static_cast<size_t>(i) * rs;
+  if (t < 7) {
+    bv.st[i * 7 + t] = r[t];
+    bv.out[i * 7 + t] = t < 3 ? r[7 + t] : 0.0;
+    bv.final7[i * 7 + t] = 0.0;
+  }
+  const size_t iN = static_cast<size_t>(i) * NN;
+  for (int n = t; n < NN; n += 256) {
+    bv.vol[iN + n] = r[10 + n]; bv.level[iN + n] = r[10 + NN + n];
+    bv.cx[iN + n] = r[10 + 2 * NN + n]; bv.cy[iN + n] = r[10 + 3 * NN + n]; bv.cz[iN + n] = r[10 + 4 * NN + n];
+  }
+  const size_t iC = static_cast<size_t>(i) * NC;
+  for (int c = t; c < NC; c += 256) carea[iC + c] = mp.M.conD[c].w * area[iC + c];
+  for (int e = t; e < ev; e += 256) {
+    bv.evT[static_cast<size_t>(i) * ev + e] = 0.0; bv.evId[static_cast<size_t>(i) * ev + e] = 0; bv.evKind[static_cast<size_t>(i) * ev + e] = 0;
+  }
+  for (int k = t; k < nmon; k += 256) bv.overT[static_cast<size_t>(i) * nmon + k] = -1.0;
+  for (int k = t; k < nmark; k += 256) bv.markT[static_cast<size_t>(i) * nmark + k] = -1.0;
+}
+
+// One workgroup per instance: its state after the launch, packed into its record (trace layout).
+__global__ __launch_bounds__(256) void noalloc_pack(const DevBatch* __restrict__ batch, int32_t nn, double* __restrict__ recOut) {
+  const DevBatch& bv = *batch;
+  const int i = blockIdx.x, t = threadIdx.x;
+  const size_t rs = static_cast<size_t>(10 + 5 * nn);
+  double* r = recOut + static_cast<size_t>(i) * rs;
+  if (t < 7) r[t] = bv.st[i * 7 + t];
+  if (t < 3) r[7 + t] = bv.out[i * 7 + t];
+  const size_t iN = static_cast<size_t>(i) * nn;
+  for (int n = t; n < nn; n += 256) {
+    r[10 + n] = bv.vol[iN + n]; r[10 + nn + n] = bv.level[iN + n];
+    r[10 + 2 * nn + n] = bv.cx[iN + n]; r[10 + 3 * nn + n] = bv.cy[iN + n]; r[10 + 4 * nn + n] = bv.cz[iN + n];
+  }
+}
+
+}  // namespace ssa
+
+namespace {
+
+std::mutex g_scratchLock;
+void* g_scratchAt[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up at the first call on that device.
+unsigned char* scratch_address() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) fail(SS_RUNTIME, "ss_batch_step_noalloc: device index beyond the library's table");
+  std::lock_guard<std::mutex> hold(g_scratchLock);
+  if (!g_scratchAt[dev]) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    g_scratchAt[dev] = p;
+  }
+  return static_cast<unsigned char*>(g_scratchAt[dev]);
+}
+
+}  // namespace
+
+extern "C" {
+
+ss_status ss_batch_step_noalloc(const ss_batch* b, int32_t count, int64_t steps, const double* recIn, const uint8_t* en, const double* area,
+                                const double* tOn, double* recOut, void* stream) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_step_noalloc: null batch");
+    need(b->models.size() == 1, "ss_batch_step_noalloc: single-model batches only");
+    need(count >= 1, "ss_batch_step_noalloc: count must be at least 1");
+    need(steps >= 0, "ss_batch_step_noalloc: negative step count");
+    need(recIn && en && area && tOn && recOut, "ss_batch_step_noalloc: null array");
+    const Model& m = *b->models[0];
+    const NoAllocPlan plan = noalloc_plan(static_cast<size_t>(count), m);
+    if (plan.total > kNoAllocScratchBytes)
+      fail(SS_INVALID_ARGUMENT, "ss_batch_step_noalloc: " + std::to_string(count) + " instances need " + std::to_string(plan.total) +
+                                    " bytes of scratch, the maximum is " + std::to_string(kNoAllocScratchBytes));
+    const size_t lds = m.lds_bytes(b->tpb);
+    if (lds > lds_limit()) fail(SS_RUNTIME, "ss_batch_step_noalloc: the model's LDS exceeds the device's limit at this launch shape");
+    unsigned char* base = scratch_address();
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+
+    // the views the kernel reads, pointing into the scratch array and at the caller's tables
+    ModelParams mp = m.params(b->tpb);
+    DevBatch bv{};
+    bv.count = count;
+    bv.instModel = G(reinterpret_cast<const int32_t*>(base + plan.instModel));
+    bv.order = G(reinterpret_cast<const int32_t*>(base + plan.order));
+    bv.nodeOff = G(reinterpret_cast<const int32_t*>(base + plan.nodeOff));
+    bv.connOff = G(reinterpret_cast<const int32_t*>(base + plan.connOff));
+    bv.evOff = G(reinterpret_cast<const int32_t*>(base + plan.evOff));
+    bv.monOff = G(reinterpret_cast<const int32_t*>(base + plan.monOff));
+    bv.markOff = G(reinterpret_cast<const int32_t*>(base + plan.markOff));
+    bv.counter = W(reinterpret_cast<int32_t*>(base + plan.counter));
+    bv.en = G(en); bv.area = G(reinterpret_cast<const double*>(base + plan.carea)); bv.tOn = G(tOn);
+    bv.st = W(reinterpret_cast<double*>(base + plan.st)); bv.out = W(reinterpret_cast<double*>(base + plan.out));
+    bv.vol = W(reinterpret_cast<double*>(base + plan.vol)); bv.level = W(reinterpret_cast<double*>(base + plan.level));
+    bv.cx = W(reinterpret_cast<double*>(base + plan.cx)); bv.cy = W(reinterpret_cast<double*>(base + plan.cy));
+    bv.cz = W(reinterpret_cast<double*>(base + plan.cz));
+    bv.evT = W(reinterpret_cast<double*>(base + plan.evT)); bv.evId = W(reinterpret_cast<int32_t*>(base + plan.evId));
+    bv.evKind = W(reinterpret_cast<uint8_t*>(base + plan.evKind)); bv.evCount = W(reinterpret_cast<int32_t*>(base + plan.evCount));
+    bv.overT = W(reinterpret_cast<double*>(base + plan.overT)); bv.markT = W(reinterpret_cast<double*>(base + plan.markT));
+    bv.foundered = W(reinterpret_cast<uint8_t*>(base + plan.foundered)); bv.founderT = W(reinterpret_cast<double*>(base + plan.founderT));
+    bv.steps = W(reinterpret_cast<long long*>(base + plan.steps)); bv.final7 = W(reinterpret_cast<double*>(base + plan.final7));
+    bv.calm = W(reinterpret_cast<double*>(base + plan.calm));
+    bv.exact = 1;
+    bv.stepLimit = nullptr;
+    ModelParams* dM = reinterpret_cast<ModelParams*>(base + plan.models);
+    DevBatch* dB = reinterpret_cast<DevBatch*>(base + plan.batch);
+
+    // the launch shape: one workgroup per instance when all are resident at once, else the refilling form
+    if (lds > 64 * 1024)
+      for (bool pers : {false, true})
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(batch_kernel(b->mixed, pers, b->tpb)), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds)),
+                  "hipFuncSetAttribute");
+    int perCU = 0, cus = 0;
+    hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(batch_kernel(b->mixed, true, b->tpb)), b->tpb, lds),
+              "occupancy");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+    const int grid = std::min(count, std::max(1, perCU) * std::max(1, cus));
+    const BatchKernel kernel = batch_kernel(b->mixed, grid < count, b->tpb);
+
+    hipLaunchKernelGGL(noalloc_stage, dim3(count), dim3(256), 0, s, mp, bv, dM, dB, reinterpret_cast<int32_t*>(base + plan.counter), recIn, area,
+                       reinterpret_cast<double*>(base + plan.carea));
+    hip_check(hipGetLastError(), "ss_batch_step_noalloc: staging launch");
+    if (steps > 0) {
+      hipLaunchKernelGGL(kernel, dim3(grid), dim3(b->tpb), lds, s, static_cast<const ModelParams*>(dM), static_cast<const DevBatch*>(dB),
+                         static_cast<long long>(steps), DBL_MAX);
+      hip_check(hipGetLastError(), "ss_batch_step_noalloc: batch launch");
+    }
+    hipLaunchKernelGGL(noalloc_pack, dim3(count), dim3(256), 0, s, static_cast<const DevBatch*>(dB), m.NN, recOut);
+    hip_check(hipGetLastError(), "ss_batch_step_noalloc: packing launch");
+  });
+}
+
+}  // extern "C"

@@ -30,10 +30,10 @@ typedef enum ss_status {
   SS_OUT_OF_RANGE = 4
 } ss_status;
 
-/* numerics of the batch: the reference's Numerics::Portable and Numerics::PortableMixed
- * (engine/include/sinksim/simulation.hpp:26); BatchOptions::mixedPrecision false / true
- * (engine/include/sinksim/cuda/batch.hpp:29) */
-enum { SS_NUMERICS_PORTABLE = 1, SS_NUMERICS_PORTABLE32 = 2 };
+/* The reference's numerics, Numerics::{Oracle, Portable, PortableMixed, Std} (engine/include/sinksim/simulation.hpp:26).
+ * A batch on the device computes portable and portable32 (BatchOptions::mixedPrecision false / true,
+ * engine/include/sinksim/cuda/batch.hpp:29); the host CPU engine (ss_cpu_*) computes all four. */
+enum { SS_NUMERICS_ORACLE = 0, SS_NUMERICS_PORTABLE = 1, SS_NUMERICS_PORTABLE32 = 2, SS_NUMERICS_STD = 3 };
 
 /* A compiled ship (docs/FORMATS.md "sinksim.compiled-ship"; engine/include/sinksim/model.hpp:15-35). */
 typedef struct ss_ship {
@@ -328,6 +328,42 @@ ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa,
  * tape's Jacobian rows and per-connection partials were in global memory (else 0), LDS bytes per simulation. SS_LOGIC
  * before any such call. No reference counterpart: a measurement and test hook. */
 ss_status ss_batch_gradient_launch(const ss_batch* b, int32_t* threads, int32_t* globalTape, int64_t* ldsBytes);
+
+/* ---------------------------------------------------------------------------------------------------------------
+ * The host CPU engine: the reference's four numerics on the CPU (the oracle's serial order with V8-semantics pow, the
+ * GPU's canonical tree order in double and in single precision, and the platform math), for the CPU tools and for
+ * callers without a device. No device is needed or touched. */
+
+/* RunOptions (engine/include/sinksim/simulation.hpp:31-35) with the numerics and the thread count of sweep::run_cpu
+ * (engine/include/sinksim/sweep.hpp:78-79). */
+typedef struct ss_cpu_options {
+  int32_t numerics;       /* SS_NUMERICS_ORACLE, _PORTABLE, _PORTABLE32 or _STD */
+  int32_t threads;        /* host threads; <= 0: as many as the hardware runs at once */
+  double tMax;            /* s: each instance runs until it founders or reaches tMax */
+  double curveEvery;      /* s between readout-curve records (RunOptions::every) */
+  int32_t stopWhenStable; /* 1: stop an instance after 20 min of calm past 30 min (engine/src/simulation.cpp:179-182) */
+} ss_cpu_options;
+
+/* sweep::run_cpu (engine/src/sweep.cpp:121-161): count instances of one ship and scenario, each running
+ * Simulation::run (engine/src/simulation.cpp:167-192) from the compiled initial state with its own connection tables
+ * en, area, tOn ([count * nconn] each, instance-major; null: the scenario's for every instance), on opt->threads
+ * host threads. Per instance: results[i] (may be null), up to evCap events at events + i * evCap (may be null), up to
+ * curveCap curve records of 7 values (t trim list water inflow draftF draftA, closed by the final record, as the
+ * reference's run history) at curves + i * curveCap * 7 with their full count in curveCount[i], and the final
+ * readouts final7 + i * 7 (each may be null). *seconds receives the wall time of the runs (no file IO). */
+ss_status ss_cpu_sweep(const ss_ship* ship, const ss_sim* sim, const ss_cpu_options* opt, int32_t count, const uint8_t* en,
+                       const double* area, const double* tOn, ss_instance_result* results, ss_event* events, int32_t evCap,
+                       double* curves, int32_t curveCap, int32_t* curveCount, double* final7, double* seconds);
+
+/* Simulation::record_trace (engine/src/simulation.cpp:194-221) on the host: one instance (connection tables of nconn
+ * values, null: the scenario's) from the compiled initial state to its founder or tMax; the initial record, a dense
+ * record after each of the first denseSteps steps and a sampled record whenever t reaches the next multiple of
+ * sampleEvery, each 10 + 5 * nnode values in the trace layout (at most denseCap / samplesCap kept; the counts are the
+ * full ones), the result and up to evCap events. */
+ss_status ss_cpu_trace(const ss_ship* ship, const ss_sim* sim, int32_t numerics, double tMax, int32_t denseSteps, double sampleEvery,
+                       const uint8_t* en, const double* area, const double* tOn, double* dense, int32_t denseCap, int32_t* denseCount,
+                       double* samples, int32_t samplesCap, int32_t* samplesCount, ss_instance_result* result, ss_event* events,
+                       int32_t evCap);
 
 #ifdef __cplusplus
 }

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsinksim_hip.so")
 
 SS_OK, SS_INVALID_ARGUMENT, SS_RUNTIME, SS_LOGIC, SS_OUT_OF_RANGE = 0, 1, 2, 3, 4
-SS_NUMERICS_PORTABLE, SS_NUMERICS_PORTABLE32 = 1, 2
+SS_NUMERICS_ORACLE, SS_NUMERICS_PORTABLE, SS_NUMERICS_PORTABLE32, SS_NUMERICS_STD = 0, 1, 2, 3
+NUMERICS = {"oracle": SS_NUMERICS_ORACLE, "portable": SS_NUMERICS_PORTABLE, "portable32": SS_NUMERICS_PORTABLE32, "std": SS_NUMERICS_STD}
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
@@ -75,6 +76,11 @@ class SsGradientOptions(C.Structure):
     _fields_ = [("checkpointEvery", C.c_int32), ("relaxed", C.c_int32), ("tMax", C.c_double), ("maxSteps", C.c_int64)]
 
 
+class SsCpuOptions(C.Structure):
+    _fields_ = [("numerics", C.c_int32), ("threads", C.c_int32), ("tMax", C.c_double), ("curveEvery", C.c_double),
+                ("stopWhenStable", C.c_int32)]
+
+
 # every exported entry point and its signature (the test of the library's exports reads this table)
 SIGNATURES = {
     "ss_last_error": (C.c_char_p, []),
@@ -110,6 +116,10 @@ SIGNATURES = {
     "ss_batch_adjoint_step": (C.c_int, [C.c_void_p, C.c_int32, _ip, _ip, _dp, C.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "ss_batch_gradient_launch": (C.c_int, [C.c_void_p, _ip, _ip, C.POINTER(C.c_int64)]),
     "ss_batch_set_stop_when_stable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ss_cpu_sweep": (C.c_int, [C.POINTER(SsShip), C.POINTER(SsSim), C.POINTER(SsCpuOptions), C.c_int32, _bp, _dp, _dp,
+                               C.POINTER(SsInstanceResult), C.POINTER(SsEvent), C.c_int32, _dp, C.c_int32, _ip, _dp, _dp]),
+    "ss_cpu_trace": (C.c_int, [C.POINTER(SsShip), C.POINTER(SsSim), C.c_int32, C.c_double, C.c_int32, C.c_double, _bp, _dp, _dp,
+                               _dp, C.c_int32, _ip, _dp, C.c_int32, _ip, C.POINTER(SsInstanceResult), C.POINTER(SsEvent), C.c_int32]),
 }
 
 _lib = None

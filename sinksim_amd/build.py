@@ -28,7 +28,7 @@ HIP_FLAGS = [
 ]
 
 LIB = os.path.join(HERE, "libsinksim_hip.so")
-LIB_SOURCES = [os.path.join(CSRC, "batch.hip"), os.path.join(CSRC, "adjoint.hip")]
+LIB_SOURCES = [os.path.join(CSRC, "batch.hip"), os.path.join(CSRC, "adjoint.hip"), os.path.join(CSRC, "cpu_engine.cpp")]
 LIB_DEPS = LIB_SOURCES + [os.path.join(CSRC, f) for f in ("step_dev.hpp", "fdlibm_dev.hpp", "adjoint_dev.hpp", "batch_host.hpp")] + [
     os.path.join(ROOT, "include", "sinksim_hip.h")]
 
@@ -47,7 +47,8 @@ def build_library(force: bool = True) -> str:
     if not force and os.path.exists(LIB) and all(os.path.getmtime(LIB) >= os.path.getmtime(d) for d in LIB_DEPS):
         return LIB
     tmp = LIB + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *LIB_SOURCES])
+    # every source is compiled as HIP: cpu_engine.cpp is host code that shares the device's fdlibm (host and device)
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, "-x", "hip", *LIB_SOURCES])
     os.replace(tmp, LIB)
     return LIB
 
@@ -57,14 +58,14 @@ APPS = os.path.join(CSRC, "apps")
 BIN = os.path.join(HERE, "bin")
 CXX = os.environ.get("CXX", "g++")
 HOST_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wextra", "-ffp-contract=off", "-fno-fast-math"]
-HOST_SOURCES = [os.path.join(HOST, f) for f in ("json.cpp", "model.cpp", "sweep.cpp")]
+HOST_SOURCES = [os.path.join(HOST, f) for f in ("json.cpp", "model.cpp", "runfiles.cpp")]
 
 
 def build_tools(force: bool = True) -> list[str]:
     """The command-line drop-ins, linked against the in-tree libsinksim_hip.so (rpath $ORIGIN/..)."""
     os.makedirs(BIN, exist_ok=True)
-    deps = HOST_SOURCES + [os.path.join(HOST, f) for f in ("json.hpp", "model.hpp", "sweep.hpp", "batch.hpp")] + [
-        os.path.join(APPS, "cli.hpp"), os.path.join(ROOT, "include", "sinksim_hip.h")]
+    deps = HOST_SOURCES + [os.path.join(HOST, f) for f in ("json.hpp", "model.hpp", "runfiles.hpp", "batch.hpp")] + [
+        os.path.join(APPS, f) for f in ("options.hpp", "runs.hpp")] + [os.path.join(ROOT, "include", "sinksim_hip.h")]
     out = []
     for app in ("sinksim_hip_run", "sinksim_sweep", "sinksim_hip_validate"):
         exe = os.path.join(BIN, app)

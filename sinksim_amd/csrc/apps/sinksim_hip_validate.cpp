@@ -16,20 +16,12 @@
 
 #include "../host/batch.hpp"
 #include "../host/json.hpp"
-#include "cli.hpp"
+#include "options.hpp"
 
 using namespace sinksim;
 namespace fs = std::filesystem;
 
 namespace {
-
-// hours and minutes as the reference prints them (apps/cli.hpp:70-75)
-std::string hm(double seconds) {
-  const int m = static_cast<int>(seconds / 60);
-  char buf[32];
-  std::snprintf(buf, sizeof buf, "%dh%02dm", m / 60, m % 60);
-  return buf;
-}
 
 struct Case {
   std::string id, label;
@@ -98,27 +90,31 @@ std::vector<Outcome> run_cases(const std::shared_ptr<const CompiledShip>& ship, 
 }  // namespace
 
 int main(int argc, char** argv) {
-  const cli::Args args = cli::Args::parse(argc, argv, {});
-  if (args.positional.empty()) {
+  const tools::CommandLine args(argc, argv, {});
+  if (!args.error().empty()) {
+    std::fprintf(stderr, "%s\n", args.error().c_str());
+    return 2;
+  }
+  if (args.positional().empty()) {
     std::fprintf(stderr, "usage: sinksim_hip_validate <catalog.json> [--numerics portable|portable32] [--compare validation.oracle.json] [--out validation.engine.json]\n");
     return 2;
   }
   bool mixed = false;
-  if (!cli::gpu_numerics(args, "sinksim_hip_validate", "portable", mixed)) return 2;
+  if (!tools::device_numerics(args, "sinksim_hip_validate", mixed, "portable")) return 2;
   try {
     std::string why;
     if (!hip::available(&why)) {
       std::fprintf(stderr, "sinksim_hip_validate: no usable HIP device (%s)\n", why.c_str());
       return 1;
     }
-    const fs::path catalogPath = args.positional[0];
+    const fs::path catalogPath = args.positional()[0];
     const std::string cat = catalogPath.string();
     const json::Value catalog = json::read_file(cat);
     if (json::string_or(catalog, "format", "") != "sinksim.catalog") throw std::runtime_error(cat + ": not a sinksim.catalog file");
     const fs::path dir = catalogPath.parent_path();
     const std::shared_ptr<const CompiledShip> ship = io::load_ship((dir / json::string(catalog, "shipFile", cat)).string());
     json::Value reference;
-    if (args.has("compare")) reference = json::read_file(args.get("compare", ""));
+    if (args.has("compare")) reference = json::read_file(args.text("compare", ""));
     const json::Value* refCases = reference.is_object() ? &json::member(reference, "cases", "reference table") : nullptr;
 
     std::vector<Case> cases;
@@ -146,7 +142,7 @@ int main(int argc, char** argv) {
       const Case& c = cases[i];
       const Outcome& r = outcome[i];
       std::printf("%-44s %s  trim %.2f\xC2\xB0  list %.2f\xC2\xB0  water %.0f t\n", c.label.c_str(),
-                  r.foundered ? ("FOUNDERS at " + hm(r.founderT)).c_str() : ("afloat (checked to " + hm(r.tEnd) + ")").c_str(), r.trimDeg,
+                  r.foundered ? ("FOUNDERS at " + tools::hours_minutes(r.founderT)).c_str() : ("afloat (checked to " + tools::hours_minutes(r.tEnd) + ")").c_str(), r.trimDeg,
                   r.listDeg, std::round(r.waterT));
       json::Value rec = json::Value::object();
       rec["label"] = c.label;
@@ -206,8 +202,8 @@ int main(int argc, char** argv) {
       doc["ship"] = ship->id;
       doc["producer"] = std::string("sinksim_hip_validate (HIP batch engine, numerics ") + (mixed ? "portable32" : "portable") + ")";
       doc["cases"] = std::move(outCases);
-      json::write_file(args.get("out", ""), doc, 2);
-      std::printf("wrote %s\n", args.get("out", "").c_str());
+      json::write_file(args.text("out", ""), doc, 2);
+      std::printf("wrote %s\n", args.text("out", "").c_str());
     }
     if (refCases) std::printf("RESULT: %s (%d mismatching cases)\n", mismatches == 0 ? "PASS" : "FAIL", mismatches);
     return mismatches == 0 ? 0 : 1;

@@ -244,54 +244,6 @@ namespace {
 
 
 template <class T>
-void need_ptr(const T* p, const char* what) {
-  need(p != nullptr, std::string("ss_batch_create: null ") + what);
-}
-
-// Range checks of every index the kernel dereferences, so that a malformed model is an error on the host.
-void validate(const ss_ship& S, const ss_sim& M) {
-  need(S.ncol >= 1 && S.nseg >= 0 && S.nnode >= 1, "ss_batch_create: empty ship");
-  need(M.nnode == S.nnode, "ss_batch_create: ship and simulation disagree on the node count");
-  need_ptr(S.x, "ship.x"); need_ptr(S.y, "ship.y"); need_ptr(S.dxdy, "ship.dxdy"); need_ptr(S.zlo, "ship.zlo"); need_ptr(S.zhi, "ship.zhi");
-  need_ptr(S.mu, "ship.mu"); need_ptr(S.vmax, "ship.vmax"); need_ptr(S.amin, "ship.amin"); need_ptr(S.aov, "ship.aov");
-  need_ptr(S.segStart, "ship.segStart"); need_ptr(S.segCount, "ship.segCount");
-  if (S.nseg > 0) { need_ptr(S.segCol, "ship.segCol"); need_ptr(S.segA, "ship.segA"); need_ptr(S.segE, "ship.segE"); }
-  for (int32_t j = 0; j < S.nseg; ++j) need(S.segCol[j] >= 0 && S.segCol[j] < S.ncol, "ss_batch_create: segment column out of range");
-  for (int32_t n = 0; n < S.nnode; ++n)
-    need(S.segStart[n] >= 0 && S.segCount[n] >= 0 && static_cast<int64_t>(S.segStart[n]) + S.segCount[n] <= S.nseg,
-         "ss_batch_create: node " + std::to_string(n) + " segment range exceeds the segment table");
-  need(M.dt > 0, "ss_batch_create: dt must be positive");
-  need(M.ngroup >= 0 && M.nconn >= 0 && M.nmonitor >= 0 && M.nmark >= 0, "ss_batch_create: negative count");
-  need_ptr(M.vol0, "sim.vol0"); need_ptr(M.level0, "sim.level0"); need_ptr(M.cx0, "sim.cx0"); need_ptr(M.cy0, "sim.cy0"); need_ptr(M.cz0, "sim.cz0");
-  if (M.ngroup > 0) {
-    need_ptr(M.gNodeStart, "sim.gNodeStart"); need_ptr(M.gNodeCount, "sim.gNodeCount"); need_ptr(M.gNodes, "sim.gNodes");
-    need_ptr(M.gMode, "sim.gMode"); need_ptr(M.gpx, "sim.gpx"); need_ptr(M.gpy, "sim.gpy"); need_ptr(M.gpz, "sim.gpz");
-  }
-  // every node must be solved by exactly one group, or its level is never set
-  std::vector<int> covered(S.nnode, 0);
-  for (int32_t g = 0; g < M.ngroup; ++g) {
-    need(M.gNodeCount[g] >= 1 && M.gNodeStart[g] >= 0, "ss_batch_create: empty group");
-    for (int32_t k = 0; k < M.gNodeCount[g]; ++k) {
-      const int32_t n = M.gNodes[M.gNodeStart[g] + k];
-      need(n >= 0 && n < S.nnode, "ss_batch_create: group member out of range");
-      covered[n]++;
-    }
-  }
-  for (int32_t n = 0; n < S.nnode; ++n) need(covered[n] == 1, "ss_batch_create: node " + std::to_string(n) + " is not in exactly one group");
-  if (M.nconn > 0) {
-    need_ptr(M.ca, "sim.ca"); need_ptr(M.cb, "sim.cb"); need_ptr(M.law, "sim.law"); need_ptr(M.cx, "sim.cx"); need_ptr(M.cy, "sim.cy");
-    need_ptr(M.cz, "sim.cz"); need_ptr(M.area, "sim.area"); need_ptr(M.coef, "sim.coef"); need_ptr(M.en, "sim.en"); need_ptr(M.tOn, "sim.tOn");
-    need_ptr(M.skipGroup, "sim.skipGroup"); need_ptr(M.monitor, "sim.monitor");
-  }
-  for (int32_t c = 0; c < M.nconn; ++c) {
-    need(M.ca[c] >= -1 && M.ca[c] < S.nnode && M.cb[c] >= 0 && M.cb[c] < S.nnode, "ss_batch_create: connection end out of range");
-    need(M.skipGroup[c] >= -1 && M.skipGroup[c] < M.ngroup, "ss_batch_create: connection skipGroup out of range");
-    need(M.monitor[c] >= -1 && M.monitor[c] < M.nmonitor, "ss_batch_create: connection monitor out of range");
-  }
-  if (M.nmark > 0) { need_ptr(M.mkx, "sim.mkx"); need_ptr(M.mky, "sim.mky"); need_ptr(M.mkz, "sim.mkz"); need_ptr(M.mkWhen, "sim.mkWhen"); }
-}
-
-template <class T>
 std::vector<T> repeat(const T* v, size_t n, size_t times) {
   std::vector<T> out;
   out.reserve(n * times);
@@ -537,7 +489,7 @@ ss_status ss_batch_create(const ss_ship* ship, const ss_sim* sim, const ss_batch
     need(ship != nullptr, "ss_batch_create: null ship");
     need(sim != nullptr, "ss_batch_create: null simulation");
     check_options(opt, "ss_batch_create");
-    validate(*ship, *sim);
+    validate(*ship, *sim, "ss_batch_create");
     std::string why;
     if (device_count(&why) == 0) fail(SS_RUNTIME, "ss_batch_create: " + why);
     std::vector<std::shared_ptr<const Model>> models;
@@ -555,7 +507,7 @@ ss_status ss_batch_create_fleet(const ss_ship* ships, const ss_sim* sims, int32_
     need(nmodels >= 1, "ss_batch_create_fleet: at least one model");
     check_options(opt, "ss_batch_create_fleet");
     need(opt->traced == 0 || nmodels == 1, "ss_batch_create_fleet: traces are recorded for single-model batches only");
-    for (int32_t k = 0; k < nmodels; ++k) validate(ships[k], sims[k]);
+    for (int32_t k = 0; k < nmodels; ++k) validate(ships[k], sims[k], "ss_batch_create_fleet");
     std::vector<int32_t> im(instModel, instModel + opt->count);
     for (int32_t m : im) need(m >= 0 && m < nmodels, "ss_batch_create_fleet: instance model out of range");
     std::string why;

@@ -254,6 +254,54 @@ inline void check_instance(const ss_batch* b, int32_t i) {
   if (i < 0 || i >= b->opt.count) fail(SS_OUT_OF_RANGE, "instance " + std::to_string(i) + " out of range");
 }
 
+template <class T>
+inline void need_ptr(const T* p, const std::string& who, const char* what) {
+  need(p != nullptr, who + ": null " + what);
+}
+
+// Range checks of every index the kernel dereferences, so that a malformed model is an error on the host.
+inline void validate(const ss_ship& S, const ss_sim& M, const std::string& who) {
+  need(S.ncol >= 1 && S.nseg >= 0 && S.nnode >= 1, who + ": empty ship");
+  need(M.nnode == S.nnode, who + ": ship and simulation disagree on the node count");
+  need_ptr(S.x, who, "ship.x"); need_ptr(S.y, who, "ship.y"); need_ptr(S.dxdy, who, "ship.dxdy"); need_ptr(S.zlo, who, "ship.zlo"); need_ptr(S.zhi, who, "ship.zhi");
+  need_ptr(S.mu, who, "ship.mu"); need_ptr(S.vmax, who, "ship.vmax"); need_ptr(S.amin, who, "ship.amin"); need_ptr(S.aov, who, "ship.aov");
+  need_ptr(S.segStart, who, "ship.segStart"); need_ptr(S.segCount, who, "ship.segCount");
+  if (S.nseg > 0) { need_ptr(S.segCol, who, "ship.segCol"); need_ptr(S.segA, who, "ship.segA"); need_ptr(S.segE, who, "ship.segE"); }
+  for (int32_t j = 0; j < S.nseg; ++j) need(S.segCol[j] >= 0 && S.segCol[j] < S.ncol, who + ": segment column out of range");
+  for (int32_t n = 0; n < S.nnode; ++n)
+    need(S.segStart[n] >= 0 && S.segCount[n] >= 0 && static_cast<int64_t>(S.segStart[n]) + S.segCount[n] <= S.nseg,
+         who + ": node " + std::to_string(n) + " segment range exceeds the segment table");
+  need(M.dt > 0, who + ": dt must be positive");
+  need(M.ngroup >= 0 && M.nconn >= 0 && M.nmonitor >= 0 && M.nmark >= 0, who + ": negative count");
+  need_ptr(M.vol0, who, "sim.vol0"); need_ptr(M.level0, who, "sim.level0"); need_ptr(M.cx0, who, "sim.cx0"); need_ptr(M.cy0, who, "sim.cy0"); need_ptr(M.cz0, who, "sim.cz0");
+  if (M.ngroup > 0) {
+    need_ptr(M.gNodeStart, who, "sim.gNodeStart"); need_ptr(M.gNodeCount, who, "sim.gNodeCount"); need_ptr(M.gNodes, who, "sim.gNodes");
+    need_ptr(M.gMode, who, "sim.gMode"); need_ptr(M.gpx, who, "sim.gpx"); need_ptr(M.gpy, who, "sim.gpy"); need_ptr(M.gpz, who, "sim.gpz");
+  }
+  // every node must be solved by exactly one group, or its level is never set
+  std::vector<int> covered(S.nnode, 0);
+  for (int32_t g = 0; g < M.ngroup; ++g) {
+    need(M.gNodeCount[g] >= 1 && M.gNodeStart[g] >= 0, who + ": empty group");
+    for (int32_t k = 0; k < M.gNodeCount[g]; ++k) {
+      const int32_t n = M.gNodes[M.gNodeStart[g] + k];
+      need(n >= 0 && n < S.nnode, who + ": group member out of range");
+      covered[n]++;
+    }
+  }
+  for (int32_t n = 0; n < S.nnode; ++n) need(covered[n] == 1, who + ": node " + std::to_string(n) + " is not in exactly one group");
+  if (M.nconn > 0) {
+    need_ptr(M.ca, who, "sim.ca"); need_ptr(M.cb, who, "sim.cb"); need_ptr(M.law, who, "sim.law"); need_ptr(M.cx, who, "sim.cx"); need_ptr(M.cy, who, "sim.cy");
+    need_ptr(M.cz, who, "sim.cz"); need_ptr(M.area, who, "sim.area"); need_ptr(M.coef, who, "sim.coef"); need_ptr(M.en, who, "sim.en"); need_ptr(M.tOn, who, "sim.tOn");
+    need_ptr(M.skipGroup, who, "sim.skipGroup"); need_ptr(M.monitor, who, "sim.monitor");
+  }
+  for (int32_t c = 0; c < M.nconn; ++c) {
+    need(M.ca[c] >= -1 && M.ca[c] < S.nnode && M.cb[c] >= 0 && M.cb[c] < S.nnode, who + ": connection end out of range");
+    need(M.skipGroup[c] >= -1 && M.skipGroup[c] < M.ngroup, who + ": connection skipGroup out of range");
+    need(M.monitor[c] >= -1 && M.monitor[c] < M.nmonitor, who + ": connection monitor out of range");
+  }
+  if (M.nmark > 0) { need_ptr(M.mkx, who, "sim.mkx"); need_ptr(M.mky, who, "sim.mky"); need_ptr(M.mkz, who, "sim.mkz"); need_ptr(M.mkWhen, who, "sim.mkWhen"); }
+}
+
 // The forward kernel's LDS bytes per simulation: the largest model's layout at the batch's workgroup size.
 inline size_t forward_lds_bytes(const ss_batch& b) {
   size_t n = 0;

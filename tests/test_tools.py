@@ -1,5 +1,6 @@
-"""The C++ command-line drop-ins on the host (no GPU): usage errors and exit codes, numerics refusals, and the exact
-results comparison, cross-checked against the reference's own sinksim_sweep --compare where that build exists."""
+"""The C++ command-line drop-ins on the host (no GPU): usage errors and exit codes, numerics refusals, the exact results
+comparison, and sinksim_sweep's CPU runs against the reference's own sinksim_sweep (built from its sources in
+oracle/_ref) in all four numerics, compared value for value by both tools' --compare."""
 import os
 import subprocess
 
@@ -29,12 +30,26 @@ def _run(*cmd):
 def test_usage_errors_exit_2():
     assert _run(_tool("sinksim_hip_run")).returncode == 2
     assert _run(_tool("sinksim_sweep")).returncode == 2
-    r = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), "x.json", "--numerics", "oracle")
-    assert r.returncode == 2 and "CPU only" in r.stderr
+    assert _run(_tool("sinksim_sweep"), "--compare").returncode == 2
+    r = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), "x.json", "--numerics", "quad")
+    assert r.returncode == 2 and "unknown --numerics" in r.stderr
+    r = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), "x.json", "--numerics", "oracle", "--device", "gpu")
+    assert r.returncode == 2 and "CPU" in r.stderr
+    r = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), "x.json", "--threads", "many")
+    assert r.returncode == 2 and "needs a number" in r.stderr
     r = _run(_tool("sinksim_hip_run"), SHIP, sim_path("titanic"), "--precision", "quad")
     assert r.returncode == 2 and "unknown --precision" in r.stderr
+    r = _run(_tool("sinksim_hip_run"), SHIP, sim_path("titanic"), "--numerics", "oracle")
+    assert r.returncode == 2 and "CPU only" in r.stderr
     r = _run(_tool("sinksim_hip_run"), SHIP, sim_path("titanic"), "--count")
     assert r.returncode == 2 and "needs a value" in r.stderr
+
+
+def test_missing_files_exit_1(tmp_path):
+    r = _run(_tool("sinksim_sweep"), os.path.join(tmp_path, "none.json"), sim_path("titanic"), "x.json")
+    assert r.returncode == 1 and r.stderr.startswith("sinksim_sweep: ")
+    r = _run(_tool("sinksim_sweep"), "--compare", os.path.join(tmp_path, "a.json"), os.path.join(tmp_path, "b.json"))
+    assert r.returncode == 1
 
 
 def _results(titanic, count=3):
@@ -110,3 +125,53 @@ def test_validate_tool_usage_and_numerics():
     assert _run(_tool("sinksim_hip_validate")).returncode == 2
     from sinksim_amd.data_paths import CATALOG
     assert _run(_tool("sinksim_hip_validate"), CATALOG, "--numerics", "oracle").returncode == 2
+
+
+def _header_and_tail(text):
+    lines = text.splitlines()
+    return lines[0], [l for l in lines[2:] if l.startswith("  ")]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SWEEP), reason="reference CPU build absent (make -C oracle ref)")
+@pytest.mark.parametrize("numerics", ["oracle", "portable", "portable32", "std"])
+def test_sinksim_sweep_is_the_reference_cpu_tool(tmp_path, numerics):
+    """sinksim_sweep on the reference's smoke sweep on the host CPU: the same header and instance lines as the
+    reference's sinksim_sweep, and results IDENTICAL by both tools' --compare (the sweep file, apply order, curves,
+    events, final readouts and step counts; tests/CMakeLists.txt:36-39 runs this sweep as the reference's fixture)."""
+    from conftest import GOLDEN
+    sw = os.path.join(GOLDEN, "smoke.sweep.json")
+    mine, ref = os.path.join(tmp_path, "mine.json"), os.path.join(tmp_path, "ref.json")
+    a = _run(_tool("sinksim_sweep"), SHIP, sim_path("titanic"), sw, "--numerics", numerics, "--threads", "4", "--results", mine)
+    assert a.returncode == 0, a.stderr
+    b = _run(REF_SWEEP, SHIP, sim_path("titanic"), sw, "--numerics", numerics, "--threads", "4", "--results", ref)
+    assert b.returncode == 0, b.stderr
+    assert _header_and_tail(a.stdout) == _header_and_tail(b.stdout)
+    for tool in (_tool("sinksim_sweep"), REF_SWEEP):
+        c = _run(tool, "--compare", ref, mine)
+        assert c.returncode == 0 and "RESULT: IDENTICAL" in c.stdout, c.stdout
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SWEEP), reason="reference CPU build absent (make -C oracle ref)")
+def test_sinksim_sweep_default_numerics_and_stop_when_stable(tmp_path, titanic):
+    """Without --numerics both tools run the oracle numerics; a sweep with stopWhenStable (the calm rule, CPU-only in
+    the reference) over variants of the britannicClosed scenario, which settles afloat and stops on the rule, ends every
+    instance at the same step in both tools."""
+    import json
+    from sinksim_amd.data_paths import titanic as load
+    ship, sim = load("britannicClosed")
+    doc = {"format": "sinksim.sweep", "formatVersion": 1, "ship": ship.id, "sim": sim.id, "tMax": 4 * 3600.0, "every": 60.0,
+           "stopWhenStable": True,
+           "instances": [{"id": "as-compiled"}, {"id": "smaller-breach", "scale": {"breach": 0.7}},
+                         {"id": "larger-breach", "scale": {"breach": 1.3}}]}
+    sw = os.path.join(tmp_path, "calm.sweep.json")
+    with open(sw, "w") as f:
+        json.dump(doc, f)
+    mine, ref = os.path.join(tmp_path, "mine.json"), os.path.join(tmp_path, "ref.json")
+    a = _run(_tool("sinksim_sweep"), SHIP, sim_path("britannicClosed"), sw, "--results", mine)
+    b = _run(REF_SWEEP, SHIP, sim_path("britannicClosed"), sw, "--results", ref)
+    assert a.returncode == 0 and b.returncode == 0, a.stderr + b.stderr
+    assert "numerics oracle" in a.stdout.splitlines()[0]
+    c = _run(REF_SWEEP, "--compare", ref, mine)
+    assert c.returncode == 0, c.stdout
+    r = S.load_results(mine)
+    assert any(not i.foundered and i.tEnd < 4 * 3600.0 for i in r.instances), "no instance stopped on the calm rule"

@@ -52,8 +52,10 @@ __device__ __forceinline__ void write_record(SSG double* rec, const Blk& B, int 
 // One instance on one workgroup: load it into LDS, step it until it founders, reaches tMax or has taken maxSteps
 // steps in this launch, store it back. Traces and curves are recorded as in the reference
 // (device_step.cuh:379-416); the readouts of the state at the end of the launch go to final7.
+// Inlined into the kernel: as a called function it saved and restored its callee-saved registers in scratch memory
+// for every instance.
 template <bool Mixed, int TPB>
-__device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double tMax, SSL double* lds, int tid) {
+__device__ __attribute__((always_inline)) void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double tMax, SSL double* lds, int tid) {
   const int NN = P.S.nn, NC = P.M.nc, evCap = P.M.evCap, nmon = P.M.nmon, nmark = P.M.nmark;
   const size_t iN = static_cast<size_t>(Bt.nodeOff[inst]), iC = static_cast<size_t>(Bt.connOff[inst]);
   const size_t iE = static_cast<size_t>(Bt.evOff[inst]), iMon = static_cast<size_t>(Bt.monOff[inst]), iMark = static_cast<size_t>(Bt.markOff[inst]);
@@ -136,8 +138,8 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
     // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
     if (curves && Bq.scal[S_T] >= Bq.scal[S_NEXTCURVE] && Bq.scal[S_CURVECOUNT] < Bt.curveMax) {
       if (tid == 0) {
-        SSG double* rec = Bt.curveRec + (static_cast<size_t>(inst) * Bt.curveMax + static_cast<size_t>(Bq.scal[S_CURVECOUNT])) * kCurveValues;
-        readouts7(Q, Bq.scal, Bq.vol, rec);
+        SSG double* rec = Bt.curveRec + (static_cast<size_t>(inst) * Bt.curveMax + static_cast<size_t>(Bq.scal[S_CURVECOUNT])) * kCurveStride;
+        readouts7(Q, Bq.scal, Bq.vol, rec, 1);
       }
       __syncthreads();
       if (tid == 0) {
@@ -189,7 +191,7 @@ __device__ void run_instance(PC& P, BC& Bt, int inst, long long maxSteps, double
       Bt.nextCurve[inst] = B.scal[S_NEXTCURVE];
       Bt.curveCount[inst] = static_cast<int32_t>(B.scal[S_CURVECOUNT]);
     }
-    readouts7(P, B.scal, B.vol, Bt.final7 + static_cast<size_t>(inst) * 7);
+    readouts7(P, B.scal, B.vol, Bt.final7 + static_cast<size_t>(inst) * 7, 0);
   }
   for (int n = tid; n < NN; n += TPB) {
     Bt.vol[iN + n] = B.vol[n]; Bt.level[iN + n] = B.level[n];
@@ -589,7 +591,7 @@ void ssa::upload(ss_batch* b) {
   b->denseCount.fill(0, traced); b->sampleCount.fill(0, traced); b->nextSample.fill(b->opt.sampleEvery, traced);
   if (b->opt.curveEvery > 0) {
     b->curveMax = static_cast<int32_t>(std::floor(b->opt.tMax / b->opt.curveEvery)) + 2;
-    b->curveRec.alloc(count * static_cast<size_t>(b->curveMax) * kCurveValues);
+    b->curveRec.alloc(count * static_cast<size_t>(b->curveMax) * kCurveStride);
     b->curveCount.fill(0, count);
     b->nextCurve.fill(0.0, count);  // the first record is the initial state, as in the oracle's run
   }
@@ -967,7 +969,10 @@ ss_status ss_batch_curve(const ss_batch* b, int32_t instance, double* rec, int32
     if (count) *count = n + 1;
     if (rec && cap > 0) {
       const int32_t keep = std::min(n, cap);
-      b->curveRec.download(rec, static_cast<size_t>(keep) * kCurveValues, i * static_cast<size_t>(b->curveMax) * kCurveValues);
+      std::vector<double> padded(static_cast<size_t>(keep) * kCurveStride);
+      b->curveRec.download(padded.data(), padded.size(), i * static_cast<size_t>(b->curveMax) * kCurveStride);
+      for (int32_t k = 0; k < keep; ++k)
+        std::copy_n(padded.begin() + static_cast<std::ptrdiff_t>(k) * kCurveStride, kCurveValues, rec + static_cast<size_t>(k) * kCurveValues);
       if (cap > n) std::copy(b->rFinal.begin() + i * 7, b->rFinal.begin() + i * 7 + 7, rec + static_cast<size_t>(n) * kCurveValues);
     }
   });

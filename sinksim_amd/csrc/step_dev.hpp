@@ -38,6 +38,7 @@ namespace ssa {
 constexpr int kThreads = 256;  // logical block of the canonical buoyancy order (the first 256 threads of a workgroup)
 constexpr int kLanes = 32;     // logical warp of the canonical trees
 constexpr int kUnits = kThreads / kLanes;  // buoyancy's 32-lane groups
+constexpr int kCurveStride = 8;             // doubles per device curve record: 7 readouts and a pad
 
 // scheme v1 (reference: engine/include/sinksim/kernel/scheme.hpp:11-28)
 constexpr double kEmptyVolume = 1e-6;
@@ -145,7 +146,7 @@ struct DevBatch {
   SSG double* nextSample;
   double curveEvery;
   int32_t curveMax;
-  SSG double* curveRec;
+  SSG double* curveRec;  // kCurveStride doubles per record (64-byte aligned)
   SSG int32_t* curveCount;
   SSG double* nextCurve;
   int32_t stopWhenStable;  // stop an instance after 20 min of calm, once past 30 min (simulation.cpp:179-182)
@@ -1042,22 +1043,25 @@ __device__ inline void node_update(PC& P, const Blk& B, int n) {
 
 // ------------------------------------------------------------------------------------------- readouts
 
-// readouts.hpp:175-191 (7 values: t trim list water(t) inflow(t/min) draftF draftA), one thread; it takes the LDS
-// arrays it reads rather than the block's view, which a call would have to copy to scratch
-__device__ __noinline__ void readouts7(PC& P, const SSL double* scal, const SSL double* vol, SSG double* r) {
+// readouts.hpp:20-36 (7 values: t trim list water(t) inflow(t/min) draftF draftA), one thread. A curve record is padded
+// to 8 doubles and written as four 16-byte stores into its own 64-byte line (padded = 1; r 64-byte aligned); the final
+// readouts go out as 7 values. It takes the LDS arrays it reads rather than the block's view, which a call would have
+// to copy to scratch.
+__device__ __noinline__ void readouts7(PC& P, const SSL double* scal, const SSL double* vol, SSG double* r, int padded) {
   const double t = scal[S_T], zO = scal[S_ZO], pitch = scal[S_PITCH], roll = scal[S_ROLL];
   const double cth = trig(pitch, 1), sth = trig(pitch, 0), cph = trig(roll, 1), sph = trig(roll, 0);
   const double R20 = -sth, R21 = cth * sph, R22 = cth * cph;
   const double tz = zO - (R20 * P.M.g0x + R21 * P.M.g0y + R22 * P.M.g0z);
   double water = 0;
   for (int n = 0; n < P.S.nn; ++n) water += vol[n];
-  r[0] = t;
-  r[1] = pitch * 180 / kPi;
-  r[2] = roll * 180 / kPi;
-  r[3] = water * P.M.rho / 1000;
-  r[4] = scal[O_INFLOW] * P.M.rho / 1000 * 60;
-  r[5] = -(R20 * P.M.xFP + R21 * 0.0 + R22 * 0.0 + tz);
-  r[6] = -(R20 * P.M.xAP + R21 * 0.0 + R22 * 0.0 + tz);
+  const double v[7] = {t, pitch * 180 / kPi, roll * 180 / kPi, water * P.M.rho / 1000, scal[O_INFLOW] * P.M.rho / 1000 * 60,
+                       -(R20 * P.M.xFP + R21 * 0.0 + R22 * 0.0 + tz), -(R20 * P.M.xAP + R21 * 0.0 + R22 * 0.0 + tz)};
+  if (padded) {
+    SSG d2v* q = reinterpret_cast<SSG d2v*>(r);
+    q[0] = d2v{v[0], v[1]}; q[1] = d2v{v[2], v[3]}; q[2] = d2v{v[4], v[5]}; q[3] = d2v{v[6], 0.0};
+  } else {
+    for (int k = 0; k < 7; ++k) r[k] = v[k];
+  }
 }
 
 __device__ inline void push_event(PC& P, const Blk& B, double t, int id, int kind) {

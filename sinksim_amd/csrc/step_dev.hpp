@@ -590,54 +590,355 @@ __device__ __forceinline__ Pass get_pass(const SSL double* Pp, int k) {
   return Pass{q[0], q[1], q[2], q[3], q[4], q[5], q[6]};
 }
 
-// The level solve of one free-surface group as a sequence of node passes (levels.hpp:13-194 with full = false),
-// restated as a state machine so that the two logical groups of a wave issue each pass together: a state names
-// the pass it waits for, and advance() takes the pass result to the next state. Every lane of a unit holds the
-// same state.
-enum : int {
-  L_START = 0,  // begin the next group of this unit
-  L_DONE,
-  L_S_FIRST,    // single node: the pass at the node's level (with its extent)
-  L_S_FULL,     // single node: the pass at the top of a full node
-  L_S_CLAMP,    // single node: the pass at the level clamped into the extent
-  L_S_NEWTON,   // single node: a Newton pass
-  L_G_FIRST,    // merged group: member k's pass at the common level (with its extent)
-  L_G_FULL,     // merged group: member k's pass at the top of a full group
-  L_G_CLAMP,    // merged group: member k's pass at the clamped level
-  L_G_NEWTON,   // merged group: member k's Newton pass
+// The level solve (levels.hpp:13-194 with full = false), one free-surface group per wave. The waves of a workgroup
+// take groups from an LDS counter. A merged group's state is the same in every lane: its member passes run two per
+// round, member k on half k & 1 of the wave at the group's common level, and the Newton update, clamps and the share
+// between the members run once per wave under uniform (scalar) branches. An unmerged group's members are solved singly,
+// members 0, 2, 4, ... on half 0 and 1, 3, 5, ... on half 1, each half running its own single-node solve; both halves
+// issue their passes together. Every pass, comparison and sum is the reference's, in the reference's order.
+
+// a condition every lane of the wave holds alike, as a scalar branch
+__device__ __forceinline__ bool uni(bool c) { return __builtin_amdgcn_readfirstlane(c ? 1 : 0) != 0; }
+
+// issue priority only (no value changes): after its first round a solving wave goes ahead of the other instances'
+// waves in their other phases on its SIMD, and after five rounds (a straggler, which the step's barrier waits for)
+// further ahead
+__device__ __forceinline__ void solve_round(int& round) {
+  ++round;
+  if (round == 1) __builtin_amdgcn_s_setprio(1);
+  if (round == 5) __builtin_amdgcn_s_setprio(2);
+}
+
+// lane L's value of v, the same in every lane (held in scalar registers)
+template <int L>
+__device__ __forceinline__ double from_lane(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), L));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), L));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+template <int L>
+__device__ __forceinline__ Pass pass_from_lane(const Pass& p) {
+  return Pass{from_lane<L>(p.vol), from_lane<L>(p.dv), from_lane<L>(p.mx), from_lane<L>(p.my), from_lane<L>(p.mz), from_lane<L>(p.zmin),
+              from_lane<L>(p.zmax)};
+}
+
+// The passes of the two members of a merged pair at level h in one round, member k on half k: both results, the same in
+// every lane (with Ext also each member's extent into zmin / zmax).
+template <bool Mixed, bool Ext>
+__device__ __forceinline__ void pair_passes(PC& P, const Blk& B, int base, double h, int lane, int half, int& round, Pass& p0, Pass& p1) {
+  const int n = B.gNodes[base + half];
+  const Pass p = node_pass<Mixed, Ext>(P, B, lane, n, h);
+  if constexpr (Ext)
+    if (lane == 0) { B.zmin[n] = p.zmin; B.zmax[n] = p.zmax; }
+  p0 = pass_from_lane<0>(p);
+  p1 = pass_from_lane<32>(p);
+  solve_round(round);
+}
+
+// levels.hpp:54-162 for a merged pair (N = 2, the common case), the members' pass results in registers.
+template <bool Mixed>
+__device__ void solve_pair(PC& P, const Blk& B, int base, int lane, int half, int& round) {
+  const bool lead = (lane | half) == 0;
+  const int m0 = B.gNodes[base], m1 = B.gNodes[base + 1];
+  const double V = (0.0 + B.vol[m0]) + B.vol[m1], Vmax = (0.0 + B.nVmax[m0]) + B.nVmax[m1];
+  const bool w0 = B.vol[m0] > 0, w1 = B.vol[m1] > 0;
+  double h;
+  if (uni(w0 && w1)) h = ((0.0 + B.level[m0]) + B.level[m1]) / 2.0;
+  else h = B.level[w0 ? m0 : m1];
+  Pass p0, p1;
+  pair_passes<Mixed, true>(P, B, base, h, lane, half, round, p0, p1);
+  double lo = p0.zmin, hi = p0.zmax;
+  if (p1.zmin < lo) lo = p1.zmin;
+  if (p1.zmax > hi) hi = p1.zmax;
+  if (uni(V <= kEmptyVolume)) {
+    if (lead) { B.vol[m0] = 0; B.level[m0] = p0.zmin; B.area[m0] = 0; B.vol[m1] = 0; B.level[m1] = p1.zmin; B.area[m1] = 0; }
+    return;
+  }
+  if (uni(V >= Vmax * kFullFraction)) {
+    const double over = jmax(0.0, V - Vmax);
+    const double dh = over / ((0.0 + B.nAov[m0]) + B.nAov[m1]);
+    if (lead) {
+      const double v = B.nVmax[m0] + dh * B.nAov[m0];
+      B.vol[m0] = v;
+      B.vol[m1] = V - (0.0 + v);
+    }
+    pair_passes<Mixed, false>(P, B, base, hi, lane, half, round, p0, p1);
+    if (lead) {
+      B.level[m0] = hi + dh; B.area[m0] = B.nAov[m0]; put_centroid(B, m0, p0);
+      B.level[m1] = hi + dh; B.area[m1] = B.nAov[m1]; put_centroid(B, m1, p1);
+    }
+    return;
+  }
+  const double tol = jmax(1e-3 * Vmax, 0.5);
+  double a = lo, b = hi;
+  if (uni(!(h >= lo && h <= hi))) {
+    h = jmin(hi, jmax(lo, h));
+    pair_passes<Mixed, false>(P, B, base, h, lane, half, round, p0, p1);
+  }
+  for (int it = 0;; ++it) {
+    const double err = ((0.0 + p0.vol) + p1.vol) - V;
+    if (uni(it >= kNewtonFast || __builtin_fabs(err) < tol)) break;
+    if (err > 0) b = h; else a = h;
+    const double dv = (0.0 + p0.dv) + p1.dv;
+    double hn = dv > kSlopeMin ? h - err / dv : 0.5 * (a + b);
+    if (!(hn > a && hn < b)) hn = 0.5 * (a + b);
+    h = hn;
+    pair_passes<Mixed, false>(P, B, base, h, lane, half, round, p0, p1);
+  }
+  if (lead) {
+    const double tot = (0.0 + p0.vol) + p1.vol;
+    double v = tot > 0 ? V * p0.vol / tot : V / 2.0;
+    if (v > B.nVmax[m0]) v = B.nVmax[m0];
+    double assigned = 0.0 + v;
+    if (V - assigned > B.nVmax[m1]) {
+      // the cascade of levels.hpp:136-155 for two members: the first takes V - vmax(last), uncapped
+      v = (V - B.nVmax[m1] - 0.0) - 0.0;
+      assigned = 0.0 + v;
+    }
+    B.vol[m0] = v;
+    B.vol[m1] = V - assigned;
+    B.level[m0] = h; B.area[m0] = p0.dv; put_centroid(B, m0, p0);
+    B.level[m1] = h; B.area[m1] = p1.dv; put_centroid(B, m1, p1);
+  }
+}
+
+// The passes of members 0..N-1 of a merged group at level h, two per round (member k on half k & 1), into the wave's
+// pass scratch Pp (with Ext also each member's extent into zmin / zmax). The lanes of each half hold their pass's result;
+// its lane 0 stores it, and the wave reads the scratch back after the compiler barrier, in its own order of LDS accesses.
+template <bool Mixed, bool Ext>
+__device__ __forceinline__ void member_passes(PC& P, const Blk& B, SSL double* Pp, int base, int N, double h, int lane, int half, int& round) {
+  for (int k0 = 0; k0 < N; k0 += 2) {
+    const int k = k0 + half;
+    const int n = k < N ? B.gNodes[base + k] : -1;
+    const Pass p = node_pass<Mixed, Ext>(P, B, lane, n, h);
+    if (lane == 0 && k < N) {
+      put_pass(Pp, k, p);
+      if constexpr (Ext) { B.zmin[n] = p.zmin; B.zmax[n] = p.zmax; }
+    }
+    solve_round(round);
+  }
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// levels.hpp:54-162: the members of a merged group under one free surface. Every value here is the same in every lane.
+template <bool Mixed>
+__device__ void solve_merged(PC& P, const Blk& B, SSL double* Pp, int base, int N, int lane, int half, int& round) {
+  const bool lead = (lane | half) == 0;  // lane 0 of the wave writes the results
+  double V = 0, Vmax = 0;
+  for (int k = 0; k < N; ++k) { V += B.vol[B.gNodes[base + k]]; Vmax += B.nVmax[B.gNodes[base + k]]; }
+  // the common starting level: the members' mean level when all are wet, else the first wet member's
+  bool allWet = true;
+  int firstWet = -1;
+  for (int k = 0; k < N; ++k) {
+    if (B.vol[B.gNodes[base + k]] > 0) { if (firstWet < 0) firstWet = k; }
+    else allWet = false;
+  }
+  double h;
+  if (uni(allWet)) {
+    double s = 0;
+    for (int k = 0; k < N; ++k) s += B.level[B.gNodes[base + k]];
+    h = s / static_cast<double>(N);
+  } else {
+    h = B.level[B.gNodes[base + (firstWet >= 0 ? firstWet : N - 1)]];
+  }
+  member_passes<Mixed, true>(P, B, Pp, base, N, h, lane, half, round);
+  double lo = Pp[5], hi = Pp[6];
+  for (int k = 1; k < N; ++k) {
+    const double zl = Pp[7 * k + 5], zh = Pp[7 * k + 6];
+    if (zl < lo) lo = zl;
+    if (zh > hi) hi = zh;
+  }
+  if (uni(V <= kEmptyVolume)) {
+    if (lead)
+      for (int k = 0; k < N; ++k) { const int m = B.gNodes[base + k]; B.vol[m] = 0; B.level[m] = Pp[7 * k + 5]; B.area[m] = 0; }
+    return;
+  }
+  if (uni(V >= Vmax * kFullFraction)) {
+    const double over = jmax(0.0, V - Vmax);
+    double aSum = 0;
+    for (int k = 0; k < N; ++k) aSum += B.nAov[B.gNodes[base + k]];
+    const double dh = over / aSum;
+    if (lead) {
+      double assigned = 0;
+      for (int k = 0; k < N - 1; ++k) {
+        const int m = B.gNodes[base + k];
+        const double v = B.nVmax[m] + dh * B.nAov[m];
+        B.vol[m] = v;
+        assigned += v;
+      }
+      B.vol[B.gNodes[base + N - 1]] = V - assigned;
+    }
+    member_passes<Mixed, false>(P, B, Pp, base, N, hi, lane, half, round);
+    if (lead)
+      for (int k = 0; k < N; ++k) {
+        const int m = B.gNodes[base + k];
+        B.level[m] = hi + dh; B.area[m] = B.nAov[m];
+        put_centroid(B, m, get_pass(Pp, k));
+      }
+    return;
+  }
+  const double tol = jmax(1e-3 * Vmax, 0.5);
+  double a = lo, b = hi;
+  if (uni(!(h >= lo && h <= hi))) {
+    h = jmin(hi, jmax(lo, h));
+    member_passes<Mixed, false>(P, B, Pp, base, N, h, lane, half, round);
+  }
+  for (int it = 0;; ++it) {
+    double sum = 0;
+    for (int k = 0; k < N; ++k) sum += Pp[7 * k];
+    const double err = sum - V;
+    if (uni(it >= kNewtonFast || __builtin_fabs(err) < tol)) break;
+    if (err > 0) b = h; else a = h;
+    double dv = 0;
+    for (int k = 0; k < N; ++k) dv += Pp[7 * k + 1];
+    double hn = dv > kSlopeMin ? h - err / dv : 0.5 * (a + b);
+    if (!(hn > a && hn < b)) hn = 0.5 * (a + b);
+    h = hn;
+    member_passes<Mixed, false>(P, B, Pp, base, N, h, lane, half, round);
+  }
+  // levels.hpp:125-161: the water shared in proportion to the geometry at the final level, capped by each member's
+  // capacity; when the last member would overflow, the members before it take V - vmax(last), filled from the one
+  // nearest the end
+  if (lead) {
+    double tot = 0;
+    for (int k = 0; k < N; ++k) tot += Pp[7 * k];
+    double assigned = 0;
+    for (int k = 0; k < N - 1; ++k) {
+      const int m = B.gNodes[base + k];
+      double v = tot > 0 ? V * Pp[7 * k] / tot : V / static_cast<double>(N);
+      if (v > B.nVmax[m]) v = B.nVmax[m];
+      B.vol[m] = v;
+      assigned += v;
+    }
+    const double vmaxLast = B.nVmax[B.gNodes[base + N - 1]];
+    if (V - assigned > vmaxLast) {
+      const double need = V - vmaxLast;
+      double capped = 0;
+      for (int k = N - 2; k >= 0; --k) {
+        double prior = 0;
+        for (int j = 0; j < k; ++j) prior += B.vol[B.gNodes[base + j]];
+        const int m = B.gNodes[base + k];
+        const double v = (need - capped) - prior;
+        if (v > B.nVmax[m] && k > 0) {
+          B.vol[m] = B.nVmax[m];
+          capped += B.nVmax[m];
+          continue;
+        }
+        B.vol[m] = v;
+        break;
+      }
+      assigned = 0;
+      for (int k = 0; k < N - 1; ++k) assigned += B.vol[B.gNodes[base + k]];
+    }
+    B.vol[B.gNodes[base + N - 1]] = V - assigned;
+    for (int k = 0; k < N; ++k) {
+      const int m = B.gNodes[base + k];
+      const Pass p = get_pass(Pp, k);
+      B.level[m] = h; B.area[m] = p.dv;
+      put_centroid(B, m, p);
+    }
+  }
+}
+
+// One single-node solve (levels.hpp:13-49) per half of a wave, as a state machine whose state names the pass it waits
+// for: the first pass at the node's level (with its extent), the pass at the top of a full node, or a Newton pass.
+enum : int { LV_DONE = 0, LV_FIRST, LV_FULL, LV_NEWTON };
+
+struct Single {
+  int st, n, it;
+  double V, Vmax, h, a, b, tol, lo, hi;
 };
 
-struct LevelSolve {
-  int st, g, N, base, k, it;
-  double V, Vmax, h, a, b, tol, lo, hi, dh;
-};
+__device__ __forceinline__ void single_begin(const Blk& B, Single& S, int n) {
+  S.n = n;
+  S.st = n >= 0 ? LV_FIRST : LV_DONE;
+  if (n < 0) return;
+  S.V = B.vol[n]; S.Vmax = B.nVmax[n]; S.h = B.level[n];
+}
 
-struct Solver {
-  PC& P;
-  const Blk& B;
-  SSL double* Pp;
-  bool lead;
-
-  __device__ __forceinline__ int member(const LevelSolve& S, int k) const { return B.gNodes[S.base + k]; }
-
-  __device__ __forceinline__ void start_single(LevelSolve& S) const {
-    const int n = member(S, S.k);
-    S.V = B.vol[n]; S.Vmax = B.nVmax[n]; S.h = B.level[n];
-    S.st = L_S_FIRST;
+// the state after the pass p the state asked for; the half's lane 0 (lead) writes the node's results
+__device__ inline void single_advance(const Blk& B, Single& S, const Pass& p, bool lead) {
+  const int n = S.n;
+  if (S.st == LV_FULL) {
+    if (lead) {
+      B.level[n] = S.hi + jmax(0.0, S.V - S.Vmax) / B.nAov[n];
+      B.area[n] = B.nAov[n];
+      put_centroid(B, n, p);
+    }
+    S.st = LV_DONE;
+    return;
   }
-  __device__ __forceinline__ void next_group(LevelSolve& S) const { S.st = L_START; }
-  __device__ __forceinline__ void next_single(LevelSolve& S) const {
-    if (++S.k < S.N) start_single(S);
-    else next_group(S);
+  if (S.st == LV_FIRST) {
+    S.lo = p.zmin; S.hi = p.zmax;
+    if (lead) { B.zmin[n] = S.lo; B.zmax[n] = S.hi; }
+    if (S.V <= kEmptyVolume) {
+      if (lead) { B.level[n] = S.lo; B.area[n] = 0; }
+      S.st = LV_DONE;
+      return;
+    }
+    if (S.V >= S.Vmax * kFullFraction) { S.st = LV_FULL; return; }
+    S.tol = jmax(1e-3 * S.Vmax, 0.5);
+    S.a = S.lo; S.b = S.hi; S.it = 0;
+    if (S.h < S.lo || S.h > S.hi) {  // the pass at the level clamped into the extent comes next
+      S.h = jmin(S.hi, jmax(S.lo, S.h));
+      S.st = LV_NEWTON;
+      return;
+    }
   }
+  // the Newton iteration after a pass at S.h
+  const double err = p.vol - S.V;
+  if (S.it >= kNewtonFast || __builtin_fabs(err) < S.tol) {
+    if (lead) { B.level[n] = S.h; B.area[n] = p.dv; put_centroid(B, n, p); }
+    S.st = LV_DONE;
+    return;
+  }
+  if (err > 0) S.b = S.h; else S.a = S.h;
+  double hn = p.dv > kSlopeMin ? S.h - err / p.dv : 0.5 * (S.a + S.b);
+  if (!(hn > S.a && hn < S.b)) hn = 0.5 * (S.a + S.b);
+  S.h = hn;
+  ++S.it;
+  S.st = LV_NEWTON;
+}
 
-  // levels.hpp:166-188: the merge decision of group g, then the first state of its solve
-  __device__ void begin(LevelSolve& S) const {
-    const int g = S.g;
-    S.N = B.gCount[g];
-    S.base = B.gStart[g];
-    const int N = S.N;
-    const uint8_t mode = B.gMode[g];
+// The members of an unmerged group solved singly, members half, half + 2, ... on each half of the wave.
+template <bool Mixed>
+__device__ void solve_singles(PC& P, const Blk& B, int base, int N, int lane, int half, int& round) {
+  int next = half;
+  Single S{};
+  single_begin(B, S, next < N ? B.gNodes[base + next] : -1);
+  next += 2;
+  while (__ballot(S.st != LV_DONE) != 0) {
+    const int n = S.st == LV_DONE ? -1 : S.n;
+    const double h = S.st == LV_FULL ? S.hi : S.h;
+    const Pass p = __ballot(S.st == LV_FIRST) != 0 ? node_pass<Mixed, true>(P, B, lane, n, h) : node_pass<Mixed, false>(P, B, lane, n, h);
+    if (S.st != LV_DONE) single_advance(B, S, p, lane == 0);
+    if (S.st == LV_DONE && next < N) {
+      single_begin(B, S, B.gNodes[base + next]);
+      next += 2;
+    }
+    solve_round(round);
+  }
+}
+
+// The level solve of every group: each wave takes the next group from the workgroup's counter (its lane 0's LDS atomic,
+// read by the whole wave), decides whether the group's members share a surface this step (levels.hpp:166-188), and
+// solves it. The decision reads only the group's own members, whose volumes and levels no other group's solve changes.
+template <bool Mixed>
+__device__ void solve_levels(PC& P, const Blk& B, int tid) {
+  const int lane = tid & (kLanes - 1), half = (tid >> 5) & 1, wave = tid >> 6;
+  const int NG = P.M.ng;
+  SSL double* Pp = B.passes + 7 * P.M.maxGroup * wave;
+  int round = 0;
+  for (;;) {
+    int g = 0;
+    if ((tid & 63) == 0) g = atomicAdd((int*)B.nextGroup, 1);
+    g = __builtin_amdgcn_readfirstlane(g);
+    if (g >= NG) break;
+    const int N = __builtin_amdgcn_readfirstlane(B.gCount[g]);
+    const int base = __builtin_amdgcn_readfirstlane(B.gStart[g]);
+    const int mode = __builtin_amdgcn_readfirstlane(B.gMode[g]);
     bool merged = false;
     if (N >= 2) {
       if (mode == 1) {
@@ -646,257 +947,15 @@ struct Solver {
         const double zMw = world_z(B.frame, B.gpx[g], B.gpy[g], B.gpz[g]);
         merged = true;
         for (int k = 0; k < N; ++k) {
-          const int n = member(S, k);
-          if (!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + P.M.gMargin)) { merged = false; break; }
+          const int n = B.gNodes[base + k];
+          if (uni(!(B.vol[n] > kEmptyVolume && B.level[n] > zMw + P.M.gMargin))) { merged = false; break; }
         }
       }
     }
-    if (lead) B.merged[g] = merged ? 1 : 0;
-    S.k = 0;
-    if (!merged) {
-      if (N > 0) start_single(S);
-      else next_group(S);
-      return;
-    }
-    // levels.hpp:54-80: the common starting level
-    double V = 0, Vmax = 0;
-    for (int k = 0; k < N; ++k) { V += B.vol[member(S, k)]; Vmax += B.nVmax[member(S, k)]; }
-    bool allWet = true;
-    int firstWet = -1;
-    for (int k = 0; k < N; ++k) {
-      if (B.vol[member(S, k)] > 0) { if (firstWet < 0) firstWet = k; }
-      else allWet = false;
-    }
-    double h;
-    if (allWet) {
-      double s = 0;
-      for (int k = 0; k < N; ++k) s += B.level[member(S, k)];
-      h = s / static_cast<double>(N);
-    } else {
-      h = B.level[member(S, firstWet >= 0 ? firstWet : N - 1)];
-    }
-    S.V = V; S.Vmax = Vmax; S.h = h;
-    S.st = L_G_FIRST;
-  }
-
-  // Moves L_START on to the next state that waits for a pass, or to L_DONE: the unit takes the next group from the
-  // workgroup's counter (its lead lane's LDS atomic, broadcast to the unit), so units that finish early take more.
-  __device__ __forceinline__ void resolve(LevelSolve& S, int NG) const {
-    while (S.st == L_START) {
-      int g = 0;
-      if (lead) g = atomicAdd((int*)B.nextGroup, 1);
-      S.g = static_cast<int>(lead32(static_cast<uint32_t>(g)));
-      if (S.g >= NG) { S.st = L_DONE; break; }
-      begin(S);
-    }
-  }
-
-  // levels.hpp:33-48: the Newton iteration of a single node after a pass
-  __device__ void newton_single(LevelSolve& S, const Pass& p) const {
-    const int n = member(S, S.k);
-    const double err = p.vol - S.V;
-    if (S.it >= kNewtonFast || __builtin_fabs(err) < S.tol) {
-      if (lead) { B.level[n] = S.h; B.area[n] = p.dv; put_centroid(B, n, p); }
-      next_single(S);
-      return;
-    }
-    if (err > 0) S.b = S.h; else S.a = S.h;
-    double hn = p.dv > kSlopeMin ? S.h - err / p.dv : 0.5 * (S.a + S.b);
-    if (!(hn > S.a && hn < S.b)) hn = 0.5 * (S.a + S.b);
-    S.h = hn;
-    ++S.it;
-    S.st = L_S_NEWTON;
-  }
-
-  // levels.hpp:125-161: the water of a merged group shared in proportion to the geometry at the final level
-  __device__ void share(LevelSolve& S) const {
-    const int N = S.N;
-    const double V = S.V;
-    if (lead) {
-      double tot = 0;
-      for (int k = 0; k < N; ++k) tot += Pp[7 * k + 0];
-      double assigned = 0;
-      for (int k = 0; k < N - 1; ++k) {
-        double v = tot > 0 ? V * Pp[7 * k + 0] / tot : V / static_cast<double>(N);
-        if (v > B.nVmax[member(S, k)]) v = B.nVmax[member(S, k)];
-        B.vol[member(S, k)] = v;
-        assigned += v;
-      }
-      const double vmaxLast = B.nVmax[member(S, N - 1)];
-      if (V - assigned > vmaxLast) {
-        const double need = V - vmaxLast;
-        double capped = 0;
-        for (int k = N - 2; k >= 0; --k) {
-          double prior = 0;
-          for (int j = 0; j < k; ++j) prior += B.vol[member(S, j)];
-          const double v = (need - capped) - prior;
-          if (v > B.nVmax[member(S, k)] && k > 0) {
-            B.vol[member(S, k)] = B.nVmax[member(S, k)];
-            capped += B.nVmax[member(S, k)];
-            continue;
-          }
-          B.vol[member(S, k)] = v;
-          break;
-        }
-        assigned = 0;
-        for (int k = 0; k < N - 1; ++k) assigned += B.vol[member(S, k)];
-      }
-      B.vol[member(S, N - 1)] = V - assigned;
-      for (int k = 0; k < N; ++k) {
-        const int n = member(S, k);
-        const Pass p = get_pass(Pp, k);
-        B.level[n] = S.h; B.area[n] = p.dv;
-        put_centroid(B, n, p);
-      }
-    }
-    next_group(S);
-  }
-
-  // levels.hpp:102-123: the Newton iteration of a merged group after a round of member passes
-  __device__ void newton_group(LevelSolve& S) const {
-    const int N = S.N;
-    double sum = 0;
-    for (int k = 0; k < N; ++k) sum += Pp[7 * k + 0];
-    const double err = sum - S.V;
-    if (S.it >= kNewtonFast || __builtin_fabs(err) < S.tol) { share(S); return; }
-    if (err > 0) S.b = S.h; else S.a = S.h;
-    double dv = 0;
-    for (int k = 0; k < N; ++k) dv += Pp[7 * k + 1];
-    double hn = dv > kSlopeMin ? S.h - err / dv : 0.5 * (S.a + S.b);
-    if (!(hn > S.a && hn < S.b)) hn = 0.5 * (S.a + S.b);
-    S.h = hn;
-    ++S.it;
-    S.k = 0;
-    S.st = L_G_NEWTON;
-  }
-
-  // The request of the current state: the node and level of the pass it waits for, and whether the pass must
-  // also give the node's extent.
-  __device__ __forceinline__ void request(const LevelSolve& S, int& n, double& h, bool& ext) const {
-    n = -1; h = 0; ext = false;
-    if (S.st == L_DONE) return;
-    n = member(S, S.k);
-    h = (S.st == L_S_FULL || S.st == L_G_FULL) ? S.hi : S.h;
-    ext = S.st == L_S_FIRST || S.st == L_G_FIRST;
-  }
-
-  __device__ void advance(LevelSolve& S, const Pass& p) const {
-    const int n = member(S, S.k);
-    switch (S.st) {
-      case L_S_FIRST: {  // levels.hpp:13-32
-        S.lo = p.zmin; S.hi = p.zmax;
-        if (lead) { B.zmin[n] = S.lo; B.zmax[n] = S.hi; }
-        if (S.V <= kEmptyVolume) {
-          if (lead) { B.level[n] = S.lo; B.area[n] = 0; }
-          next_single(S);
-          return;
-        }
-        if (S.V >= S.Vmax * kFullFraction) { S.st = L_S_FULL; return; }
-        S.tol = jmax(1e-3 * S.Vmax, 0.5);
-        S.a = S.lo; S.b = S.hi; S.it = 0;
-        if (S.h < S.lo || S.h > S.hi) {
-          S.h = jmin(S.hi, jmax(S.lo, S.h));
-          S.st = L_S_CLAMP;
-          return;
-        }
-        newton_single(S, p);
-        return;
-      }
-      case L_S_FULL:
-        if (lead) {
-          B.level[n] = S.hi + jmax(0.0, S.V - S.Vmax) / B.nAov[n];
-          B.area[n] = B.nAov[n];
-          put_centroid(B, n, p);
-        }
-        next_single(S);
-        return;
-      case L_S_CLAMP:
-      case L_S_NEWTON:
-        newton_single(S, p);
-        return;
-      case L_G_FIRST: {  // levels.hpp:81-99
-        put_pass(Pp, S.k, p);
-        if (lead) { B.zmin[n] = p.zmin; B.zmax[n] = p.zmax; }
-        if (S.k == 0) { S.lo = p.zmin; S.hi = p.zmax; }
-        else { if (p.zmin < S.lo) S.lo = p.zmin; if (p.zmax > S.hi) S.hi = p.zmax; }
-        if (++S.k < S.N) return;
-        const int N = S.N;
-        if (S.V <= kEmptyVolume) {
-          if (lead) {
-            for (int k = 0; k < N; ++k) { const int m = member(S, k); B.vol[m] = 0; B.level[m] = Pp[7 * k + 5]; B.area[m] = 0; }
-          }
-          next_group(S);
-          return;
-        }
-        if (S.V >= S.Vmax * kFullFraction) {
-          const double over = jmax(0.0, S.V - S.Vmax);
-          double aSum = 0;
-          for (int k = 0; k < N; ++k) aSum += B.nAov[member(S, k)];
-          S.dh = over / aSum;
-          double assigned = 0;
-          for (int k = 0; k < N - 1; ++k) {
-            const double v = B.nVmax[member(S, k)] + S.dh * B.nAov[member(S, k)];
-            if (lead) B.vol[member(S, k)] = v;
-            assigned += v;
-          }
-          if (lead) B.vol[member(S, N - 1)] = S.V - assigned;
-          S.k = 0;
-          S.st = L_G_FULL;
-          return;
-        }
-        S.tol = jmax(1e-3 * S.Vmax, 0.5);
-        S.a = S.lo; S.b = S.hi; S.it = 0;
-        if (!(S.h >= S.lo && S.h <= S.hi)) {
-          S.h = jmin(S.hi, jmax(S.lo, S.h));
-          S.k = 0;
-          S.st = L_G_CLAMP;
-          return;
-        }
-        newton_group(S);
-        return;
-      }
-      case L_G_FULL:
-        if (lead) {
-          B.level[n] = S.hi + S.dh; B.area[n] = B.nAov[n];
-          put_centroid(B, n, p);
-        }
-        if (++S.k == S.N) next_group(S);
-        return;
-      default:  // L_G_CLAMP, L_G_NEWTON
-        put_pass(Pp, S.k, p);
-        if (++S.k < S.N) return;
-        newton_group(S);
-        return;
-    }
-  }
-};
-
-// The level solve of every group by the workgroup's 32-lane units, each taking the next group from the counter
-// when it is free; the two units of a wave issue their passes together, round after round, until both are done.
-template <bool Mixed>
-__device__ void solve_levels(PC& P, const Blk& B, int tid) {
-  const int lane = tid & (kLanes - 1), unit = tid / kLanes;
-  const int NG = P.M.ng;
-  const Solver sv{P, B, B.passes + 7 * P.M.maxGroup * unit, lane == 0};
-  LevelSolve S{};
-  S.st = L_START;
-  S.g = -1;
-  int round = 0;
-  while (true) {
-    sv.resolve(S, NG);
-    int n;
-    double h;
-    bool ext;
-    sv.request(S, n, h, ext);
-    if (__ballot(n >= 0) == 0) break;
-    const Pass p = __ballot(ext) != 0 ? node_pass<Mixed, true>(P, B, lane, n, h) : node_pass<Mixed, false>(P, B, lane, n, h);
-    if (n >= 0) sv.advance(S, p);
-    // issue priority only (no value changes): after its first round a solving wave goes ahead of the other
-    // instances' waves in their other phases on its SIMD, and after five rounds (a straggler, which the step's
-    // barrier waits for) further ahead
-    ++round;
-    if (round == 1) __builtin_amdgcn_s_setprio(1);
-    if (round == 5) __builtin_amdgcn_s_setprio(2);
+    if ((tid & 63) == 0) B.merged[g] = merged ? 1 : 0;
+    if (merged && N == 2) solve_pair<Mixed>(P, B, base, lane, half, round);
+    else if (merged) solve_merged<Mixed>(P, B, Pp, base, N, lane, half, round);
+    else solve_singles<Mixed>(P, B, base, N, lane, half, round);
   }
   if (round >= 1) __builtin_amdgcn_s_setprio(0);
 }

@@ -335,6 +335,7 @@ std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
   size_t members = 0;  // length of the group member list the groups index into
   for (int32_t g = 0; g < b.NG; ++g) members = std::max(members, static_cast<size_t>(M.gNodeStart[g] + M.gNodeCount[g]));
   b.members = static_cast<int32_t>(members);
+  b.nsea = static_cast<int32_t>(std::count_if(M.ca, M.ca + M.nconn, [](int32_t a) { return a < 0; }));
   if (b.lds_bytes(kThreads) > ldsLimit)
     fail(SS_RUNTIME, "ss_batch_create: this model needs " + std::to_string(b.lds_bytes(kThreads)) +
                          " bytes of LDS per simulation, the device allows " + std::to_string(ldsLimit));
@@ -369,8 +370,9 @@ std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t lds
   b.gpx.upload(M.gpx, ng); b.gpy.upload(M.gpy, ng); b.gpz.upload(M.gpz, ng);
   std::vector<int4> conI(std::max<size_t>(ncn, 1));
   std::vector<double4> conD(std::max<size_t>(ncn, 1));
+  int32_t seaSlots = 0;  // a sea connection's a end carries -1 - its place in the sea list (increasing connection order)
   for (size_t c = 0; c < ncn; ++c) {
-    conI[c] = make_int4(M.ca[c], M.cb[c], M.skipGroup[c], M.law[c]);
+    conI[c] = make_int4(M.ca[c] >= 0 ? M.ca[c] : -1 - seaSlots++, M.cb[c], M.skipGroup[c], M.law[c]);
     conD[c] = make_double4(M.cx[c], M.cy[c], M.cz[c], M.coef[c]);
   }
   b.conI.upload(conI); b.conD.upload(conD);

@@ -113,7 +113,7 @@ using namespace ssa;
 
 // LDS layout of one simulation, in doubles; byte and int arrays are packed into whole doubles.
 inline LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, int32_t nmark, int32_t maxGroup, int32_t evCap, int32_t nMembers,
-                     int32_t units) {
+                     int32_t units, int32_t nsea) {
   LdsLayout L{};
   int32_t off = 0;
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
@@ -122,7 +122,7 @@ inline LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, in
   L.vol = take(NN); L.level = take(NN); L.cx = take(NN); L.cy = take(NN); L.cz = take(NN); L.area = take(NN); L.zmin = take(NN);
   L.zmax = take(NN); L.hEff = take(NN); L.aEff = take(NN); L.excess = take(NN);
   L.tX = take(NN); L.tY = take(NN); L.tF = take(NN); L.tM = take(NN); L.tIp = take(NN); L.tIr = take(NN);
-  L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC);
+  L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC); L.seaDV = take(nsea);
   L.passes = take(7 * units * maxGroup);  // each solver unit's pass scratch
   L.warpHyd = take(5 * kUnits);
   L.frame = take(F_COUNT);
@@ -139,7 +139,7 @@ inline LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, in
 // One model (a hull and a flooding scenario) on the device: the geometry records, the network tables and their
 // views, with the scenario's per-instance defaults kept on the host.
 struct Model {
-  int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0, members = 0;
+  int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0, members = 0, nsea = 0;
   double cost = 0;  // work per step, relative: the order in which a batch hands its instances out
   // ship
   Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record
@@ -158,7 +158,7 @@ struct Model {
   std::vector<double> coef0;  // the discharge coefficients, which the per-instance products coef * area take
 
   // the LDS layout of one simulation on a workgroup of tpb threads (tpb / 32 solver units), and its size
-  LdsLayout layout(int tpb) const { return lds_layout(NN, NC, NG, nmon, nmark, maxGroup, evCap, members, tpb / kLanes); }
+  LdsLayout layout(int tpb) const { return lds_layout(NN, NC, NG, nmon, nmark, maxGroup, evCap, members, tpb / kLanes, nsea); }
   size_t lds_bytes(int tpb) const { return static_cast<size_t>(layout(tpb).total) * sizeof(double); }
 
   ModelParams params(int tpb) const {

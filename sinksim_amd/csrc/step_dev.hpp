@@ -100,7 +100,7 @@ struct DevNet {
   int32_t nc;
   gci *incStart, *incCount;
   const SSG i4v* inc;   // per incidence entry (node order, increasing connection): connection, its a, its b, side (1 = b)
-  const SSG i4v* conI;  // per connection: a, b, skipGroup, law
+  const SSG i4v* conI;  // per connection: a (a sea connection's -1 - its place in the sea list), b, skipGroup, law
   const SSG d4v* conD;  // per connection: x, y, z, coef
   gci* flowOrder;       // the connection of each flow slot: weirs, then orifices, each in increasing order
   int32_t seaCount;
@@ -162,7 +162,7 @@ struct DevBatch {
 struct LdsLayout {
   int32_t vol, level, cx, cy, cz, area, zmin, zmax, hEff, aEff, excess;
   int32_t tX, tY, tF, tM, tIp, tIr;
-  int32_t dV, q, carea, ctOn, passes, warpHyd, frame, scal, over, overT, markT, evT, nextGroup;
+  int32_t dV, q, carea, ctOn, seaDV, passes, warpHyd, frame, scal, over, overT, markT, evT, nextGroup;
   int32_t nMu, nVmax, nAov, nAmin, gpx, gpy, gpz;  // copies of the node and group tables
   int32_t deg, evId, nSegStart, nSegCount, gStart, gCount, gNodes;  // int32 arrays, offsets in doubles
   int32_t merged, cen, evKind, gMode;                               // byte arrays, offsets in doubles
@@ -189,7 +189,7 @@ enum : int { S_T = 0, S_ZO, S_PITCH, S_ROLL, S_VZ, S_WTH, S_WPH, O_INFLOW, O_VB,
 struct Blk {
   SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
   SSL double *tX, *tY, *tF, *tM, *tIp, *tIr;
-  SSL double *dV, *q, *carea, *ctOn, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
+  SSL double *dV, *q, *carea, *ctOn, *seaDV, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
   SSL int32_t* nextGroup;  // the next free-surface group a solver unit takes (reset to 0 before each step's solve)
   SSL double *nMu, *nVmax, *nAov, *nAmin, *gpx, *gpy, *gpz;
   SSL int32_t *deg, *evId, *nSegStart, *nSegCount, *gStart, *gCount, *gNodes;
@@ -203,7 +203,7 @@ __device__ inline Blk carve(SSL double* base, PC& P) {
   b.area = base + L.area; b.zmin = base + L.zmin; b.zmax = base + L.zmax; b.hEff = base + L.hEff; b.aEff = base + L.aEff;
   b.excess = base + L.excess;
   b.tX = base + L.tX; b.tY = base + L.tY; b.tF = base + L.tF; b.tM = base + L.tM; b.tIp = base + L.tIp; b.tIr = base + L.tIr;
-  b.dV = base + L.dV; b.q = base + L.q; b.carea = base + L.carea; b.ctOn = base + L.ctOn; b.passes = base + L.passes;
+  b.dV = base + L.dV; b.q = base + L.q; b.carea = base + L.carea; b.ctOn = base + L.ctOn; b.seaDV = base + L.seaDV; b.passes = base + L.passes;
   b.warpHyd = base + L.warpHyd; b.frame = base + L.frame; b.scal = base + L.scal; b.over = base + L.over; b.overT = base + L.overT;
   b.markT = base + L.markT; b.evT = base + L.evT;
   b.nextGroup = reinterpret_cast<SSL int32_t*>(base + L.nextGroup);
@@ -1317,30 +1317,37 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
   __syncthreads();
 
   // (c) flows per connection
-  // wave 0 carries the most work from here to the end of the step (the second flow slots, the node gathers and
-  // monitor sums, the tail while the other waves wait): it issues at the highest priority, ahead of the other
-  // instances' waves on its SIMD (solving waves included)
+  // wave 0 carries the most work from here to the end of the step (the node gathers, the tail while the other waves
+  // wait): it issues at the highest priority, ahead of the other instances' waves on its SIMD (solving waves included)
   if (tid < 64) __builtin_amdgcn_s_setprio(3);
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
     const double t = B.scal[S_T];
     // slots in flowOrder: a wave's lanes mostly follow one flow law, and the submerged weirs' powers run in the
-    // first waves only (each connection's flow depends on itself alone, so the order changes no value)
-    for (int k = tid; k < P.M.nc; k += TPB) {
+    // first waves only (each connection's flow depends on itself alone, so the order changes no value). Slots beyond
+    // the workgroup's size go to its threads in reverse, the last wave's first: that wave's slots are orifices
+    for (int r = 0; r * TPB < P.M.nc; ++r) {
+      const int k = r * TPB + ((r & 1) ? TPB - 1 - tid : tid);
+      if (k >= P.M.nc) continue;
       const int c = P.M.flowOrder[k];
-      B.dV[c] = connection_flow(P, B.frame, t, B, c);
+      const double dv = connection_flow(P, B.frame, t, B, c);
+      B.dV[c] = dv;
+      const int a = P.M.conI[c].x;
+      if (a < 0) B.seaDV[-1 - a] = dv;  // a sea connection's volume also at its place in the sea list
     }
   }
   __syncthreads();
 
-  // (c) gathers, volume updates and load terms per node; monitor sums (events.hpp:73-75, per monitor in increasing
-  // connection order); the sea inflow (flows.hpp:288-292) on the last thread
+  // (c) gathers, volume updates and load terms per node (the first waves); monitor sums (events.hpp:17-19, per monitor
+  // in increasing connection order) on the threads after the nodes' waves; the sea inflow (flows.hpp:92-96) on the
+  // last thread
   {
     PC& P = model_at(pa);
     const Blk B = carve(lds, P);
     for (int n = tid; n < P.S.nn; n += TPB) node_update(P, B, n);
-    for (int m = tid; m < P.M.nmon; m += TPB) {
+    const int mon0 = (P.S.nn + 63) / 64 * 64 % TPB;
+    for (int m = (tid - mon0 + TPB) % TPB; m < P.M.nmon; m += TPB) {
       double o = 0;
       const int s = P.M.monStart[m], e = s + P.M.monCount[m];
       for (int k0 = s; k0 < e; k0 += kFetch) {
@@ -1358,7 +1365,7 @@ __device__ __forceinline__ void device_step(uint64_t pa, SSL double* lds, int ti
     }
     if (tid == TPB - 1) {
       double s = 0;
-      for (int k = 0; k < P.M.seaCount; ++k) s += B.dV[P.M.sea[k]];
+      for (int k = 0; k < P.M.seaCount; ++k) s += B.seaDV[k];  // the sea connections in increasing order
       B.scal[S_SEA] = s;
     }
   }

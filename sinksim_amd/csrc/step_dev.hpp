@@ -504,6 +504,8 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
 // issued while the previous record is evaluated. Unrolled by two so that the two records in flight keep their
 // registers (no copies between iterations); records are addressed by byte offsets advanced by step records (no
 // multiplication per record). A lane past its last record reloads its current one (in bounds) and stops.
+// The scheduling barriers keep the next record's loads ahead of the current record's arithmetic: without them
+// the compiler sinks the loads below the evaluation and waits for them at once (vmcnt(0)).
 template <class T, class Tab, class Fn>
 __device__ __forceinline__ void stream(const Tab* table, int first, int end, int step, Fn&& eval) {
   int j = first;
@@ -517,12 +519,14 @@ __device__ __forceinline__ void stream(const Tab* table, int first, int end, int
     const bool m1 = j1 < end;
     const uint32_t off1 = m1 ? off + stride : off;
     const Geo<T> g1 = geo_at(table, off1);
+    __builtin_amdgcn_sched_barrier(0);
     eval(g0);
     if (!m1) break;
     const int j2 = j1 + step;
     const bool m2 = j2 < end;
     off = m2 ? off1 + stride : off1;
     g0 = geo_at(table, off);
+    __builtin_amdgcn_sched_barrier(0);
     eval(g1);
     if (!m2) break;
     j = j2;

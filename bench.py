@@ -375,8 +375,7 @@ def main():
 
     if args.copy_probe > 0:
         import ctypes
-        from sinksim_amd.build import PROBE
-        probe = ctypes.CDLL(PROBE)
+        probe = ctypes.CDLL(os.path.join(ROOT, "tests", "probe", "libsinksim_probe.so"))
         probe.ssp_copy_probe.argtypes = [ctypes.c_longlong]
         probe.ssp_last_error.restype = ctypes.c_char_p
         if probe.ssp_copy_probe(args.copy_probe << 20) != 0:

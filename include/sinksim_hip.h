@@ -49,7 +49,7 @@ typedef struct ss_ship {
 
 /* A compiled simulation (docs/FORMATS.md "sinksim.compiled-sim"; engine/include/sinksim/model.hpp:49-96). */
 typedef struct ss_sim {
-  int32_t nnode;     /* must equal the ship's node count (engine/cuda/batch.cu:272) */
+  int32_t nnode;     /* must equal the ship's node count (engine/cuda/batch.cu:183) */
   double rho, g, dt;
   int32_t ngroup;
   const int32_t *gNodeStart, *gNodeCount, *gNodes; /* [ngroup], [ngroup], [sum of counts] */
@@ -198,7 +198,7 @@ ss_status ss_batch_run(ss_batch* b, double* seconds);
 
 /* Layer 1 of the boundary: kernel::step (engine/include/sinksim/kernel/step.hpp:41-54) over the batch, as the
  * reference's Simulation::step (engine/src/simulation.cpp:147-154) calls it after a restore in the acceptance procedure
- * (apps/sinksim_check.cpp:109-127). Every instance takes exactly `steps` steps (perInstance[i] for instance i when
+ * (apps/sinksim_check.cpp:110-126). Every instance takes exactly `steps` steps (perInstance[i] for instance i when
  * perInstance is not null) from its staged or restored state, whether or not it founders or passes tMax, as repeated
  * calls of step() do. The first call uploads the staged instances; later calls continue from the device state. Results,
  * traces and state records read as after ss_batch_run; restore, the setters and ss_batch_run are refused afterwards
@@ -335,7 +335,7 @@ ss_status ss_batch_gradient_launch(const ss_batch* b, int32_t* threads, int32_t*
  * callers without a device. No device is needed or touched. */
 
 /* RunOptions (engine/include/sinksim/simulation.hpp:31-35) with the numerics and the thread count of sweep::run_cpu
- * (engine/include/sinksim/sweep.hpp:78-79). */
+ * (engine/include/sinksim/sweep.hpp:95-96). */
 typedef struct ss_cpu_options {
   int32_t numerics;       /* SS_NUMERICS_ORACLE, _PORTABLE, _PORTABLE32 or _STD */
   int32_t threads;        /* host threads; <= 0: as many as the hardware runs at once */

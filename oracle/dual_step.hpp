@@ -1,7 +1,7 @@
 // ORACLE — test infrastructure only; the product never includes or links this file.
 //
 // A forward-mode (dual-number) build of the portable step: the reference's single-source step in Tree order
-// with the portable math policy (engine/include/sinksim/kernel/step.hpp:140-153, Order::Tree, MathPortable),
+// with the portable math policy (engine/include/sinksim/kernel/step.hpp:41-54, Order::Tree, MathPortable),
 // restated over a scalar type T that carries, beside each value, its derivatives along up to N directions.
 // Geometry and constants stay double; the dynamic state, the frame, every partial, pass, head, flow, load and
 // the parameters are T. Branches are taken on values, so the values are those of the portable oracle bit for
@@ -115,8 +115,8 @@ template <int N> inline D<N> dpow(const D<N>& x, double y) {
   return r;
 }
 
-template <class T> inline T tmax(const T& a, const T& b) { return (b > a) ? b : a; }  // math.hpp:338
-template <class T> inline T tmin(const T& a, const T& b) { return (b < a) ? b : a; }  // math.hpp:339
+template <class T> inline T tmax(const T& a, const T& b) { return (b > a) ? b : a; }  // math.hpp:29
+template <class T> inline T tmin(const T& a, const T& b) { return (b < a) ? b : a; }  // math.hpp:30
 
 // ---------------------------------------------------------------- records over T (types.hpp:159-182)
 
@@ -178,13 +178,13 @@ inline void absorb_t(PP<T>& a, const PP<T>& b) {
   if (b.zmax > a.zmax) a.zmax = b.zmax;
 }
 template <class P, int L>
-inline P halving_t(P (&v)[L]) {  // reduce.hpp:102-107
+inline P halving_t(P (&v)[L]) {  // reduce.hpp:74-79
   for (int off = L / 2; off > 0; off >>= 1)
     for (int l = 0; l < off; ++l) absorb_t(v[l], v[l + off]);
   return v[0];
 }
 
-// one column of the buoyancy integral (reduce.hpp:67-77)
+// one column of the buoyancy integral (reduce.hpp:39-49)
 template <class T>
 inline void column(const Ship& S, const FrameT<T>& F, const T& iR, Index i, HP<T>& p) {
   const double x = S.x[i], y = S.y[i], lo = S.zlo[i], hi = S.zhi[i];
@@ -198,7 +198,7 @@ inline void column(const Ship& S, const FrameT<T>& F, const T& iR, Index i, HP<T
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
 }
 
-// one segment of a node pass (reduce.hpp:80-99)
+// one segment of a node pass (reduce.hpp:52-71)
 template <class T>
 inline void segment(const Ship& S, const FrameT<T>& F, const T& iR, const T& h, Index j, PP<T>& p) {
   const Index i = S.segCol[j];
@@ -222,7 +222,7 @@ inline void segment(const Ship& S, const FrameT<T>& F, const T& iR, const T& h, 
 }
 
 template <class T>
-inline BuoyT<T> finish_buoy(const HP<T>& p) {  // hydro.hpp:231-239
+inline BuoyT<T> finish_buoy(const HP<T>& p) {  // hydro.hpp:37-45
   BuoyT<T> o;
   o.V = p.V;
   o.x = p.V > 0 ? p.Mx / p.V : T(0);
@@ -233,7 +233,7 @@ inline BuoyT<T> finish_buoy(const HP<T>& p) {  // hydro.hpp:231-239
 }
 
 template <class T>
-inline PassT<T> finish_pass(const Ship& S, Index n, const T& iR, const PP<T>& p) {  // hydro.hpp:295-303
+inline PassT<T> finish_pass(const Ship& S, Index n, const T& iR, const PP<T>& p) {  // hydro.hpp:101-109
   const double mu = S.mu[n];
   PassT<T> r;
   r.vol = p.vol * mu;
@@ -243,7 +243,7 @@ inline PassT<T> finish_pass(const Ship& S, Index n, const T& iR, const PP<T>& p)
   return r;
 }
 
-// hydro.hpp:242-258 (Tree order)
+// hydro.hpp:48-64 (Tree order)
 template <class T>
 inline BuoyT<T> buoyancy(const Ship& S, const FrameT<T>& F) {
   const T iR = 1.0 / F.R22;
@@ -261,7 +261,7 @@ inline BuoyT<T> buoyancy(const Ship& S, const FrameT<T>& F) {
   return finish_buoy(total);
 }
 
-// hydro.hpp:306-316 (Tree order)
+// hydro.hpp:112-122 (Tree order)
 template <class T>
 inline PassT<T> node_pass(const Ship& S, const FrameT<T>& F, Index n, const T& h) {
   const Index s0 = S.segStart[n], s1 = s0 + S.segCount[n];
@@ -318,7 +318,7 @@ class Sim {
     for (auto& m : markT_) m = -1;
   }
 
-  // step.hpp:140-153 with MathPortable and Order::Tree; returns nothing, fills the event-signal hooks
+  // step.hpp:41-54 with MathPortable and Order::Tree; returns nothing, fills the event-signal hooks
   void step() {
     const FrameT<T> F = pose(M_.body, zO, pitch, roll);
     const BuoyT<T> hb = buoyancy(S_, F);
@@ -359,7 +359,7 @@ class Sim {
   // in the discrete program is that of the last iterate (often the previous step's level, carried unchanged);
   // relaxed, it is that of the root of vol(h) = V at the final level h: the tangent gains one Newton step,
   // d h += (dV - dvol) / a, with a the pass's slope at h floored at the space's minimum effective area amin (the
-  // floor the flow limiter puts under the free-surface area, flows.hpp:211-216: near the top or bottom of a space
+  // floor the flow limiter puts under the free-surface area, flows.hpp:15-18: near the top or bottom of a space
   // the slope vanishes and 1 / dv would make the linearised flows unstable). The value of h is untouched.
   static bool relax_level(T& h, const T& resid, const T& dv, double amin) {
     if constexpr (std::is_same_v<T, double>) {
@@ -548,14 +548,14 @@ class Sim {
     }
   }
 
-  bool counts(Index c) const {  // flows.hpp:217-222
+  bool counts(Index c) const {  // flows.hpp:21-26
     if (!M_.en[c]) return false;
     const Index a = M_.ca[c], b = M_.cb[c];
     if (a >= 0 && hEff_[a] < kNoHeadTest && hEff_[b] < kNoHeadTest) return false;
     return true;
   }
 
-  // flows.hpp:234-274 with the connection's area and coefficient carrying their factors
+  // flows.hpp:38-78 with the connection's area and coefficient carrying their factors
   T flow(const FrameT<T>& F, Index c) {
     if (!M_.en[c] || M_.tOn[c] > t) { q_[c] = T(0); return T(0); }
     if (M_.skipGroup[c] >= 0 && merged_[M_.skipGroup[c]]) { q_[c] = T(0); return T(0); }
@@ -595,7 +595,7 @@ class Sim {
     return dV;
   }
 
-  // flows.hpp:318-330 and 295-307
+  // flows.hpp:121-134 and 99-111
   void flows_and_volumes(const FrameT<T>& F) {
     const Index NN = S_.nn;
     for (Index n = 0; n < NN; ++n) {

@@ -3,7 +3,7 @@
 //
 // The host side of the CPU restatement: a simulation holder with the reference's Simulation semantics
 // (engine/src/simulation.cpp:34-221: reset, restore, step, run with the 30 s history, record_trace) and the
-// threaded sweep runner of engine/src/sweep.cpp:231-271, exported as a C API for ctypes; and the driver of the
+// threaded sweep runner of engine/src/sweep.cpp:121-161, exported as a C API for ctypes; and the driver of the
 // dual-number build of the portable step (dual_step.hpp): a run with its loss and one step with its tangent.
 #include <algorithm>
 #include <atomic>
@@ -165,7 +165,7 @@ class Sim {
     return steps;
   }
 
-  // one record of the complete state, the trace layout (trace.cpp:57-67)
+  // one record of the complete state, the trace layout (trace.cpp:11-21)
   void write_record(double* rec) const {
     const Index NN = ship_.nn;
     rec[0] = st_.t; rec[1] = st_.zO; rec[2] = st_.pitch; rec[3] = st_.roll; rec[4] = st_.vz; rec[5] = st_.wth; rec[6] = st_.wph;
@@ -205,7 +205,7 @@ class Sim {
   Outputs out_{};
   Events ev_{};
 
-  // model.cpp:137-167: node -> connection CSR in increasing connection order, sea connections in order
+  // model.cpp:26-56: node -> connection CSR in increasing connection order, sea connections in order
   void build_incidence() {
     const Index NN = ship_.nn, NC = sim_.nc;
     incCount_.assign(NN, 0);
@@ -453,12 +453,12 @@ struct orc_params {
 // The loss of a run (no counterpart in the reference; it extends the reference's calibration objective,
 // tools/py/calibrate_gpu.py:35-64, to terms with derivatives): weighted squared errors against observations of
 //   readout terms j: the trim (kind 0: pitch * 180 / pi) or list (kind 1: roll * 180 / pi) after step rdStep[j]
-//                    (readouts.hpp:181-182); a step after the end of the run scores rdAfter[j] (no derivative), as
+//                    (readouts.hpp:26-27); a step after the end of the run scores rdAfter[j] (no derivative), as
 //                    the reference objective scores trim and list after foundering;
 //   event terms e:   the time at which monitor idx (kind 0) first exceeds the threshold, mark idx (kind 1) first
 //                    goes under (or above), or the ship founders (kind 2), from the event's signed signal s_k at
 //                    step k: monitor flow sum - threshold; -world z of an 'under' mark, +world z of an 'above' one;
-//                    for foundering the largest margin of the founder rule (events.hpp:31-37): hullTopBelow - hull
+//                    for foundering the largest margin of the founder rule (events.hpp:32-37): hullTopBelow - hull
 //                    top, |pitch| - pitchAbsAbove, and after minT buoyancyBelow - V (a tie keeps the earlier). With
 //                    k the first step at which s_k > 0, t* = t_k - dt * s_k / (s_k - s_{k-1}), the interpolated
 //                    crossing (t_1 if k = 1). An event that never fires scores the founder time if the ship
@@ -494,7 +494,7 @@ void drop(const T& x, int nd, double* v, double* dv, size_t stride, size_t i) {
       for (int k = 0; k < nd; ++k) dv[k * stride + i] = x.d[k];
 }
 
-// the state as a trace record (trace.cpp:57-67 layout; inflow is not carried and stays 0)
+// the state as a trace record (trace.cpp:11-21 layout; inflow is not carried and stays 0)
 template <class T>
 void state_record(const dual::Sim<T>& d, Index NN, int nd, double* rec, double* drec) {
   const size_t R = 10 + 5 * static_cast<size_t>(NN);

@@ -2,11 +2,11 @@
 //
 // A CPU restatement of the reference's single-source flooding step, operation for operation, so that the
 // HIP path can be held to it bit for bit. Three summation orders exist, as in the reference
-// (engine/include/sinksim/kernel/step.hpp:117):
-//   Serial     the oracle's serial loops (hydro.hpp:208-229, 261-293)
-//   Tree       256 thread partials / 32-lane warp trees in double ("portable", reduce.hpp:12-107)
-//   TreeMixed  the same with single-precision partials widened to double ("portable32", reduce.hpp:109-192)
-// and three math policies (math.hpp:336-384): Portable (fdlibm sin/cos/pow), Oracle (fdlibm sin/cos and
+// (engine/include/sinksim/kernel/step.hpp:18):
+//   Serial     the oracle's serial loops (hydro.hpp:14-35, 67-99)
+//   Tree       256 thread partials / 32-lane warp trees in double ("portable", reduce.hpp:12-79)
+//   TreeMixed  the same with single-precision partials widened to double ("portable32", reduce.hpp:81-164)
+// and three math policies (math.hpp:27-75): Portable (fdlibm sin/cos/pow), Oracle (fdlibm sin/cos and
 // V8 13.6 pow semantics over the platform pow, third_party/v8math/v8_math.hpp:26-36) and Std (platform).
 //
 // Parity pins (tests/): the reference's golden traces for the three numerics (dense single steps, sampled
@@ -48,10 +48,10 @@ constexpr Real kPitchClamp = 1.35;
 constexpr Real kRollClamp = 1.4;
 constexpr Real kPi = 3.141592653589793;
 
-constexpr int kLanes = 32;    // logical warp width of the canonical tree (reduce.hpp:40)
-constexpr int kThreads = 256; // logical block width (reduce.hpp:41)
+constexpr int kLanes = 32;    // logical warp width of the canonical tree (reduce.hpp:12)
+constexpr int kThreads = 256; // logical block width (reduce.hpp:13)
 
-// JavaScript Math.max / Math.min for ordinary numbers (math.hpp:338-339)
+// JavaScript Math.max / Math.min for ordinary numbers (math.hpp:29-30)
 inline Real jmax(Real a, Real b) { return (b > a) ? b : a; }
 inline Real jmin(Real a, Real b) { return (b < a) ? b : a; }
 
@@ -191,7 +191,7 @@ struct Events {
   Real founderT;
 };
 
-inline void push_event(Events& ev, Real t, Index id, Byte kind) {  // types.hpp:270-277
+inline void push_event(Events& ev, Real t, Index id, Byte kind) {  // types.hpp:240-247
   if (ev.count < ev.capacity) {
     ev.items[ev.count] = Event{t, id, kind};
     ev.count++;
@@ -222,7 +222,7 @@ inline Frame pose(const Body& b, Real zO, Real th, Real ph) {
 
 inline Real world_z(const Frame& F, Real x, Real y, Real z) { return F.R20 * x + F.R21 * y + F.R22 * z + F.tz; }
 
-// ---------------------------------------------------------------- partials (reduce.hpp:44-192)
+// ---------------------------------------------------------------- partials (reduce.hpp:16-164)
 
 template <class T>
 struct HP {
@@ -250,7 +250,7 @@ inline void absorb(PP<T>& a, const PP<T>& b) {
   if (b.zmax > a.zmax) a.zmax = b.zmax;
 }
 
-// the canonical halving tree over L partials: l absorbs l + off for off = L/2 ... 1 (reduce.hpp:102-107)
+// the canonical halving tree over L partials: l absorbs l + off for off = L/2 ... 1 (reduce.hpp:74-79)
 template <class P, int L>
 inline P halving(P (&v)[L]) {
   for (int off = L / 2; off > 0; off >>= 1)
@@ -271,7 +271,7 @@ inline FrameF frame_f(const Frame& F) {
   return f;
 }
 
-// one column of the buoyancy integral, double (reduce.hpp:67-77)
+// one column of the buoyancy integral, double (reduce.hpp:39-49)
 inline void column(const Ship& S, const Frame& F, Real iR, Index i, HP<Real>& p) {
   const Real x = S.x[i], y = S.y[i], lo = S.zlo[i], hi = S.zhi[i];
   const Real base = F.R20 * x + F.R21 * y + F.tz;
@@ -285,7 +285,7 @@ inline void column(const Ship& S, const Frame& F, Real iR, Index i, HP<Real>& p)
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
 }
 
-// one column, single precision (reduce.hpp:159-170)
+// one column, single precision (reduce.hpp:131-142)
 inline void column_f(const Ship& S, const FrameF& f, Index i, HP<float>& p) {
   const float x = static_cast<float>(S.x[i]), y = static_cast<float>(S.y[i]);
   const float lo = static_cast<float>(S.zlo[i]), hi = static_cast<float>(S.zhi[i]);
@@ -300,7 +300,7 @@ inline void column_f(const Ship& S, const FrameF& f, Index i, HP<float>& p) {
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
 }
 
-// one segment of a node pass, double (reduce.hpp:80-99)
+// one segment of a node pass, double (reduce.hpp:52-71)
 inline void segment(const Ship& S, const Frame& F, Real iR, Real h, Index j, PP<Real>& p) {
   const Index i = S.segCol[j];
   const Real x = S.x[i], y = S.y[i], a = S.segA[j], b = S.segE[j];
@@ -324,7 +324,7 @@ inline void segment(const Ship& S, const Frame& F, Real iR, Real h, Index j, PP<
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
 }
 
-// one segment, single precision (reduce.hpp:172-192)
+// one segment, single precision (reduce.hpp:144-164)
 inline void segment_f(const Ship& S, const FrameF& f, float h, Index j, PP<float>& p) {
   const Index i = S.segCol[j];
   const float x = static_cast<float>(S.x[i]), y = static_cast<float>(S.y[i]);
@@ -357,7 +357,7 @@ inline PP<Real> widen(const PP<float>& p) {
                   static_cast<Real>(p.mz), static_cast<Real>(p.zmin), static_cast<Real>(p.zmax)};
 }
 
-inline Buoy finish_buoy(const HP<Real>& p) {  // hydro.hpp:231-239
+inline Buoy finish_buoy(const HP<Real>& p) {  // hydro.hpp:37-45
   Buoy o;
   o.V = p.V;
   o.x = p.V > 0 ? p.Mx / p.V : 0;
@@ -367,7 +367,7 @@ inline Buoy finish_buoy(const HP<Real>& p) {  // hydro.hpp:231-239
   return o;
 }
 
-inline Pass finish_pass(const Ship& S, Index n, Real iR, const PP<Real>& p) {  // hydro.hpp:295-303
+inline Pass finish_pass(const Ship& S, Index n, Real iR, const PP<Real>& p) {  // hydro.hpp:101-109
   const Real mu = S.mu[n];
   Pass r;
   r.vol = p.vol * mu;
@@ -382,13 +382,13 @@ inline Pass finish_pass(const Ship& S, Index n, Real iR, const PP<Real>& p) {  /
 template <Order O>
 inline Buoy buoyancy(const Ship& S, const Frame& F) {
   if constexpr (O == Order::Serial) {
-    // hydro.hpp:208-229: a single running sum; the same arithmetic as one partial holding every column
+    // hydro.hpp:14-35: a single running sum; the same arithmetic as one partial holding every column
     const Real iR = 1 / F.R22;
     HP<Real> p = hp_zero<Real>();
     for (Index i = 0; i < S.ncol; ++i) column(S, F, iR, i, p);
     return finish_buoy(p);
   } else if constexpr (O == Order::Tree) {
-    // hydro.hpp:242-258: partial t sums columns t, t+256, ...; 8 warp trees; warps combined 0..7 in order
+    // hydro.hpp:48-64: partial t sums columns t, t+256, ...; 8 warp trees; warps combined 0..7 in order
     const Real iR = 1 / F.R22;
     HP<Real> warps[kThreads / kLanes];
     for (int w = 0; w < kThreads / kLanes; ++w) {
@@ -403,7 +403,7 @@ inline Buoy buoyancy(const Ship& S, const Frame& F) {
     for (int w = 1; w < kThreads / kLanes; ++w) absorb(total, warps[w]);
     return finish_buoy(total);
   } else {
-    // hydro.hpp:319-335: single-precision partials and warp trees, widened before the in-order combine
+    // hydro.hpp:125-141: single-precision partials and warp trees, widened before the in-order combine
     const FrameF f = frame_f(F);
     HP<float> warps[kThreads / kLanes];
     for (int w = 0; w < kThreads / kLanes; ++w) {
@@ -425,13 +425,13 @@ inline Pass node_pass(const Ship& S, const Frame& F, Index n, Real h) {
   ORC_COUNT(passes);
   const Index s0 = S.segStart[n], s1 = s0 + S.segCount[n];
   if constexpr (O == Order::Serial) {
-    // hydro.hpp:261-293
+    // hydro.hpp:67-99
     const Real iR = 1 / F.R22;
     PP<Real> p = pp_zero<Real>();
     for (Index j = s0; j < s1; ++j) segment(S, F, iR, h, j, p);
     return finish_pass(S, n, iR, p);
   } else if constexpr (O == Order::Tree) {
-    // hydro.hpp:306-316
+    // hydro.hpp:112-122
     const Real iR = 1 / F.R22;
     PP<Real> lanes[kLanes];
     for (int l = 0; l < kLanes; ++l) {
@@ -440,7 +440,7 @@ inline Pass node_pass(const Ship& S, const Frame& F, Index n, Real h) {
     }
     return finish_pass(S, n, iR, halving(lanes));
   } else {
-    // hydro.hpp:338-349
+    // hydro.hpp:144-155
     const FrameF f = frame_f(F);
     const float hf = static_cast<float>(h);
     PP<float> lanes[kLanes];
@@ -630,7 +630,7 @@ inline void solve_levels(const Ship& S, const Net& M, const Frame& F, bool full,
 
 // ---------------------------------------------------------------- (c) network (flows.hpp)
 
-inline bool counts(const Net& M, const Scratch& sc, Index c) {  // flows.hpp:217-222
+inline bool counts(const Net& M, const Scratch& sc, Index c) {  // flows.hpp:21-26
   if (!M.en[c]) return false;
   const Index a = M.ca[c], b = M.cb[c];
   if (a >= 0 && sc.hEff[a] < kNoHeadTest && sc.hEff[b] < kNoHeadTest) return false;
@@ -638,7 +638,7 @@ inline bool counts(const Net& M, const Scratch& sc, Index c) {  // flows.hpp:217
 }
 
 template <class Mth>
-inline Real flow(const Net& M, const Frame& F, const State& st, const Scratch& sc, Index c) {  // flows.hpp:234-274
+inline Real flow(const Net& M, const Frame& F, const State& st, const Scratch& sc, Index c) {  // flows.hpp:38-78
   if (!M.en[c] || M.tOn[c] > st.t) { M.q[c] = 0; return 0; }
   if (M.skipGroup[c] >= 0 && sc.merged[M.skipGroup[c]]) { M.q[c] = 0; return 0; }
   const Index a = M.ca[c], b = M.cb[c];
@@ -679,7 +679,7 @@ inline Real flow(const Net& M, const Frame& F, const State& st, const Scratch& s
   return dV;
 }
 
-// flows.hpp:318-330, the serial driver
+// flows.hpp:121-134, the serial driver
 template <class Mth>
 inline Real flows_and_volumes(const Ship& S, const Net& M, const Frame& F, State& st, Scratch& sc) {
   const Index NN = S.nn;
@@ -706,7 +706,7 @@ inline Real flows_and_volumes(const Ship& S, const Net& M, const Frame& F, State
   }
   Real seaIn = 0;
   for (Index k = 0; k < M.seaCount; ++k) seaIn += sc.dV[M.sea[k]];
-  for (Index n = 0; n < NN; ++n) {  // flows.hpp:295-307
+  for (Index n = 0; n < NN; ++n) {  // flows.hpp:99-111
     Real v = st.vol[n] + sc.acc[n];
     if (v < 0) v = 0;
     const Real vm = S.vmax[n];
@@ -719,7 +719,7 @@ inline Real flows_and_volumes(const Ship& S, const Net& M, const Frame& F, State
     sc.excess[n] = excess;
     st.vol[n] = v;
   }
-  for (Index n = 0; n < NN; ++n)  // flows.hpp:310-314
+  for (Index n = 0; n < NN; ++n)  // flows.hpp:114-118
     if (sc.excess[n] > 0) seaIn -= sc.excess[n];
   return seaIn;
 }
@@ -802,7 +802,7 @@ inline void track_events(const Net& M, const Frame& F, const State& st, const Ou
   }
 }
 
-// ---------------------------------------------------------------- the step (step.hpp:140-153)
+// ---------------------------------------------------------------- the step (step.hpp:41-54)
 
 template <class Mth, Order O>
 inline void step(const Ship& S, const Net& M, State& st, Scratch& sc, Outputs& out, Events& ev) {
@@ -819,7 +819,7 @@ inline void step(const Ship& S, const Net& M, State& st, Scratch& sc, Outputs& o
   track_events(M, F, st, out, sc, ev);
 }
 
-// readouts.hpp:175-191
+// readouts.hpp:20-36
 struct Readouts {
   Real t, trimDeg, listDeg, draftF, draftA, waterM3, waterT, inflowTpm;
 };

@@ -252,7 +252,7 @@ class OracleSim:
 
 def sweep(ship, sim, numerics: str, en, area, tOn, t_max: float, every: float, threads: int = 0,
           stop_when_stable: bool = False, model: Model | None = None) -> list[dict]:
-    """sweep::run_cpu (engine/src/sweep.cpp:231-271) on per-instance connection tables [count, nconn]."""
+    """sweep::run_cpu (engine/src/sweep.cpp:121-161) on per-instance connection tables [count, nconn]."""
     model = model or Model(ship, sim)
     en, area, tOn = _u(en), _f(area), _f(tOn)
     count = en.shape[0]

@@ -80,22 +80,8 @@ def build_tools(force: bool = True) -> list[str]:
     return out
 
 
-PROBE = os.path.join(HERE, "libsinksim_probe.so")
-
-
-def build_probe(force: bool = True) -> str:
-    """The device-arithmetic probe (csrc/probe.hip), a test hook built with the same flags as the engine."""
-    src = os.path.join(CSRC, "probe.hip")
-    deps = [src] + LIB_DEPS[1:]
-    if not force and os.path.exists(PROBE) and all(os.path.getmtime(PROBE) >= os.path.getmtime(d) for d in deps):
-        return PROBE
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", PROBE + ".tmp", src])
-    os.replace(PROBE + ".tmp", PROBE)
-    return PROBE
-
-
 def build_all(force: bool = True) -> list[str]:
-    return [build_library(force), *build_tools(force), build_probe(force)]
+    return [build_library(force), *build_tools(force)]
 
 
 if __name__ == "__main__":

@@ -33,7 +33,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-# The four calibration kinds of the reference's objective (tools/py/calibrate_gpu.py:30: breach, over, down, top).
+# The four calibration kinds of the reference's objective (tools/py/calibrate_gpu.py:25: breach, over, down, top).
 CALIBRATION_KINDS = ("breach", "over", "down", "top")
 
 # The observation weights of data/observations/titanic-1912.json (objective): trim sigma 0.3 deg up to 45 min and

@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FWD ? 4 : 
         } else if (kind == 1) {
           s = P.M.mkWhen[D.evIdx[e]] == 0 ? -r[6] : r[6];
         } else {
-          // the founder rule's largest margin (events.hpp:31-37)
+          // the founder rule's largest margin (events.hpp:32-37)
           const double pitch = B.scal[S_PITCH];
           s = P.M.hullTopBelow - B.scal[O_TOP];
           const double sp = __builtin_fabs(pitch) - P.M.pitchAbsAbove;

@@ -171,7 +171,7 @@ __device__ __forceinline__ void argmin_level(double& z, double& u, double& v, do
   const double oz = down<OFF>(z), ou = down<OFF>(u), ov = down<OFF>(v), ow = down<OFF>(w);
   if (oz < z) { z = oz; u = ou; v = ov; w = ow; }
 }
-// The pass of node n at level h by a 32-lane unit (hydro.hpp:306-316 values, in the canonical tree), with its
+// The pass of node n at level h by a 32-lane unit (hydro.hpp:112-122 values, in the canonical tree), with its
 // moment sums; the record is in every lane of the unit.
 __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, int lane, int n, double h) {
   const double R20 = fr[F_R20], R21 = fr[F_R21], R22 = fr[F_R22], tz = fr[F_TZ], iR = fr[F_IR];
@@ -541,7 +541,7 @@ __device__ inline void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, 
   }
 }
 
-// buoyancy partials (hydro.hpp:242-258 values) with the moment sums of the partly wet columns and the column that
+// buoyancy partials (hydro.hpp:48-64 values) with the moment sums of the partly wet columns and the column that
 // gives the hull top (carried through the canonical tree: a later lane replaces only a strictly higher top); each
 // 32-lane group's result goes to bw (5 values, 10 sums, the top column's x, y and top)
 template <bool GT>
@@ -619,7 +619,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
   if (TPB == kThreads || tid < kThreads) buoyancy_rec(P, T, tid);
   solve_levels_tv<TPB>(P, B, T, T.mp, AM.maxGroup, tid, relaxed);
   __syncthreads();
-  // (c) heads, effective areas (and which of aov / amin / area gave it) and limiter degrees (flows.hpp:211-231)
+  // (c) heads, effective areas (and which of aov / amin / area gave it) and limiter degrees (flows.hpp:15-35)
   for (int n = tid; n < NN; n += TPB) {
     const double v = T.oVol[n];
     T.hEff[n] = v > kEmptyVolume ? T.oLvl[n] : kNoHead;
@@ -644,7 +644,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
     T.deg[n] = d;
   }
   __syncthreads();
-  // (c) flows with their partial derivatives (flows.hpp:234-274): cp = d dV / d(ha, hb, zo, aEff_a, aEff_b, ln ca)
+  // (c) flows with their partial derivatives (flows.hpp:38-78): cp = d dV / d(ha, hb, zo, aEff_a, aEff_b, ln ca)
   const double t = T.sv[0];
   for (int k = tid; k < NC; k += TPB) {
     const int c = P.M.flowOrder[k];  // weirs, then orifices (the forward's flow slots)
@@ -742,7 +742,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
     T.dV[c] = dV;
   }
   __syncthreads();
-  // (c) gathers and volume updates (flows.hpp:277-307)
+  // (c) gathers and volume updates (flows.hpp:81-111)
   for (int n = tid; n < NN; n += TPB) {
     const int s = P.M.incStart[n], e = s + P.M.incCount[n];
     double acc = 0;

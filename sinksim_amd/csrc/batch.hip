@@ -325,7 +325,7 @@ void check_options(const ss_batch_options* opt, const char* who) {
   need(opt->curveEvery >= 0, w + ": curveEvery must not be negative");
 }
 
-// Uploads one model: geometry records, network tables (model.cpp:137-167 for the incidence), LDS layout.
+// Uploads one model: geometry records, network tables (model.cpp:26-56 for the incidence), LDS layout.
 std::shared_ptr<Model> build_model(const ss_ship& S, const ss_sim& M, size_t ldsLimit) {
   std::shared_ptr<Model> mp(new Model);
   Model& b = *mp;

@@ -74,7 +74,7 @@ void write_file(const std::string& path, const Value& v, int indent = 2);
 // shortest round-trip decimal; -0 as 0; non-finite as null
 std::string format_number(double d);
 
-// member access with the reference's error texts (engine/src/json.cpp:367-408)
+// member access with the reference's error texts (engine/src/json.cpp:86-115)
 const Value& member(const Value& obj, const char* key, const std::string& where);
 double number(const Value& obj, const char* key, const std::string& where);
 long long integer(const Value& obj, const char* key, const std::string& where);

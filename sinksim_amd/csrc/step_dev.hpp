@@ -2,12 +2,12 @@
 // scratch and connection overrides in LDS, the shared hull geometry streamed from L2.
 //
 // Bits: the step reproduces the reference's portable numerics (double) and portable32 numerics (mixed) exactly
-// (reference: engine/include/sinksim/kernel/step.hpp:140-153 with Order::Tree / Order::TreeMixed). Three sums
+// (reference: engine/include/sinksim/kernel/step.hpp:41-54 with Order::Tree / Order::TreeMixed). Three sums
 // have a canonical order and are reproduced as such:
 //   - buoyancy: 256 logical partials, partial t holding columns t, t+256, ...; each 32-partial group reduced by
 //     the halving tree (l absorbs l+16, l+8, ... l+1); the 8 group results combined 0..7 in order
-//     (reduce.hpp:12-107, hydro.hpp:242-258). A logical 32-lane group is one half of a wave64.
-//   - node pass: 32 lane partials over the node's segments with stride 32, then the same tree (hydro.hpp:306-316).
+//     (reduce.hpp:12-79, hydro.hpp:48-64). A logical 32-lane group is one half of a wave64.
+//   - node pass: 32 lane partials over the node's segments with stride 32, then the same tree (hydro.hpp:112-122).
 //   - every other sum (sea inflow, node gathers, loads, monitors) is serial in the reference's index order, one
 //     chain per lane where several independent chains exist.
 // All other arithmetic is element-wise and written operation for operation as in the reference; the library is
@@ -58,7 +58,7 @@ constexpr double kPitchClamp = 1.35;
 constexpr double kRollClamp = 1.4;
 constexpr double kPi = 3.141592653589793;
 
-// JavaScript Math.max / Math.min for ordinary numbers (reference: kernel/math.hpp:338-339)
+// JavaScript Math.max / Math.min for ordinary numbers (reference: kernel/math.hpp:29-30)
 __device__ __forceinline__ double jmax(double a, double b) { return (b > a) ? b : a; }
 __device__ __forceinline__ double jmin(double a, double b) { return (b < a) ? b : a; }
 
@@ -228,8 +228,8 @@ __device__ inline Blk carve(SSL double* base, PC& P) {
 // Out of line: called four lanes at a time once per step, and for the 30 s readouts.
 __device__ __noinline__ double trig(double x, int wantCos) { return fdl::sin_or_cos(x, wantCos); }
 
-// reference: kernel/frame.hpp:10-21, written to the frame slots, with iR = 1 / R22 (hydro.hpp:209,245,264,309)
-// and the f32 frame of reduce.hpp:118-126
+// reference: kernel/frame.hpp:10-21, written to the frame slots, with iR = 1 / R22 (hydro.hpp:15,51,70,115)
+// and the f32 frame of reduce.hpp:86-98
 __device__ inline void write_frame(PC& P, SSL double* fr, double zO, double cth, double sth, double cph, double sph) {
   const double R00 = cth, R01 = sth * sph, R02 = sth * cph;
   const double R10 = 0, R11 = cph, R12 = -sph;
@@ -345,7 +345,7 @@ struct PP {
   T vol, dv, mx, my, mz, zmin, zmax;
 };
 
-// One level of the halving tree (reduce.hpp:102-107: l absorbs l + 16, 8, 4, 2, 1; the total in lane 0 of the
+// One level of the halving tree (reduce.hpp:74-79: l absorbs l + 16, 8, 4, 2, 1; the total in lane 0 of the
 // group). Lanes l < OFF absorb lane l + OFF; the other lanes are never read again, so they add too (their sums are
 // discarded) and no lane mask is needed.
 template <int OFF, class T>
@@ -428,7 +428,7 @@ __device__ __forceinline__ Geo<float> geo_at(const SSG f4v* table, uint32_t off)
 __device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<double>); }
 __device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<float>); }
 
-// one column into a buoyancy partial (reduce.hpp:67-77 / 159-170)
+// one column into a buoyancy partial (reduce.hpp:39-49 / 131-142)
 __device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, HP<double>& p) {
   const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
@@ -452,7 +452,7 @@ __device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, HP<fl
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
 }
 
-// One segment into a node-pass partial (reduce.hpp:80-99 / 172-192). The world extent of the node (zmin, zmax)
+// One segment into a node-pass partial (reduce.hpp:52-71 / 144-164). The world extent of the node (zmin, zmax)
 // depends on the pose only, not on the level: it is accumulated (Ext) in the first pass of a node in a step and
 // reused by the later ones, which the reference recomputes to the same values.
 template <bool Ext>
@@ -537,8 +537,8 @@ struct Pass {
   double vol, dv, mx, my, mz, zmin, zmax;
 };
 
-// The node pass of node n at level h by one logical 32-lane group (hydro.hpp:306-316 / 338-349; finish_pass
-// hydro.hpp:295-303); the result is in every lane of the group. Both groups of a wave call it together, each with
+// The node pass of node n at level h by one logical 32-lane group (hydro.hpp:112-122 / 144-155; finish_pass
+// hydro.hpp:101-109); the result is in every lane of the group. Both groups of a wave call it together, each with
 // its own request; n < 0 is no request (an empty pass whose result is not used). zmin and zmax are computed only
 // with Ext.
 template <bool Mixed, bool Ext>
@@ -970,7 +970,7 @@ constexpr int kFetch = 4;  // incidence records a lane fetches before using them
 
 __device__ __forceinline__ double head_of(const Blk& B, int n) { return B.vol[n] > kEmptyVolume ? B.level[n] : kNoHead; }
 
-// flows.hpp:211-231: effective head and area of node n, and the number of connections that count toward the
+// flows.hpp:15-35: effective head and area of node n, and the number of connections that count toward the
 // limiter there (an enabled connection not between two empty spaces). The heads of the other ends are
 // recomputed from vol and level, so that one pass over the nodes suffices. Four lanes (a quad) per node: lane j
 // counts the node's incidence entries j, j + 4, ... (integer counts: their sum is the same in any order).
@@ -1010,7 +1010,7 @@ __device__ __noinline__ double villemonte(double r) {
 __device__ __noinline__ double pow_weir(double r) { return fdl::pow(r, kWeirExponent); }
 __device__ __noinline__ double pow_villemonte(double g) { return fdl::pow(g, kVillemonteExponent); }
 
-// flows.hpp:234-274: the volume moved through connection c this step (a -> b positive), after the limiter. carea
+// flows.hpp:38-78: the volume moved through connection c this step (a -> b positive), after the limiter. carea
 // holds the instance's products coef * area, the first multiplication of both flow laws, formed on the host.
 __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, const Blk& B, int c) {
   const i4v ci = P.M.conI[c];
@@ -1054,7 +1054,7 @@ __device__ inline double connection_flow(PC& P, const SSL double* fr, double t, 
   return dV;
 }
 
-// flows.hpp:277-307 for node n (the gather in increasing connection order and the capped volume update), then
+// flows.hpp:81-111 for node n (the gather in increasing connection order and the capped volume update), then
 // the node's terms of the loads (body.hpp:21-34), which depend on this node only once its volume is final.
 __device__ inline void node_update(PC& P, const Blk& B, int n) {
   const int s = P.M.incStart[n], e = s + P.M.incCount[n];
@@ -1079,7 +1079,7 @@ __device__ inline void node_update(PC& P, const Blk& B, int n) {
     const double cap = vm + B.nAov[n] * head;
     if (v > cap) { excess = v - cap; v = cap; }
   }
-  // The excess is subtracted from the sea inflow in node order (flows.hpp:310-314) and the loads take every node
+  // The excess is subtracted from the sea inflow in node order (flows.hpp:114-118) and the loads take every node
   // with !(vol <= 0) (body.hpp:21-34). The tail adds these terms in node order without tests: a skipped node
   // contributes -0.0 (x + (-0.0) == x for every x) and a subtracted term is stored negated (x - y == x + (-y)).
   B.excess[n] = -excess;
@@ -1161,7 +1161,7 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   }
 }
 
-// the 8 group results combined in order, then finish_hydro (hydro.hpp:231-239, 255-257)
+// the 8 group results combined in order, then finish_hydro (hydro.hpp:37-45, 61-63)
 __device__ inline void finish_buoyancy(const Blk& B, double& V, double& bx, double& by, double& bz, double& top) {
   double tV = B.warpHyd[0], tMx = B.warpHyd[1], tMy = B.warpHyd[2], tMz = B.warpHyd[3], tTop = B.warpHyd[4];
   for (int w = 1; w < kUnits; ++w) {
@@ -1183,7 +1183,7 @@ __device__ __forceinline__ double bcast(double v, int from) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// The tail of a step on wave 0 (flows.hpp:310-314, step.hpp:147-152, body.hpp, events.hpp). Lanes 0..6 run the
+// The tail of a step on wave 0 (flows.hpp:114-118, step.hpp:48-53, body.hpp, events.hpp). Lanes 0..6 run the
 // seven serial node-order chains (tauX, tauY, Fz, mw, Iw_p, Iw_r, and the sea inflow less the excess) over terms
 // node_update stored gated and signed, eight nodes' values fetched at a time; every lane then integrates the same uniform values, the lanes test the monitors
 // and marks side by side and lane 0 records the crossings in order; lanes 0..3 evaluate the pose of the next step.

@@ -252,7 +252,7 @@ class CompiledSim:
         return int(self.vol0.size)
 
     def kind_index(self, name: str) -> int:
-        """CompiledSim::kindIndex (engine/src/model.cpp:131-135): -1 when the kind is not defined."""
+        """CompiledSim::kindIndex (engine/src/model.cpp:20-24): -1 when the kind is not defined."""
         try:
             return self.kindNames.index(name)
         except ValueError:
@@ -461,5 +461,5 @@ def check_pairing(ship: CompiledShip, sim: CompiledSim) -> None:
 
 
 def max_group_size(sim: CompiledSim) -> int:
-    """CompiledSim::maxGroupSize (engine/src/model.cpp:124-129)."""
+    """CompiledSim::maxGroupSize (engine/src/model.cpp:13-18)."""
     return max(1, int(sim.gNodeCount.max()) if sim.gNodeCount.size else 1)

@@ -62,7 +62,7 @@ def _integer(o: dict, key: str, where: str) -> int:
 
 
 def load_sweep_doc(doc: dict, sim, path: str = "<sweep>") -> Sweep:
-    """sweep::load_sweep (engine/src/sweep.cpp:140-185) on a parsed document."""
+    """sweep::load_sweep (engine/src/sweep.cpp:30-75) on a parsed document."""
     if doc.get("format") != "sinksim.sweep":
         raise RuntimeError(f"{path}: format is not sinksim.sweep")
     if _integer(doc, "formatVersion", path) != 1:
@@ -120,7 +120,7 @@ def load_sweep(path: str, sim) -> Sweep:
 
 
 def sweep_document(sw: Sweep, sim) -> dict:
-    """The document save_sweep writes (engine/src/sweep.cpp:187-222)."""
+    """The document save_sweep writes (engine/src/sweep.cpp:77-112)."""
     insts = []
     for inst in sw.instances:
         j = {"id": inst.id}
@@ -213,7 +213,7 @@ CURVE_KEYS = ("t", "trim", "list", "water", "inflow", "draftF", "draftA")
 
 
 def results_document(r: Results) -> dict:
-    """save_results (engine/src/sweep.cpp:273-311)."""
+    """save_results (engine/src/sweep.cpp:163-201)."""
     insts = []
     for x in r.instances:
         c = np.asarray(x.curve).reshape(-1, 7)
@@ -232,7 +232,7 @@ def save_results(r: Results, path: str) -> None:
 
 
 def load_results(path: str) -> Results:
-    """load_results (engine/src/sweep.cpp:313-342)."""
+    """load_results (engine/src/sweep.cpp:203-232)."""
     with open(path, "r", encoding="utf-8") as f:
         doc = json.load(f)
     if doc.get("format") != "sinksim.batch-results":
@@ -257,7 +257,7 @@ class Diff:
 
 
 def compare(a: Results, b: Results) -> Diff:
-    """sweep::compare (engine/src/sweep.cpp:344-391): exact, value for value."""
+    """sweep::compare (engine/src/sweep.cpp:234-281): exact, value for value."""
     d = Diff()
 
     def note(where, x, y):

@@ -1,4 +1,4 @@
-"""sinksim.trace files (docs/FORMATS.md; reference: engine/src/trace.cpp:57-225).
+"""sinksim.trace files (docs/FORMATS.md; reference: engine/src/trace.cpp:11-179).
 
 A trace holds the complete state at every record, dense for the first steps and sampled afterwards. Here the
 records are [n, 10 + 5 * nodes] arrays in the engine's record layout: t zO pitch roll vz wth wph inflow Vb hullTop,
@@ -65,7 +65,7 @@ def _column(rec: np.ndarray, f: str, nodes: int) -> np.ndarray:
 
 
 def load_trace(path: str, verify: bool = True) -> Trace:
-    """io::load_trace (engine/src/trace.cpp:152-187); reads .json or .json.gz."""
+    """io::load_trace (engine/src/trace.cpp:106-141); reads .json or .json.gz."""
     opener = gzip.open if path.endswith(".gz") else open
     with opener(path, "rb") as fh:
         doc = json.loads(fh.read())
@@ -107,7 +107,7 @@ def _num(x: float):
 
 
 def save_trace(tr: Trace, path: str) -> None:
-    """io::save_trace (engine/src/trace.cpp:189-225): columnar records with per-column and chained FNV hashes."""
+    """io::save_trace (engine/src/trace.cpp:143-179): columnar records with per-column and chained FNV hashes."""
     doc = {"format": "sinksim.trace", "formatVersion": TRACE_FORMAT_VERSION, "role": tr.role, "ship": tr.ship,
            "sim": tr.sim, "meta": {"title": tr.title, "producer": tr.producer, "engine": tr.engine},
            "dt": tr.dt, "nodes": tr.nodes, "denseSteps": tr.denseSteps, "sampleEvery": tr.sampleEvery,

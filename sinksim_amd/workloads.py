@@ -77,7 +77,7 @@ def uneven_fleet(count: int = 4096, seed: int = 20261008, processes: int = 0):
 
 def twin_experiment(count: int = 256, nparams: int = 12, seed: int = 20261008, span: float = 0.3):
     """The gradient workload's parameters: per instance hidden true scale factors and an independent starting guess,
-    both log-uniform in [exp(-span), exp(span)] (the reference's search box, tools/py/calibrate_gpu.py:90-91),
+    both log-uniform in [exp(-span), exp(span)] (the reference's search box, tools/py/calibrate_gpu.py:84,106),
     from one seed. Returns (theta_true, theta_guess), each [count, nparams]."""
     rng = np.random.default_rng(seed)
     truth = np.exp(rng.uniform(-span, span, (count, nparams)))
@@ -101,6 +101,6 @@ def calibration_hull():
 
 def calibration_problems(count: int, seed: int = 20261008, span: float = 0.3, nparams: int = 24):
     """Hidden true log scale factors of `count` twin experiments, uniform in [-span, span] (the reference's search
-    box around scales 1, tools/py/calibrate_gpu.py:90-91): [count, nparams]."""
+    box around scales 1, tools/py/calibrate_gpu.py:84,106): [count, nparams]."""
     rng = np.random.default_rng(seed)
     return rng.uniform(-span, span, (count, nparams))

@@ -12,7 +12,7 @@ from conftest import GOLDEN, ROOT
 
 pytestmark = pytest.mark.gpu
 
-PROBE = os.path.join(ROOT, "sinksim_amd", "libsinksim_probe.so")
+PROBE = os.path.join(ROOT, "tests", "probe", "libsinksim_probe.so")
 
 
 def _lib():

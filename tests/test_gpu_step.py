@@ -1,5 +1,5 @@
 """The single-step entry point (kernel::step over the batch, ss_batch_step) held to the reference's acceptance
-procedure of apps/sinksim_check.cpp:109-127, through the C ABI, at tolerance 0:
+procedure of apps/sinksim_check.cpp:110-126, through the C ABI, at tolerance 0:
 
 - dense single steps: restore dense record i, step once, compare with dense record i + 1 (all 400 steps of the golden,
   one instance each);

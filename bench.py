@@ -28,13 +28,15 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two operations)
 # Algorithmic FP64 operations per simulation step on this workload, counted by the oracle's operation counter over
-# the first 32 instances of the sweep (per step: 4,074 wet and 335 dry column evaluations, 2,760 partial, 1,799 full
-# and 5,135 dry segment evaluations, 72.8 node passes, 51.6 active connections, 34.5 pow calls) and priced by
-# oracle/pyoracle.py:flops_per_step: 2.44e5. tests/test_bench_constants.py re-counts a sample and pins this figure.
-FLOPS_PER_STEP = 2.44e5
+# the first 32 instances of the sweep, each to its own end (per step: 4,062 wet and 346 dry column evaluations, 2,865
+# partial, 1,229 full and 5,258 dry segment evaluations, 70.4 node passes, 44.3 active connections, 24.1 pow calls)
+# and priced by oracle/pyoracle.py:flops_per_step: 2.34e5. tests/test_bench_constants.py pins the counter against an
+# independent count of the reference's own kernels on the 1912 run and re-counts a sample of this workload.
+FLOPS_PER_STEP = 2.34e5
 # HBM bytes per launch of run_batch<false, false, 256> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
 # plain copy kernel: profiles/run_batch_fp64_titanic64_batch1024.md).
 TRAFFIC_BYTES_PER_LAUNCH = 235.1e6
+TRAFFIC_SOURCE = "profiles/run_batch_fp64_titanic64_batch1024.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
 def parse():
@@ -63,15 +65,29 @@ def parse():
     ap.add_argument("--tmax", type=float, default=None, help="s (default 4 h titanic, 2 h uneven)")
     ap.add_argument("--numerics", default="portable", choices=["portable", "portable32"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=48, help="instances in the CPU-baseline sample")
+    ap.add_argument("--cpu-sample", type=int, default=8, help="instances per host thread in the CPU-baseline sample")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first fill and copy MIB MiB with the probe library's plain kernels (ssp_fill, ssp_copy): the "
                          "known byte counts that calibrate a profiler's FETCH_SIZE / WRITE_SIZE")
     return ap.parse_args()
 
 
-def cpu_baseline(sw, sim, count: int) -> dict:
-    """The reference's CPU sweep runner (or the oracle's) on the first `count` instances, all host cores."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(sw, sim, per_thread: int, numerics: str) -> dict:
+    """The reference's CPU sweep runner (or the oracle's restatement) on the first `per_thread` x cores instances of the
+    same sweep, every host core of a one-GPU host's share, in portable and portable32 numerics. The rate is the tool's
+    own aggregate line (run_cpu alone: no file IO, no process start); the leg of the bench's numerics is the value."""
+    import re
     from sinksim_amd import sweep as S
     from sinksim_amd.data_paths import SHIP, sim_path
     cores = os.cpu_count() or 1
@@ -80,33 +96,38 @@ def cpu_baseline(sw, sim, count: int) -> dict:
     except AttributeError:
         pass
     cores = min(cores, 16)  # a one-GPU host's CPU share
+    count = min(len(sw.instances), per_thread * cores)
     sample = S.Sweep(ship=sw.ship, sim=sw.sim, tMax=sw.tMax, every=sw.every, instances=sw.instances[:count])
     ref = os.path.join(ROOT, "oracle", "_ref", "sinksim_sweep")
+    legs = {}
     if os.path.exists(ref):
+        kind = "reference"
         with tempfile.TemporaryDirectory() as d:
             path = os.path.join(d, "sample.sweep.json")
             S.save_sweep(sample, path, sim)
-            res = os.path.join(d, "r.json")
-            t0 = time.perf_counter()
-            subprocess.run([ref, SHIP, sim_path(sim.id), path, "--numerics", "portable", "--threads", str(cores),
-                            "--results", res], check=True, capture_output=True)
-            secs = time.perf_counter() - t0
-            steps = sum(x.steps for x in S.load_results(res).instances)
-        kind = "reference"
+            for num in ("portable", "portable32"):
+                out = subprocess.run([ref, SHIP, sim_path(sim.id), path, "--numerics", num, "--threads", str(cores)],
+                                     check=True, capture_output=True, text=True).stdout
+                m = re.search(r"(\d+) steps in ([0-9.]+) s \(([0-9]+) steps/s aggregate\)", out)
+                if not m:
+                    raise RuntimeError(f"no aggregate line in the CPU sweep's output:\n{out}")
+                legs[num] = (int(m.group(1)), float(m.group(2)))
     else:
+        kind = "port"
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyoracle
         from sinksim_amd.data_paths import titanic
         ship, _ = titanic(sim.id)
         en, area, tOn = S.connection_tables(sample, sim)
-        t0 = time.perf_counter()
-        out = pyoracle.sweep(ship, sim, "portable", en, area, tOn, sw.tMax, sw.every, threads=cores)
-        secs = time.perf_counter() - t0
-        steps = sum(o["steps"] for o in out)
-        kind = "port"
-    return {"value": round(steps / secs, 1), "unit": "sim-steps/s", "cores": cores, "kind": kind,
-            "sample": f"first {count} instances of the same 1,024-instance sweep, portable numerics, "
-                      f"{steps} steps in {secs:.1f} s (file IO included)"}
+        for num in ("portable", "portable32"):
+            t0 = time.perf_counter()
+            out = pyoracle.sweep(ship, sim, num, en, area, tOn, sw.tMax, sw.every, threads=cores)
+            legs[num] = (sum(o["steps"] for o in out), time.perf_counter() - t0)
+    steps, secs = legs[numerics]
+    return {"value": round(steps / secs, 1), "unit": "sim-steps/s", "cores": cores, "kind": kind, "cpu": _cpu_model(),
+            "legs": {k: round(v[0] / v[1], 1) for k, v in legs.items()},
+            "sample": f"first {count} instances ({per_thread} per thread) of the same 1,024-instance sweep; {numerics}: "
+                      f"{steps} steps in {secs:.1f} s (the sweep's run alone, no file IO)"}
 
 
 def _device() -> str:
@@ -415,13 +436,18 @@ def main():
                    "sim_steps_per_bench_step": tot_steps // args.steps,
                    # full sinkings per minute at instance 0's length, as apps/sinksim_cuda_run.cpp:138-140 reports it
                    "sinkings_per_min": round(value / max(steps0, 1) * 60.0, 1), "device": _device()},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        # the FP64 vector ALU bounds this kernel (no dense contraction, so no MFMA); 78.6 TFLOP/s is MI355X's FP64
+        # vector peak with an FMA counted as two operations
+        "roofline": {"bound": "valu-fp64", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                     # HBM bytes per launch from the FETCH_SIZE and WRITE_SIZE counters of a profiled run of this
+                     # workload, scaled by a copy kernel's reading (a constant here; the run itself is not counted)
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH if (args.count == 1024 and not mixed) else None,
+                     "traffic_source": TRAFFIC_SOURCE if (args.count == 1024 and not mixed) else None,
                      "kernel": "run_batch<false>" if not mixed else "run_batch<true>", "kernel_ms": round(per_launch_ms, 3)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample, args.numerics)
     print(json.dumps(out), flush=True)
 
 

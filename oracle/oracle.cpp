@@ -378,6 +378,10 @@ void orc_sweep(const orc_ship* s, const orc_sim* m, int numerics, int32_t count,
   if (threads <= 0) threads = static_cast<int>(std::thread::hardware_concurrency());
   if (threads <= 0) threads = 1;
   const int n = std::min<int>(threads, count);
+  if (n <= 1) {  // on the calling thread (its thread-local operation counter then sees the runs)
+    worker();
+    return;
+  }
   std::vector<std::thread> pool;
   for (int t = 0; t < n; ++t) pool.emplace_back(worker);
   for (std::thread& t : pool) t.join();

@@ -318,29 +318,21 @@ def pow_v8(x: float, y: float) -> float:
 
 
 def op_counts(ship, sim, numerics, en, area, tOn, t_max, every, model: Model | None = None) -> dict:
-    """Algorithmic operation counts of running every instance (one thread): column evaluations wet/dry, segment
-    evaluations partial/full/dry, node passes, steps, active connections, pow calls, wet nodes in the loads."""
+    """Algorithmic operation counts of running every instance to founder or t_max (one thread, the sweep's own run
+    loop, so no step past the end is counted): column evaluations wet/dry, segment evaluations partial/full/dry,
+    node passes, steps, active connections, pow calls, wet nodes in the loads."""
     model = model or Model(ship, sim)
     L = lib()
     L.orc_count_begin.argtypes = []
     L.orc_count_end.argtypes = [C.POINTER(C.c_int64)]
-    en, area, tOn = _u(en), _f(area), _f(tOn)
     L.orc_count_begin()
-    for i in range(en.shape[0]):
-        S = OracleSim(ship, sim, numerics, model)
-        for c in range(sim.connections):
-            S.set_conn(c, bool(en[i, c]), float(area[i, c]), float(tOn[i, c]))
-        S.reset()
-        while True:
-            ev, f, _ = S.events()
-            t = S.record()[0]
-            if f or t >= t_max:
-                break
-            S.step(min(4096, max(1, int((t_max - t) / sim.dt))))
+    runs = sweep(ship, sim, numerics, en, area, tOn, t_max, every, threads=1, model=model)
     out = np.zeros(10, np.int64)
     L.orc_count_end(out.ctypes.data_as(C.POINTER(C.c_int64)))
     keys = ("cwet", "cdry", "spart", "sfull", "sdry", "passes", "steps", "active", "pows", "wetNodes")
-    return dict(zip(keys, (int(v) for v in out)))
+    c = dict(zip(keys, (int(v) for v in out)))
+    assert c["steps"] == sum(r["steps"] for r in runs), (c["steps"], [r["steps"] for r in runs])
+    return c
 
 
 def flops_per_step(c: dict, nn: int, nconn: int, nsea: int) -> float:

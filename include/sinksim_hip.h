@@ -305,8 +305,11 @@ typedef struct ss_gradient_options {
 /* Runs every instance from its staged initial state and returns its loss [count], the loss's gradient with respect to
  * theta [count * nparams] (null: forward only, no reverse sweep), the loss's signals [count * (nRead + nEvent)]: readout
  * values then event times (may be null), the steps run [count], the recomputed steps whose values differed from the
- * forward's [count] (always 0 for a correct kernel; may be null) and the kernel time (ms, may be null). FP64
- * (portable) batches only. The launch takes 512 threads per simulation where the forward layout and the whole tape
+ * forward's [count] (always 0 for a correct kernel; may be null) and the kernel time (ms, may be null). The forward
+ * runs in the batch's numerics: a portable batch is FP64 throughout; a portable32 batch runs the forward and its
+ * recomputations with the portable32 step (the batch kernel's trajectory bit for bit) and the reverse in FP64, with
+ * the local Jacobians of the FP64 step at the portable32 states; its mismatch count checks each recomputed segment's
+ * last state against the forward's. The launch takes 512 threads per simulation where the forward layout and the whole tape
  * fit in LDS, else 256, else the same two with the tape's Jacobian rows and per-connection partials in global memory
  * (ss_batch_gradient_launch reports which). The batch may be run any number of times this way; its own run() is
  * unaffected. */
@@ -319,7 +322,7 @@ ss_status ss_batch_gradient(ss_batch* b, int32_t nparams, const int32_t* pa, con
  * entries 0 and 7 are ignored) and seeds on the step's monitor flow
  * sums [nmonitor] and marks' world z [nmark] (either may be null), the adjoint of the state before it (lambdaOut,
  * trace layout), the loss's derivative with respect to log theta (gradLog [nparams]) and the recomputed state after
- * the step (recordOut, trace layout; inflow and hull top 0). */
+ * the step (recordOut, trace layout; inflow and hull top 0). FP64 (portable) batches only. */
 ss_status ss_batch_adjoint_step(ss_batch* b, int32_t nparams, const int32_t* pa, const int32_t* pc, const double* theta, int32_t relaxed,
                                 const double* record, const double* lambda, const double* monitorSeed, const double* markSeed,
                                 double* lambdaOut, double* gradLog, double* recordOut);

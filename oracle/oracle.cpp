@@ -175,6 +175,10 @@ class Sim {
     }
   }
 
+  // the world z of mark i at the current state's pose, and monitor m's flow sum of the last step
+  Real mark_z(Index i) const { return world_z(frame_now(), sim_.mkx[i], sim_.mky[i], sim_.mkz[i]); }
+  Real monitor_sum(Index m) const { return over_[m]; }
+
   Index nodes() const { return ship_.nn; }
   const State& state() const { return st_; }
   const Outputs& outputs() const { return out_; }
@@ -626,6 +630,70 @@ int64_t orc_dual_run(const orc_ship* s, const orc_sim* m, const uint8_t* en, con
   if (nd <= 12) return dual_run<12>(s, m, en, area, tOn, prm, theta, nd, seed, tMax, maxSteps, L, lossOut, sigOut, recOut);
   if (nd <= 24) return dual_run<24>(s, m, en, area, tOn, prm, theta, nd, seed, tMax, maxSteps, L, lossOut, sigOut, recOut);
   return -1;
+}
+
+// The loss of one run (orc_loss) in plain arithmetic and any numerics: the sweep's step with per-connection
+// discharge coefficients coef[nc] (the scenario's, scaled) and areas area[nc] (scaled), from the scenario's initial
+// state to tMax, founder or maxSteps; the same terms, signals and order as the dual build's run. Writes the loss and
+// the signals (readout values, then event times); returns the step count.
+int64_t orc_loss_run(const orc_ship* s, const orc_sim* m, int numerics, const uint8_t* en, const double* area, const double* tOn,
+                     const double* coef, double tMax, int64_t maxSteps, const orc_loss* L, double* lossOut, double* sigOut) {
+  orc_sim mc = *m;
+  mc.coef = coef;
+  Sim o(s, &mc, numerics);
+  for (int32_t c = 0; c < m->nc; ++c) o.set_conn(c, en[c], area[c], tOn[c]);
+  const int32_t nT = L ? L->nRead : 0, nE = L ? L->nEv : 0;
+  std::vector<double> rd(nT, 0.0), tstar(nE, 0.0), prev(nE, 0.0), zpre(nE, 0.0);
+  std::vector<char> rdSet(nT, 0), fired(nE, 0);
+  const double dt = m->dt;
+  int64_t steps = 0;
+  while (o.state().t < tMax && !o.events().foundered && steps < maxSteps) {
+    for (int32_t e = 0; e < nE; ++e)
+      if (L->evKind[e] == 1) zpre[e] = o.mark_z(L->evIdx[e]);
+    o.step();
+    ++steps;
+    const State& st = o.state();
+    for (int32_t j = 0; j < nT; ++j)
+      if (L->rdStep[j] == steps) { rd[j] = (L->rdKind[j] == 0 ? st.pitch : st.roll) * 180.0 / kPi; rdSet[j] = 1; }
+    for (int32_t e = 0; e < nE; ++e) {
+      double sg;
+      const int32_t kind = L->evKind[e], idx = L->evIdx[e];
+      if (kind == 0) sg = o.monitor_sum(idx) - m->monThreshold;
+      else if (kind == 1) sg = m->mkWhen[idx] == 0 ? -zpre[e] : zpre[e];
+      else {
+        sg = m->founder[0] - o.outputs().hullTop;
+        const double sp = std::fabs(st.pitch) - m->founder[1];
+        if (sp > sg) sg = sp;
+        if (st.t > m->founder[3]) {
+          const double sb = m->founder[2] - o.outputs().Vb;
+          if (sb > sg) sg = sb;
+        }
+      }
+      if (!fired[e] && sg > 0) {
+        fired[e] = 1;
+        tstar[e] = steps == 1 ? st.t : st.t - dt * sg / (sg - prev[e]);
+      }
+      prev[e] = sg;
+    }
+  }
+  const bool foundered = o.events().foundered != 0;
+  double loss = 0;
+  for (int32_t j = 0; j < nT; ++j) {
+    if (!rdSet[j]) rd[j] = L->rdAfter[j];
+    const double r = rd[j] - L->rdObs[j];
+    loss += L->rdW[j] * (r * r);
+  }
+  for (int32_t e = 0; e < nE; ++e) {
+    if (!fired[e]) tstar[e] = foundered ? o.events().founderT : L->evMissing[e];
+    const double r = tstar[e] - L->evObs[e];
+    loss += L->evW[e] * (r * r);
+  }
+  if (lossOut) *lossOut = loss;
+  if (sigOut) {
+    for (int32_t j = 0; j < nT; ++j) sigOut[j] = rd[j];
+    for (int32_t e = 0; e < nE; ++e) sigOut[nT + e] = tstar[e];
+  }
+  return steps;
 }
 
 // One step of the dual build from a state record and its tangent along one direction (drec, same layout;

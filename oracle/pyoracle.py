@@ -99,6 +99,9 @@ def lib() -> C.CDLL:
                                C.c_double, C.c_int64, C.POINTER(OrcLoss), _dp, _dp, _dp]
     L.orc_dual_step.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), _bp, _dp, _dp, C.POINTER(OrcParams), _dp, _dp, _dp, _dp,
                                 _dp, _dp, _dp, _dp]
+    L.orc_loss_run.restype = C.c_int64
+    L.orc_loss_run.argtypes = [C.POINTER(OrcShip), C.POINTER(OrcSim), C.c_int, _bp, _dp, _dp, _dp, C.c_double, C.c_int64,
+                               C.POINTER(OrcLoss), _dp, _dp]
     _lib = L
     return L
 
@@ -400,6 +403,26 @@ def dual_run(model: Model, en, area, tOn, pa, pc, theta, seed=None, t_max: float
     return {"steps": int(steps), "loss": float(loss[0]), "dloss": loss[1:].copy(),
             "signals": sig[:nsig].copy(), "dsignals": sig[max(nsig, 1):].reshape(nd, -1)[:, :nsig].copy() if nsig else np.zeros((nd, 0)),
             "record": rec[:R].copy(), "drecord": rec[R:].reshape(nd, R).copy()}
+
+
+def loss_run(model: Model, numerics: str, en, area, tOn, pa, pc, theta, spec, t_max: float = 1e30,
+             max_steps: int = 2**62) -> dict:
+    """One instance's loss in plain arithmetic and the given numerics: connection c takes area * theta[pa[c]] and
+    coefficient coef * theta[pc[c]] (-1: no factor), the loss terms and signals of the dual build's run. Returns steps,
+    loss and signals [nRead + nEv]."""
+    theta = _f(theta)
+    pa, pc = np.asarray(pa), np.asarray(pc)
+    area = _f(np.where(pa >= 0, np.asarray(area, np.float64) * theta[np.maximum(pa, 0)], area))
+    coef = _f(np.where(pc >= 0, model.sim.cCoef * theta[np.maximum(pc, 0)], model.sim.cCoef))
+    L, keep = _loss(spec)
+    en, tOn = _u(en), _f(tOn)
+    nsig = spec.signals
+    loss = np.zeros(1)
+    sig = np.zeros(max(nsig, 1))
+    steps = lib().orc_loss_run(C.byref(model.s), C.byref(model.m), NUMERICS[numerics], _p(en, C.c_uint8), _p(area, C.c_double),
+                               _p(tOn, C.c_double), _p(coef, C.c_double), float(t_max), int(max_steps), C.byref(L),
+                               _p(loss, C.c_double), _p(sig, C.c_double))
+    return {"steps": int(steps), "loss": float(loss[0]), "signals": sig[:nsig].copy()}
 
 
 def dual_step(model: Model, en, area, tOn, pa, pc, theta, dtheta, rec, drec, relaxed: bool = False):

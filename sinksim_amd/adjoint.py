@@ -23,6 +23,7 @@ class GradientOptions:
     relaxed: bool = True
     t_max: float = 0.0       # 0: the batch's
     max_steps: int = 0       # <= 0: none
+    numerics: str = "portable"  # the forward's: "portable" (FP64) or "portable32" (FP32 partials; the reverse stays FP64)
 
 
 class GradientRunner:
@@ -37,8 +38,11 @@ class GradientRunner:
         self.params = params
         self.spec = spec
         self.options = options or GradientOptions()
-        self.batch = Batch.fleet(self.models, self.inst_model, BatchOptions(count=self.count, mixed_precision=False, traced_instances=0,
-                                                                             curve_every=0.0, t_max=t_max))
+        if self.options.numerics not in ("portable", "portable32"):
+            raise ValueError(f"numerics must be portable or portable32, not {self.options.numerics!r}")
+        self.batch = Batch.fleet(self.models, self.inst_model,
+                                 BatchOptions(count=self.count, mixed_precision=self.options.numerics == "portable32",
+                                              traced_instances=0, curve_every=0.0, t_max=t_max))
         if tables is not None:
             self.batch.set_connections(*tables)
         self.kernel_ms = 0.0

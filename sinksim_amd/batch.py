@@ -286,7 +286,8 @@ class Batch:
         [count, signals] when given) and its gradient with respect to theta [count, np], by the adjoint kernel with a
         checkpoint every `checkpoint_every` steps. params: a calib.ParamMap (one model) or one per model of a fleet.
         Returns loss [count], grad [count, np] (None without want_grad), signals [count, signals], steps [count],
-        mismatches [count] (recomputed steps whose values differed from the forward's: 0) and the kernel time (ms)."""
+        mismatches [count] (recomputed steps whose values differed from the forward's: 0) and the kernel time (ms).
+        The forward runs in the batch's numerics; a portable32 batch's reverse is FP64 at the portable32 states."""
         pa, pc, th, npar = self._param_tables(params, theta)
         nT, nE = len(spec.read_steps), len(spec.ev_kind)
         ts = np.ascontiguousarray(spec.read_steps, np.int64)

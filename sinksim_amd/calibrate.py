@@ -317,13 +317,13 @@ def _area_rows(sim, params, x):
     return np.asarray(sim.cArea, np.float64)[None, :] * fac
 
 
-def objective(tp: TwinProblems, x, rows=None) -> np.ndarray:
+def objective(tp: TwinProblems, x, rows=None, numerics: str = "portable") -> np.ndarray:
     """The differentiable loss (the adjoint's forward, no reverse sweep) of problems `rows` (default all) at log
     factors x [len(rows), np]."""
     from .adjoint import GradientOptions, GradientRunner
     rows = np.arange(tp.count) if rows is None else np.asarray(rows)
     run = GradientRunner([(tp.ship, tp.sim)], np.zeros(len(rows), np.int32), tp.params, tp.spec, tp.steps * tp.sim.dt,
-                         GradientOptions(checkpoint_every=1 << 16, relaxed=True, max_steps=tp.steps))
+                         GradientOptions(checkpoint_every=1 << 16, relaxed=True, max_steps=tp.steps, numerics=numerics))
     try:
         r = run(np.exp(np.asarray(x, np.float64)), observations=tp.obs[rows], want_grad=False)
     finally:
@@ -401,16 +401,17 @@ def twin_search(tp: TwinProblems, rows=None, rounds: int = 5, batch: int = 264, 
 
 
 def twin_lbfgs(tp: TwinProblems, rows=None, iterations: int = 12, target=None, checkpoint_every: int = 0, memory: int = 6,
-               max_step: float = 0.3, lo: float = -1.6, hi: float = 1.6, stop=None, log=None) -> Trace:
+               max_step: float = 0.3, lo: float = -1.6, hi: float = 1.6, stop=None, log=None, numerics: str = "portable") -> Trace:
     """lbfgs_core for each problem of `rows` from scales 1 (x = 0), every iteration one adjoint call over all of them
-    (relaxed gradients, through the torch.autograd.Function); target: per-problem objectives to stop at."""
+    (relaxed gradients, through the torch.autograd.Function); target: per-problem objectives to stop at; numerics:
+    the forward's (portable32: FP32 partials in the forward and its recomputations, the reverse in FP64)."""
     import torch
 
     from .adjoint import GradientOptions, GradientRunner, flood_loss
     rows = np.arange(tp.count) if rows is None else np.asarray(rows)
     P = len(rows)
     runner = GradientRunner([(tp.ship, tp.sim)], np.zeros(P, np.int32), tp.params, tp.spec, tp.steps * tp.sim.dt,
-                            GradientOptions(checkpoint_every=checkpoint_every, relaxed=True, max_steps=tp.steps))
+                            GradientOptions(checkpoint_every=checkpoint_every, relaxed=True, max_steps=tp.steps, numerics=numerics))
     obs = tp.obs[rows]
 
     def evaluate(x):

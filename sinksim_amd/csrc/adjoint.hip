@@ -13,6 +13,12 @@
 //      (jac_step, whose values are compared with the ring's next state) and pulled back (reverse_step).
 // The parameters enter each connection's product coef * area (the host forms the scaled products); the gradient
 // is the sum of the log-derivatives over the connections of a parameter, divided by the parameter.
+//
+// A batch of portable32 numerics runs the forward and the segment recomputations with the portable32 step (FP32
+// column and segment arithmetic), bit for bit the batch kernel's trajectory, and the reverse in FP64: the local
+// Jacobians are the FP64 step's, taken at the portable32 trajectory's states. The recomputation is then checked at
+// the segment ends (each recomputed segment's last state against the forward's next checkpoint) instead of step by
+// step, since the FP64 recomputation inside jac_step does not reproduce portable32 values.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -20,6 +26,7 @@
 #include <climits>
 #include <cmath>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/sinksim_hip.h"
@@ -58,43 +65,32 @@ struct DevAdj {
 using DC = const SSC DevAdj;
 using AMC = const SSC AdjModel;
 
-// the state of the forward layout as a record of 7 + 5 NN values (t zO pitch roll vz wth wph, vol, level, cx, cy, cz)
+// The state of the forward layout as a record of 7 + 5 NN values (t zO pitch roll vz wth wph, vol, level, cx, cy,
+// cz). Each copy runs as one loop over the scalars and one over the spaces with the five arrays side by side (a
+// select among the arrays by a computed index would become a table of their addresses in scratch memory).
 __device__ inline void save_state(PC& P, const Blk& B, SSG double* dst, int tid, int tpb) {
   const int NN = P.S.nn;
-  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
-    double v;
-    if (i < 7) v = B.scal[S_T + i];
-    else {
-      const int j = i - 7, a = j / NN, n = j - a * NN;
-      const SSL double* src = a == 0 ? B.vol : a == 1 ? B.level : a == 2 ? B.cx : a == 3 ? B.cy : B.cz;
-      v = src[n];
-    }
-    dst[i] = v;
+  if (tid < 7) dst[tid] = B.scal[S_T + tid];
+  for (int n = tid; n < NN; n += tpb) {
+    SSG double* d = dst + 7 + n;
+    d[0] = B.vol[n]; d[NN] = B.level[n]; d[2 * NN] = B.cx[n]; d[3 * NN] = B.cy[n]; d[4 * NN] = B.cz[n];
   }
 }
 __device__ inline void restore_state(PC& P, const Blk& B, const SSG double* src, int tid, int tpb) {
   const int NN = P.S.nn;
-  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
-    const double v = src[i];
-    if (i < 7) B.scal[S_T + i] = v;
-    else {
-      const int j = i - 7, a = j / NN, n = j - a * NN;
-      SSL double* d = a == 0 ? B.vol : a == 1 ? B.level : a == 2 ? B.cx : a == 3 ? B.cy : B.cz;
-      d[n] = v;
-    }
+  if (tid < 7) B.scal[S_T + tid] = src[tid];
+  for (int n = tid; n < NN; n += tpb) {
+    const SSG double* s = src + 7 + n;
+    B.vol[n] = s[0]; B.level[n] = s[NN]; B.cx[n] = s[2 * NN]; B.cy[n] = s[3 * NN]; B.cz[n] = s[4 * NN];
   }
 }
 template <bool GT>
 __device__ inline void load_tape_state(PC& P, const TapeT<GT>& T, const SSG double* src, int tid, int tpb) {
   const int NN = P.S.nn;
-  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
-    const double v = src[i];
-    if (i < 7) T.sv[i] = v;
-    else {
-      const int j = i - 7, a = j / NN, n = j - a * NN;
-      SSL double* d = a == 0 ? T.iVol : a == 1 ? T.iLvl : a == 2 ? T.iCx : a == 3 ? T.iCy : T.iCz;
-      d[n] = v;
-    }
+  if (tid < 7) T.sv[tid] = src[tid];
+  for (int n = tid; n < NN; n += tpb) {
+    const SSG double* s = src + 7 + n;
+    T.iVol[n] = s[0]; T.iLvl[n] = s[NN]; T.iCx[n] = s[2 * NN]; T.iCy[n] = s[3 * NN]; T.iCz[n] = s[4 * NN];
   }
 }
 
@@ -102,18 +98,20 @@ __device__ inline void load_tape_state(PC& P, const TapeT<GT>& T, const SSG doub
 template <bool GT>
 __device__ inline void verify_step(PC& P, const TapeT<GT>& T, const SSG double* next, int tid, int tpb, SSL int32_t* bad) {
   const int NN = P.S.nn;
-  for (int i = tid; i < 7 + 5 * NN; i += tpb) {
-    double v;
-    if (i < 7) {
-      const SSL double* o = T.tot;
-      v = i == 0 ? T.sv[0] + P.M.dt : i == 1 ? o[T_ZO1] : i == 2 ? o[T_P1] : i == 3 ? o[T_R1] : i == 4 ? o[T_VZ1] : i == 5 ? o[T_WTH1] : o[T_WPH1];
-    } else {
-      const int j = i - 7, a = j / NN, n = j - a * NN;
-      const SSL double* s = a == 0 ? T.vol1 : a == 1 ? T.oLvl : a == 2 ? T.oCx : a == 3 ? T.oCy : T.oCz;
-      v = s[n];
-    }
-    if (__builtin_bit_cast(uint64_t, v) != __builtin_bit_cast(uint64_t, next[i])) atomicAdd((int32_t*)bad, 1);
+  auto differ = [](double a, double b) { return __builtin_bit_cast(uint64_t, a) != __builtin_bit_cast(uint64_t, b); };
+  int nbad = 0;
+  if (tid < 7) {
+    const SSL double* o = T.tot;
+    const double v = tid == 0 ? T.sv[0] + P.M.dt : tid == 1 ? o[T_ZO1] : tid == 2 ? o[T_P1] : tid == 3 ? o[T_R1] : tid == 4 ? o[T_VZ1]
+                     : tid == 5 ? o[T_WTH1] : o[T_WPH1];
+    nbad += differ(v, next[tid]);
   }
+  for (int n = tid; n < NN; n += tpb) {
+    const SSG double* s = next + 7 + n;
+    nbad += differ(T.vol1[n], s[0]) + differ(T.oLvl[n], s[NN]) + differ(T.oCx[n], s[2 * NN]) + differ(T.oCy[n], s[3 * NN]) +
+            differ(T.oCz[n], s[4 * NN]);
+  }
+  if (nbad) atomicAdd((int32_t*)bad, nbad);
 }
 
 // Loads instance inst into the forward layout (run_instance's load, with the scaled products as carea).
@@ -157,8 +155,16 @@ __device__ __forceinline__ double event_signal(PC& P, int kind, int idx, double 
 
 constexpr int kMaxTPB = 512;
 
+// the views of the instance in the kernel's own phases (StepViews)
+#define SSA_VIEWS                                   \
+  const StepViews<GT> views_(pa, ta, lds, gslice);  \
+  PC& P = views_.P;                                 \
+  const Blk& B = views_.B;                          \
+  const TapeT<GT>& T = views_.T
+
 // The forward-only instantiation (FWD) is held to 128 VGPRs, so that two 512-thread workgroups share a compute unit.
-template <int kTPB, bool GT, bool FWD>
+// Mixed: the forward and its recomputations in portable32 numerics.
+template <int kTPB, bool GT, bool FWD, bool Mixed>
 __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FWD ? 4 : 1, 8))) void adjoint_run(const ModelParams* __restrict__ models, const AdjModel* __restrict__ amodels,
                                                      const DevBatch* __restrict__ batch, const DevAdj* __restrict__ adj) {
   BC& Bt = *(BC*)batch;
@@ -169,46 +175,57 @@ __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FWD ? 4 : 
   const int tid = threadIdx.x;
   const int inst = blockIdx.x;
   const int mi = Bt.instModel[inst];
-  PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(mi) * sizeof(ModelParams));
+  PC& P0 = *(PC*)((const SSC char*)models + static_cast<size_t>(mi) * sizeof(ModelParams));
   AMC& AMs = *(AMC*)((const SSC char*)amodels + static_cast<size_t>(mi) * sizeof(AdjModel));
-  AdjModel AM{};
-  AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
-  const Blk B = carve(lds, P);
-  const TapeT<GT> T = carve_tape<GT>(lds + P.L.total, D.gtape + static_cast<long long>(inst) * D.gStride, AMs.A);
-  const int NN = P.S.nn;
+  const AdjScalars AM{AMs.maxGroup, AMs.np, AMs.mode};
+  const uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P0));
+  const uint64_t ta = uniform64(reinterpret_cast<uint64_t>(&AMs));
+  SSG double* const gslice = D.gtape + static_cast<long long>(inst) * D.gStride;
+  const int NN = P0.S.nn;
   const int SS = 7 + 5 * NN;
   const int nT = D.nRead, nE = D.nEv, nSig = nT + nE;
   SSG double* ckpt = D.ckpt + D.ckptOff[inst];
   SSG double* ring = D.ring + D.ringOff[inst];
   const long long K = D.K;
-  // signals: readout value, readout set; per event (kEvRec): fired, step, s_k, s_{k-1}, previous s, time at firing,
-  // z before the step, previous margin code, code at firing, code before firing (founder margins: 0 hull top,
-  // +1 / -1 pitch by its sign, 2 buoyancy)
+  // signals (the tape's sig slots): readout value, readout set; per event (kEvRec): fired, step, s_k, s_{k-1},
+  // previous s, time at firing, z before the step, previous margin code, code at firing, code before firing (founder
+  // margins: 0 hull top, +1 / -1 pitch by its sign, 2 buoyancy)
   constexpr int kEvRec = 10;
-  SSL double* rdV = T.sig;
-  SSL double* rdSet = rdV + nT;
-  SSL double* ev = rdSet + nT;
 
-  load_instance(P, Bt, D, B, inst, tid, kTPB);
-  for (int i = tid; i < 2 * nT + kEvRec * nE; i += kTPB) T.sig[i] = 0;
-  if (tid == 0) bad = 0;
-  __syncthreads();
-  if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
-  __syncthreads();
+  {
+    SSA_VIEWS;
+    load_instance(P, Bt, D, B, inst, tid, kTPB);
+    for (int i = tid; i < 2 * nT + kEvRec * nE; i += kTPB) T.sig[i] = 0;
+    if (tid == 0) bad = 0;
+    __syncthreads();
+    if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
+    __syncthreads();
+  }
 
   // 1. forward with checkpoints and the loss's ingredients
-  const uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P));
   long long done = 0;
   for (;;) {
-    if (B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= D.tMax || done >= D.maxSteps) break;
-    if (done % K == 0) save_state(P, B, ckpt + (done / K) * SS, tid, kTPB);
-    if (tid == 0)
-      for (int e = 0; e < nE; ++e)
-        if (D.evKind[e] == 1) { const int i = D.evIdx[e]; ev[kEvRec * e + 6] = world_z(B.frame, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]); }
-    __syncthreads();
-    device_step<false, kTPB>(pa, lds, tid);
+    {
+      SSA_VIEWS;
+      // every thread reads the same state (after the step's last barrier): the exit is uniform, and so are the step
+      // count and every loop over it below
+      const int stop = B.scal[S_FOUNDERED] != 0 || B.scal[S_T] >= D.tMax || done >= D.maxSteps;
+      if (__builtin_amdgcn_readfirstlane(stop)) break;
+      if (done % K == 0) save_state(P, B, ckpt + (done / K) * SS, tid, kTPB);
+      SSL double* ev = T.sig + 2 * nT;
+      if (tid == 0)
+        for (int e = 0; e < nE; ++e)
+          if (D.evKind[e] == 1) { const int i = D.evIdx[e]; ev[kEvRec * e + 6] = world_z(B.frame, P.M.mkx[i], P.M.mky[i], P.M.mkz[i]); }
+      __syncthreads();
+    }
+    device_step<Mixed, kTPB>(pa, lds, tid);
     ++done;
-    if (tid == 0) {
+    {
+      SSA_VIEWS;  // outside the branch: the addresses stay uniform
+      if (tid != 0) goto next;
+      SSL double* rdV = T.sig;
+      SSL double* rdSet = rdV + nT;
+      SSL double* ev = rdSet + nT;
       for (int j = 0; j < nT; ++j)
         if (D.rdStep[j] == done) { rdV[j] = (D.rdKind[j] == 0 ? B.scal[S_PITCH] : B.scal[S_ROLL]) * 180 / kPi; rdSet[j] = 1; }
       for (int e = 0; e < nE; ++e) {
@@ -237,120 +254,151 @@ __global__ __launch_bounds__(kTPB) __attribute__((amdgpu_waves_per_eu(FWD ? 4 : 
         r[7] = code;
       }
     }
+  next:
     __syncthreads();
   }
 
-  if (K == 1) save_state(P, B, ckpt + done * SS, tid, kTPB);  // the state after the last step
-  // 2. loss, signals, seeds
-  if (tid == 0) {
-    const double tEnd = B.scal[S_T], dt = P.M.dt;
-    const bool foundered = B.scal[S_FOUNDERED] != 0;
-    SSG double* sigOut = D.signals ? D.signals + static_cast<size_t>(inst) * nSig : nullptr;
-    gcd* obs = D.obs + static_cast<size_t>(inst) * nSig;
-    double L = 0;
-    int ns = 0;
-    auto seed = [&](long long step, int kind, int idx, double value) {
-      SSL double* sd = T.seeds + 4 * ns++;
-      sd[0] = static_cast<double>(step); sd[1] = kind; sd[2] = idx; sd[3] = value;
-    };
-    for (int j = 0; j < nT; ++j) {
-      const double v = rdSet[j] != 0 ? rdV[j] : D.rdAfter[j];
-      const double r = v - obs[j];
-      L += D.rdW[j] * (r * r);
-      if (sigOut) sigOut[j] = v;
-      if (rdSet[j] != 0) seed(D.rdStep[j], D.rdKind[j] == 0 ? 0 : 3, 0, 2 * D.rdW[j] * r * 180 / kPi);
-    }
-    // a founder margin's seed: hull top (s = hullTopBelow - top), pitch (s = |pitch| - pitchAbsAbove), buoyancy
-    auto margin_seed = [&](long long step, double code, double g) {
-      if (code == 0) seed(step, 4, 0, -g);
-      else if (code == 2) seed(step, 5, 0, -g);
-      else seed(step, 0, 0, g * code);
-    };
-    for (int e = 0; e < nE; ++e) {
-      const SSL double* r = ev + kEvRec * e;
-      const bool fired = r[0] != 0;
-      const long long k = static_cast<long long>(r[1]);
-      double ts;
-      if (fired && k > 1) ts = r[5] - dt * r[2] / (r[2] - r[3]);
-      else if (fired) ts = r[5];
-      else ts = foundered ? tEnd : D.evMissing[e];
-      const double res = ts - obs[nT + e];
-      L += D.evW[e] * (res * res);
-      if (sigOut) sigOut[nT + e] = ts;
-      if (fired && k > 1) {
-        const double Gt = 2 * D.evW[e] * res, sk = r[2], sp = r[3], del = sk - sp;
-        const double gs = Gt * dt * sp / (del * del), gsp = -Gt * dt * sk / (del * del);
-        const int kind = D.evKind[e], idx = D.evIdx[e];
-        if (kind == 0) { seed(k, 1, idx, gs); seed(k - 1, 1, idx, gsp); }
-        else if (kind == 1) {
-          const double sign = P.M.mkWhen[idx] == 0 ? -1.0 : 1.0;
-          seed(k, 2, idx, sign * gs); seed(k - 1, 2, idx, sign * gsp);
-        } else { margin_seed(k, r[8], gs); margin_seed(k - 1, r[9], gsp); }
+  {
+    SSA_VIEWS;
+    // the state after the last step: K = 1 stores every state; a portable32 gradient keeps the last one as the end
+    // of the last segment's check
+    if (K == 1) save_state(P, B, ckpt + done * SS, tid, kTPB);
+    else if (Mixed && !FWD && D.wantGrad) save_state(P, B, ckpt + ((done + K - 1) / K) * SS, tid, kTPB);
+    // 2. loss, signals, seeds
+    if (tid == 0) {
+      const SSL double* rdV = T.sig;
+      const SSL double* rdSet = rdV + nT;
+      const SSL double* ev = rdSet + nT;
+      const double tEnd = B.scal[S_T], dt = P.M.dt;
+      const bool foundered = B.scal[S_FOUNDERED] != 0;
+      SSG double* sigOut = D.signals ? D.signals + static_cast<size_t>(inst) * nSig : nullptr;
+      gcd* obs = D.obs + static_cast<size_t>(inst) * nSig;
+      double L = 0;
+      int ns = 0;
+      auto seed = [&](long long step, int kind, int idx, double value) {
+        SSL double* sd = T.seeds + 4 * ns++;
+        sd[0] = static_cast<double>(step); sd[1] = kind; sd[2] = idx; sd[3] = value;
+      };
+      for (int j = 0; j < nT; ++j) {
+        const double v = rdSet[j] != 0 ? rdV[j] : D.rdAfter[j];
+        const double r = v - obs[j];
+        L += D.rdW[j] * (r * r);
+        if (sigOut) sigOut[j] = v;
+        if (rdSet[j] != 0) seed(D.rdStep[j], D.rdKind[j] == 0 ? 0 : 3, 0, 2 * D.rdW[j] * r * 180 / kPi);
       }
+      // a founder margin's seed: hull top (s = hullTopBelow - top), pitch (s = |pitch| - pitchAbsAbove), buoyancy
+      auto margin_seed = [&](long long step, double code, double g) {
+        if (code == 0) seed(step, 4, 0, -g);
+        else if (code == 2) seed(step, 5, 0, -g);
+        else seed(step, 0, 0, g * code);
+      };
+      for (int e = 0; e < nE; ++e) {
+        const SSL double* r = ev + kEvRec * e;
+        const bool fired = r[0] != 0;
+        const long long k = static_cast<long long>(r[1]);
+        double ts;
+        if (fired && k > 1) ts = r[5] - dt * r[2] / (r[2] - r[3]);
+        else if (fired) ts = r[5];
+        else ts = foundered ? tEnd : D.evMissing[e];
+        const double res = ts - obs[nT + e];
+        L += D.evW[e] * (res * res);
+        if (sigOut) sigOut[nT + e] = ts;
+        if (fired && k > 1) {
+          const double Gt = 2 * D.evW[e] * res, sk = r[2], sp = r[3], del = sk - sp;
+          const double gs = Gt * dt * sp / (del * del), gsp = -Gt * dt * sk / (del * del);
+          const int kind = D.evKind[e], idx = D.evIdx[e];
+          if (kind == 0) { seed(k, 1, idx, gs); seed(k - 1, 1, idx, gsp); }
+          else if (kind == 1) {
+            const double sign = P.M.mkWhen[idx] == 0 ? -1.0 : 1.0;
+            seed(k, 2, idx, sign * gs); seed(k - 1, 2, idx, sign * gsp);
+          } else { margin_seed(k, r[8], gs); margin_seed(k - 1, r[9], gsp); }
+        }
+      }
+      // latest step first (insertion sort: a few dozen records), the order the reverse sweep consumes them in
+      for (int a = 1; a < ns; ++a)
+        for (int b = a; b > 0 && T.seeds[4 * (b - 1)] < T.seeds[4 * b]; --b)
+          for (int q = 0; q < 4; ++q) { const double t = T.seeds[4 * b + q]; T.seeds[4 * b + q] = T.seeds[4 * (b - 1) + q]; T.seeds[4 * (b - 1) + q] = t; }
+      T.nseeds[0] = ns;
+      D.loss[inst] = L;
+      D.steps[inst] = done;
     }
-    // latest step first (insertion sort: a few dozen records), the order the reverse sweep consumes them in
-    for (int a = 1; a < ns; ++a)
-      for (int b = a; b > 0 && T.seeds[4 * (b - 1)] < T.seeds[4 * b]; --b)
-        for (int q = 0; q < 4; ++q) { const double t = T.seeds[4 * b + q]; T.seeds[4 * b + q] = T.seeds[4 * (b - 1) + q]; T.seeds[4 * (b - 1) + q] = t; }
-    T.nseeds[0] = ns;
-    D.loss[inst] = L;
-    D.steps[inst] = done;
+    __syncthreads();
   }
-  __syncthreads();
   if (FWD || !D.wantGrad) return;
 
-  // 3. reverse
-  for (int i = tid; i < 6 + 5 * NN; i += kTPB) { T.lamA[i] = 0; T.lamB[i] = 0; }
-  for (int p = tid; p < AM.np; p += kTPB) T.G[p] = 0;
-  const size_t iC = static_cast<size_t>(Bt.connOff[inst]);
-  load_adjoint_tables<kTPB>(P, T, D.pa + iC, D.pc + iC, tid);
-  SSL double* lamCur = T.lamA;
-  SSL double* lamNext = T.lamB;
-  if (K == 1) {
-    // every state is stored: no recomputation
-    for (long long k = done; k >= 1; --k) {
-      load_tape_state(P, T, ckpt + (k - 1) * SS, tid, kTPB);
-      __syncthreads();
-      jac_step<kTPB>(P, AM, B, T, tid);
-      verify_step(P, T, ckpt + k * SS, tid, kTPB, (SSL int32_t*)&bad);
-      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext);
-      SSL double* sw = lamCur;
-      lamCur = lamNext;
-      lamNext = sw;
-    }
+  // 3. reverse, from the last step to the first. K = 1: every start state is stored. K > 1: when the sweep enters a
+  // segment of K steps, the segment is recomputed from its checkpoint into the ring of K + 1 states.
+  {
+    SSA_VIEWS;
+    for (int i = tid; i < 6 + 5 * NN; i += kTPB) { T.lamA[i] = 0; T.lamB[i] = 0; }
+    for (int p = tid; p < AM.np; p += kTPB) T.G[p] = 0;
+    const size_t iC = static_cast<size_t>(Bt.connOff[inst]);
+    load_adjoint_tables<kTPB>(P, T, D.pa + iC, D.pc + iC, tid);
   }
-  const long long nSeg = K == 1 ? 0 : (done + K - 1) / K;
-  for (long long seg = nSeg - 1; seg >= 0; --seg) {
-    restore_state(P, B, ckpt + seg * SS, tid, kTPB);
-    if (tid == 0) *B.nextGroup = 0;
-    __syncthreads();
-    if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
-    __syncthreads();
-    const long long len = std::min(K, done - seg * K);
-    for (long long i = 0; i < len; ++i) {
-      save_state(P, B, ring + i * SS, tid, kTPB);
-      __syncthreads();
-      device_step<false, kTPB>(pa, lds, tid);
+  long long segLo = done;  // the first step of the segment in the ring (none yet)
+  for (long long k = done; k >= 1; --k) {
+    const SSG double* src = ckpt + (k - 1) * SS;  // the step's start state
+    const SSG double* next = ckpt + k * SS;       // and the state after it
+    if (K > 1) {
+      if (k - 1 < segLo) {
+        const long long seg = (k - 1) / K;
+        segLo = seg * K;
+        const long long len = std::min(K, done - segLo);
+        {
+          SSA_VIEWS;
+          restore_state(P, B, ckpt + seg * SS, tid, kTPB);
+          if (tid == 0) *B.nextGroup = 0;
+          __syncthreads();
+          if (tid < 64) pose_wave0(P, B, tid, B.scal[S_ZO], B.scal[S_PITCH], B.scal[S_ROLL]);
+          __syncthreads();
+        }
+        for (long long i = 0; i < len; ++i) {
+          {
+            SSA_VIEWS;
+            save_state(P, B, ring + i * SS, tid, kTPB);
+          }
+          __syncthreads();
+          device_step<Mixed, kTPB>(pa, lds, tid);
+        }
+        {
+          SSA_VIEWS;
+          save_state(P, B, ring + len * SS, tid, kTPB);
+        }
+        __threadfence_block();
+        __syncthreads();
+        if (Mixed) {
+          // the recomputed segment ends where the forward's next checkpoint (or its last state) begins
+          const SSG double* want = ckpt + (seg + 1) * SS;
+          const SSG double* got = ring + len * SS;
+          for (int j = tid; j < SS; j += kTPB)
+            if (__builtin_bit_cast(uint64_t, got[j]) != __builtin_bit_cast(uint64_t, want[j])) atomicAdd((int32_t*)&bad, 1);
+        }
+      }
+      src = ring + (k - 1 - segLo) * SS;
+      next = src + SS;
     }
-    save_state(P, B, ring + len * SS, tid, kTPB);
-    __threadfence_block();
-    __syncthreads();
-    for (long long i = len - 1; i >= 0; --i) {
-      const long long k = seg * K + i + 1;
-      load_tape_state(P, T, ring + i * SS, tid, kTPB);
-      __syncthreads();
-      jac_step<kTPB>(P, AM, B, T, tid);
-      verify_step(P, T, ring + (i + 1) * SS, tid, kTPB, (SSL int32_t*)&bad);
-      reverse_step<kTPB>(P, AM, B, T, tid, k, lamCur, lamNext);
-      SSL double* sw = lamCur;
-      lamCur = lamNext;
-      lamNext = sw;
+    // the adjoint of the state after step k is in lamA when done - k is even
+    const bool inA = ((done - k) & 1) == 0;
+    const ViewAddrs va{pa, ta, reinterpret_cast<uint64_t>(gslice)};
+    {
+      SSA_VIEWS;
+      load_tape_state(P, T, src, tid, kTPB);
     }
+    __syncthreads();
+    jac_step<kTPB, GT>(va, AM, tid);
+    if (!Mixed) {
+      SSA_VIEWS;
+      verify_step(P, T, next, tid, kTPB, (SSL int32_t*)&bad);
+    }
+    reverse_step<kTPB, GT>(va, AM, tid, k, inA);
   }
   __syncthreads();
-  if (tid < AM.np) {
-    const double th = D.theta[static_cast<size_t>(inst) * AM.np + tid];
-    D.grad[static_cast<size_t>(inst) * AM.np + tid] = T.G[tid] / th;
+  {
+    SSA_VIEWS;
+    if (tid < AM.np) {
+      const double th = D.theta[static_cast<size_t>(inst) * AM.np + tid];
+      D.grad[static_cast<size_t>(inst) * AM.np + tid] = T.G[tid] / th;
+    }
   }
   if (tid == 0) D.mismatch[inst] = bad;
 }
@@ -370,8 +418,7 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
   const int mi = Bt.instModel[inst];
   PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(mi) * sizeof(ModelParams));
   AMC& AMs = *(AMC*)((const SSC char*)amodels + static_cast<size_t>(mi) * sizeof(AdjModel));
-  AdjModel AM{};
-  AM.maxGroup = AMs.maxGroup; AM.np = AMs.np; AM.mode = AMs.mode;
+  const AdjScalars AM{AMs.maxGroup, AMs.np, AMs.mode};
   const Blk B = carve(lds, P);
   const TapeT<GT> T = carve_tape<GT>(lds + P.L.total, D.gtape, AMs.A);
   const int NN = P.S.nn;
@@ -400,8 +447,8 @@ __global__ __launch_bounds__(kTPB) void adjoint_one_step(const ModelParams* __re
   }
   __syncthreads();
   load_adjoint_tables<kTPB>(P, T, D.pa, D.pc, tid);
-  jac_step<kTPB>(P, AM, B, T, tid);
-  reverse_step<kTPB>(P, AM, B, T, tid, 1, T.lamA, T.lamB);
+  jac_step_views<kTPB>(P, AM, B, T, tid);
+  reverse_step_views<kTPB>(P, AM, B, T, tid, 1, T.lamA, T.lamB);
   for (int i = tid; i < 10 + 5 * NN; i += kTPB) {
     double lam = 0, rec = 0;
     if (i >= 1 && i <= 6) lam = T.lamB[i - 1];
@@ -495,7 +542,7 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
   need(s.np >= 1 && s.np <= kMaxParams, "ss_batch_gradient: the parameter count must be 1..64");
   need(s.pa != nullptr && s.pc != nullptr && s.theta != nullptr, "ss_batch_gradient: null parameter table");
 
-  need(!b->mixed, "ss_batch_gradient: the gradient is FP64 (portable numerics)");
+  need(!(oneStep && b->mixed), "ss_batch_adjoint_step: the one-step form is FP64 (portable numerics)");
   const int32_t nT = loss ? loss->nRead : 0, nE = loss ? loss->nEvent : 0;
   need(nT >= 0 && nE >= 0 && nT + nE <= 256, "ss_batch_gradient: at most 256 loss terms");
   const size_t nconn = b->hArea.size();
@@ -625,7 +672,8 @@ void prepare(ss_batch* b, AdjRun& R, const Setup& s, const ss_loss* loss, const 
     ckOff[i] = ck;
     rgOff[i] = rg;
     if (!oneStep) {
-      ck += (caps[i] / K + 1) * sizes[i];
+      // a portable32 gradient also keeps the last state (the last segment's check)
+      ck += (caps[i] / K + (b->mixed && wantGrad && K > 1 ? 2 : 1)) * sizes[i];
       if (K > 1 && wantGrad) rg += (static_cast<long long>(K) + 1) * sizes[i];
     }
   }
@@ -672,12 +720,16 @@ using AdjKernel = void (*)(const ModelParams*, const AdjModel*, const DevBatch*,
 
 void launch(bool oneStep, ss_batch* b, AdjRun& R, int grid, double* ms) {
   const bool wide = R.tpb == kMaxTPB;
+  auto run = [&](auto mixed) -> AdjKernel {
+    constexpr bool M = decltype(mixed)::value;
+    return R.forwardOnly ? (wide ? adjoint_run<kMaxTPB, false, true, M> : adjoint_run<kThreads, false, true, M>)
+           : (R.global ? (wide ? adjoint_run<kMaxTPB, true, false, M> : adjoint_run<kThreads, true, false, M>)
+                       : (wide ? adjoint_run<kMaxTPB, false, false, M> : adjoint_run<kThreads, false, false, M>));
+  };
   const AdjKernel kernel =
       oneStep ? (R.global ? (wide ? adjoint_one_step<kMaxTPB, true> : adjoint_one_step<kThreads, true>)
                           : (wide ? adjoint_one_step<kMaxTPB, false> : adjoint_one_step<kThreads, false>))
-              : R.forwardOnly ? (wide ? adjoint_run<kMaxTPB, false, true> : adjoint_run<kThreads, false, true>)
-              : (R.global ? (wide ? adjoint_run<kMaxTPB, true, false> : adjoint_run<kThreads, true, false>)
-                          : (wide ? adjoint_run<kMaxTPB, false, false> : adjoint_run<kThreads, false, false>));
+              : b->mixed ? run(std::true_type{}) : run(std::false_type{});
   hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(R.lds)),
             "hipFuncSetAttribute");
   hipEvent_t e0 = nullptr, e1 = nullptr;

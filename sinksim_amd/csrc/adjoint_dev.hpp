@@ -53,6 +53,11 @@ struct AdjModel {
   int32_t np;
   int32_t mode;  // 0 exact, 1 relaxed
 };
+// The three values of AdjModel the step's functions read, passed by value (a local AdjModel copy passed by
+// reference lived in scratch memory)
+struct AdjScalars {
+  int32_t maxGroup, np, mode;
+};
 
 // ------------------------------------------------------------------------------------------- tangent values
 
@@ -301,7 +306,7 @@ __device__ __forceinline__ void put_out(const TapeT<GT>& T, int n, int lane, TV 
 
 // levels.hpp:13-49 (full = false) for space n, with tangents along (V, h0, pose) = (vol[n], level[n], pose).
 template <bool GT>
-__device__ inline void solve_single_tv(PC& P, const Blk& B, const TapeT<GT>& T, int lane, int n, bool relaxed) {
+__device__ __forceinline__ void solve_single_tv(PC& P, const Blk& B, const TapeT<GT>& T, int lane, int n, bool relaxed) {
   const SSL double* fr = T.fr;
   const FrameTan f = frame_tan(T.dfp, lane);
   const double mu = B.nMu[n], iR = fr[F_IR];
@@ -356,7 +361,7 @@ __device__ inline void solve_single_tv(PC& P, const Blk& B, const TapeT<GT>& T, 
 // volume and h0 its starting level (the members' mean level when all are wet, else one member's level). The
 // members' pass records are kept in the unit's scratch (mp); every member's pass of a round is at the same level.
 template <bool GT>
-__device__ inline void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mp, int lane, int g, int first, int N, bool relaxed) {
+__device__ __forceinline__ void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mp, int lane, int g, int first, int N, bool relaxed) {
   const SSL double* fr = T.fr;
   const FrameTan f = frame_tan(T.dfp, lane);
   const double iR = fr[F_IR];
@@ -397,15 +402,14 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, S
     if (q.zmin.v < lo.v) lo = q.zmin;
     if (q.zmax.v > hi.v) hi = q.zmax;
   }
-  // every member's extent top (zmax) tangent, which does not depend on the level
-  TV zmx[8];
-  for (int k = 0; k < N && k < 8; ++k) zmx[k] = Pk(k).zmax;
+  // a member's extent top (zmax) and its tangent do not depend on the level: every round's record holds them, so
+  // the outputs read them from the last round's records (no per-member array, which would live in scratch memory)
 
   if (V.v <= kEmptyVolume) {
     for (int k = 0; k < N; ++k) {
       const int n = members[k];
       const PassTV q = Pk(k);
-      put_out(T, n, lane, q.zmin, tc(0.0), tc(0.0), tc(T.iCx[n]), tc(T.iCy[n]), tc(T.iCz[n]), true, zmx[k]);
+      put_out(T, n, lane, q.zmin, tc(0.0), tc(0.0), tc(T.iCx[n]), tc(T.iCy[n]), tc(T.iCz[n]), true, q.zmax);
     }
     return;
   }
@@ -414,22 +418,18 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, S
     double aSum = 0;
     for (int k = 0; k < N; ++k) aSum += B.nAov[members[k]];
     const TV dh = over / aSum;
+    // member k < N - 1 takes its capacity plus its pressed-up share (recomputed at the output), the last the rest
     TV assigned = tc(0.0);
-    TV vk[8];
-    for (int k = 0; k < N - 1; ++k) {
-      const TV v = B.nVmax[members[k]] + dh * B.nAov[members[k]];
-      vk[k] = v;
-      assigned = assigned + v;
-    }
-    vk[N - 1] = V - assigned;
+    for (int k = 0; k < N - 1; ++k) assigned = assigned + (B.nVmax[members[k]] + dh * B.nAov[members[k]]);
     hP = hi;
     passes(hi.v);
     for (int k = 0; k < N; ++k) {
       const int n = members[k];
       const PassTV q = Pk(k);
       const bool c = !(q.vol.v > kCentroidMinVolume);
-      put_out(T, n, lane, hi + dh, vk[k], tc(B.nAov[n]), c ? tc(T.iCx[n]) : q.mx / q.vol, c ? tc(T.iCy[n]) : q.my / q.vol,
-              c ? tc(T.iCz[n]) : q.mz / q.vol, c, zmx[k]);
+      const TV vk = k < N - 1 ? B.nVmax[n] + dh * B.nAov[n] : V - assigned;
+      put_out(T, n, lane, hi + dh, vk, tc(B.nAov[n]), c ? tc(T.iCx[n]) : q.mx / q.vol, c ? tc(T.iCy[n]) : q.my / q.vol,
+              c ? tc(T.iCz[n]) : q.mz / q.vol, c, q.zmax);
     }
     return;
   }
@@ -466,49 +466,78 @@ __device__ inline void solve_group_tv(PC& P, const Blk& B, const TapeT<GT>& T, S
   }
   TV tot = tc(0.0);
   for (int k = 0; k < N; ++k) tot = tot + Pk(k).vol;
-  TV vk[8];
-  TV assigned = tc(0.0);
-  for (int k = 0; k < N - 1; ++k) {
+  // member k's proportional share, capped at its capacity; recomputed (the same operations on the same records)
+  // wherever it is needed rather than kept in a per-member array
+  auto share = [&](int k) -> TV {
     TV v = tot.v > 0 ? V * Pk(k).vol / tot : V / static_cast<double>(N);
     if (v.v > B.nVmax[members[k]]) v = tc(B.nVmax[members[k]]);
-    vk[k] = v;
-    assigned = assigned + v;
-  }
+    return v;
+  };
+  TV assigned = tc(0.0);
+  for (int k = 0; k < N - 1; ++k) assigned = assigned + share(k);
+  // the cascade: from member N - 2 down, members take their capacity until member kc takes what is left; the
+  // members below kc keep their shares (kc = N - 1: no cascade)
+  int kc = N - 1;
+  TV vc = tc(0.0);
   const double vmaxLast = B.nVmax[members[N - 1]];
   if ((V - assigned).v > vmaxLast) {
     const TV need = V - vmaxLast;
     TV capped = tc(0.0);
     for (int k = N - 2; k >= 0; --k) {
       TV prior = tc(0.0);
-      for (int j = 0; j < k; ++j) prior = prior + vk[j];
+      for (int j = 0; j < k; ++j) prior = prior + share(j);
       const TV v = (need - capped) - prior;
       if (v.v > B.nVmax[members[k]] && k > 0) {
-        vk[k] = tc(B.nVmax[members[k]]);
         capped = capped + B.nVmax[members[k]];
         continue;
       }
-      vk[k] = v;
+      kc = k;
+      vc = v;
       break;
     }
     assigned = tc(0.0);
-    for (int k = 0; k < N - 1; ++k) assigned = assigned + vk[k];
+    for (int k = 0; k < N - 1; ++k) assigned = assigned + (k < kc ? share(k) : k == kc ? vc : tc(B.nVmax[members[k]]));
   }
-  vk[N - 1] = V - assigned;
   for (int k = 0; k < N; ++k) {
     const int n = members[k];
     const PassTV q = Pk(k);
     const bool c = !(q.vol.v > kCentroidMinVolume);
-    put_out(T, n, lane, h, vk[k], q.dv, c ? tc(T.iCx[n]) : q.mx / q.vol, c ? tc(T.iCy[n]) : q.my / q.vol, c ? tc(T.iCz[n]) : q.mz / q.vol,
-            c, zmx[k]);
+    const TV vk = k == N - 1 ? V - assigned : k < kc ? share(k) : k == kc ? vc : tc(B.nVmax[n]);
+    put_out(T, n, lane, h, vk, q.dv, c ? tc(T.iCx[n]) : q.mx / q.vol, c ? tc(T.iCy[n]) : q.my / q.vol, c ? tc(T.iCz[n]) : q.mz / q.vol,
+            c, q.zmax);
   }
 }
+
+// ------------------------------------------------------------------------------------------- views and calls
+
+__device__ __forceinline__ const SSC AdjModel& adj_model_at(uint64_t ta) {
+  asm volatile("" : "+s"(ta));
+  return *(const SSC AdjModel*)ta;
+}
+
+// The addresses the views of an instance are carved from: its model, its tape layout (AdjModel) and its slice of the
+// global tape. Uniform.
+struct ViewAddrs {
+  uint64_t pa, ta, gs;
+};
+
+// The instance's views (the model, the forward layout and the tape: about a hundred LDS addresses), carved from the
+// model and tape-layout addresses behind a scalar barrier, so that they are derived where a phase needs them rather
+// than held in registers across the kernel or passed through scratch memory to a function kept out of line.
+template <bool GT>
+struct StepViews {
+  PC& P;
+  const Blk B;
+  const TapeT<GT> T;
+  __device__ __forceinline__ StepViews(uint64_t pa, uint64_t ta, SSL double* lds, SSG double* gslice)
+      : P(model_at(pa)), B(carve(lds, P)), T(carve_tape<GT>(lds + P.L.total, gslice, adj_model_at(ta).A)) {}
+};
 
 // levels.hpp:166-194: unit u of the TPB / 32 units takes groups u, u + TPB / 32, ... (the merge decision is the
 // forward's: every group's outputs are the same whichever unit computes them)
 template <int TPB, bool GT>
-__device__ inline void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, SSL double* mpBase, int maxGroup, int tid, bool relaxed) {
+__device__ __forceinline__ void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, int maxGroup, int tid, bool relaxed) {
   const int lane = tid & (kLanes - 1), unit = tid / kLanes;
-  SSL double* mp = mpBase + unit * maxGroup * kPassRec;
   // each unit takes the next group from the counter (its lane 0's LDS atomic, broadcast to the unit), so units that
   // start late (after the buoyancy partials) or draw cheap groups take more
   for (;;) {
@@ -533,7 +562,7 @@ __device__ inline void solve_levels_tv(PC& P, const Blk& B, const TapeT<GT>& T, 
     }
     if (lane == 0) T.gMer[g] = merged ? 1.0 : 0.0;
     if (merged) {
-      solve_group_tv(P, B, T, mp, lane, g, first, N, relaxed);
+      solve_group_tv(P, B, T, T.mp + unit * maxGroup * kPassRec, lane, g, first, N, relaxed);
     } else {
       if (lane == 0) T.gH0[g] = -2.0;  // members solved singly
       for (int k = 0; k < N; ++k) solve_single_tv(P, B, T, lane, B.gNodes[first + k], relaxed);
@@ -597,7 +626,7 @@ __device__ inline void buoyancy_rec(PC& P, const TapeT<GT>& T, int tid) {
 // The step's tape from the start state in the tape's input slots (sv, iVol..iCz). Ends with a barrier. The
 // values written (oLvl.., vol1, the integrated pose and velocities in tot) equal the forward step's.
 template <int TPB, bool GT>
-__device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT>& T, int tid) {
+__device__ __forceinline__ void jac_step_views(PC& P, const AdjScalars AM, const Blk& B, const TapeT<GT>& T, int tid) {
   const int NN = P.S.nn, NC = P.M.nc;
   const bool relaxed = AM.mode == 1;
   // pose and its derivatives (frame.hpp:9-21)
@@ -617,7 +646,7 @@ __device__ void jac_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT
   // (a) buoyancy with its moment sums (the canonical 256 partials: the first 256 threads), (b) the level solve with
   // tangents (every unit; with 512 threads the second 256 start it while the first compute the partials)
   if (TPB == kThreads || tid < kThreads) buoyancy_rec(P, T, tid);
-  solve_levels_tv<TPB>(P, B, T, T.mp, AM.maxGroup, tid, relaxed);
+  solve_levels_tv<TPB>(P, B, T, AM.maxGroup, tid, relaxed);
   __syncthreads();
   // (c) heads, effective areas (and which of aov / amin / area gave it) and limiter degrees (flows.hpp:15-35)
   for (int n = tid; n < NN; n += TPB) {
@@ -883,8 +912,8 @@ __device__ __forceinline__ void add_world_z(FrameGrad& G, double gz, double x, d
 // wph, vol, level, cx, cy, cz) and the loss's seeds at step k, the adjoint of the state before it in lamNext, and
 // the parameters' log-derivatives added to G.
 template <int TPB, bool GT>
-__device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const TapeT<GT>& T, int tid, long long k, const SSL double* lamCur,
-                             SSL double* lamNext) {
+__device__ __forceinline__ void reverse_step_views(PC& P, const AdjScalars AM, const Blk& B, const TapeT<GT>& T, int tid, long long k,
+                                                   const SSL double* lamCur, SSL double* lamNext) {
   const int NN = P.S.nn, NC = P.M.nc;
   const SSL double* fr = T.fr;
   const SSL double* o = T.tot;
@@ -1139,6 +1168,29 @@ __device__ void reverse_step(PC& P, const AdjModel& AM, const Blk& B, const Tape
     }
   }
   __syncthreads();
+}
+
+// The Jacobian and reverse steps as calls that take scalars (the addresses the views are carved from, the adjoint's
+// settings) and carve the views themselves: the kernel that runs them also runs the forward step, and the three
+// inlined together exceed the register file; views passed by reference to a function kept out of line would go
+// through scratch memory at every call.
+template <int TPB, bool GT>
+__device__ __attribute__((noinline)) void jac_step(ViewAddrs va, AdjScalars AM, int tid) {
+  extern __shared__ double lds_dyn[];
+  const StepViews<GT> v(uniform64(va.pa), uniform64(va.ta), (SSL double*)lds_dyn, (SSG double*)uniform64(va.gs));
+  const AdjScalars A{__builtin_amdgcn_readfirstlane(AM.maxGroup), __builtin_amdgcn_readfirstlane(AM.np),
+                     __builtin_amdgcn_readfirstlane(AM.mode)};
+  jac_step_views<TPB>(v.P, A, v.B, v.T, tid);
+}
+// inA: the adjoint of the state after the step is in lamA (the state before it goes to lamB), else the other way
+template <int TPB, bool GT>
+__device__ __attribute__((noinline)) void reverse_step(ViewAddrs va, AdjScalars AM, int tid, long long k, bool inA) {
+  extern __shared__ double lds_dyn[];
+  const StepViews<GT> v(uniform64(va.pa), uniform64(va.ta), (SSL double*)lds_dyn, (SSG double*)uniform64(va.gs));
+  const AdjScalars A{__builtin_amdgcn_readfirstlane(AM.maxGroup), __builtin_amdgcn_readfirstlane(AM.np),
+                     __builtin_amdgcn_readfirstlane(AM.mode)};
+  const bool a = __builtin_amdgcn_readfirstlane(inA ? 1 : 0) != 0;
+  reverse_step_views<TPB>(v.P, A, v.B, v.T, tid, k, a ? v.T.lamA : v.T.lamB, a ? v.T.lamB : v.T.lamA);
 }
 
 }  // namespace ssa

@@ -167,3 +167,52 @@ def test_relaxed_derivatives_keep_values_and_descend(titanic):
     d = rx["dloss"][0]
     assert np.sign(d) == np.sign(fd) and 0.5 < d / fd < 2.0, (d, fd)
     assert abs(ex["dloss"][0]) > 1e6 * abs(fd)
+
+
+def test_plain_loss_run_equals_the_dual_builds_value(titanic):
+    """The oracle's plain loss run (any numerics; the portable32 gradient's checker) in portable numerics gives the
+    dual build's loss and signals bit for bit: the same events, interpolations and terms, with area and coefficient
+    factors applied as the gradient applies them. In portable32 numerics it differs (FP32 partials), by little."""
+    import pyoracle
+    from sinksim_amd.calib import kind_params, titanic_loss
+    ship, sim = titanic
+    pm = kind_params(sim)
+    spec = titanic_loss(sim)
+    en, area, tOn = _tables(sim)
+    theta = np.exp(np.random.default_rng(3).uniform(-0.3, 0.3, pm.count))
+    model = _model(ship, sim)
+    d = pyoracle.dual_run(model, en, area, tOn, pm.pa, pm.pc, theta, seed=np.zeros((1, pm.count)), max_steps=12000, spec=spec)
+    r = pyoracle.loss_run(model, "portable", en, area, tOn, pm.pa, pm.pc, theta, spec, max_steps=12000)
+    assert r["steps"] == d["steps"] == 12000
+    assert r["loss"] == d["loss"] and np.array_equal(r["signals"], d["signals"])
+    assert np.isfinite(r["signals"]).all() and (r["signals"][:3] != 0).all()  # the three trim readouts were taken
+    r32 = pyoracle.loss_run(model, "portable32", en, area, tOn, pm.pa, pm.pc, theta, spec, max_steps=12000)
+    assert r32["loss"] != r["loss"] and abs(r32["loss"] / r["loss"] - 1) < 1e-4
+
+
+@pytest.mark.parametrize("amin,slope", [(4.0, 1000.0), (2500.0, 2500.0)])
+def test_relaxed_level_tangent_is_one_over_the_surface_area(amin, slope):
+    """Closed form of the relaxed level derivative on the box-barge rig (tests/test_kernel.cpp:163-227 of the
+    reference: node 0 holds 3,000 m^3 at level -1 m over a 1,000 m^2 free surface, at even keel, no connections). A
+    unit of volume added to node 0 raises its level by 1 / 1,000 m, or by 1 / amin where the space's minimum effective
+    area amin exceeds the free surface (the floor); the volume solve at the stored level converges without a Newton
+    step, so the discrete program's (exact) level does not move at all."""
+    import pyoracle
+    from sinksim_amd.synth import box_barge
+    ship, sim = box_barge()
+    ship.amin[0] = amin
+    sim.vol0[0] = 50 * 20 * 3
+    sim.level0[0] = -1.0
+    model = pyoracle.Model(ship, sim)
+    o = pyoracle.OracleSim(ship, sim, "portable", model)
+    rec = o.record()
+    nn = ship.nodes
+    drec = np.zeros_like(rec)
+    drec[10] = 1.0  # d vol[0]
+    empty = np.zeros(0)
+    for relaxed, want in ((True, 1.0 / slope), (False, 0.0)):
+        r2, dr2, _, _ = pyoracle.dual_step(model, np.zeros(0, np.uint8), empty, empty, np.zeros(0, np.int32), np.zeros(0, np.int32),
+                                           np.ones(1), np.zeros(1), rec, drec, relaxed=relaxed)
+        assert r2[10 + nn] == -1.0 and r2[10] == 3000.0  # the level and the volume themselves
+        assert dr2[10] == 1.0                             # the volume carries its tangent
+        assert dr2[10 + nn] == pytest.approx(want, rel=1e-12, abs=0.0), (relaxed, dr2[10 + nn])

@@ -4,6 +4,9 @@ handed out by a work counter and refilled as simulations end.
 - Every instance of a fleet of seeded synthetic hulls (1,000-4,000 columns, 8-48 spaces) and titanic64 equals the
   oracle's CPU run of that instance, in both numerics, with the grid forced down to a few workgroups so that each
   workgroup runs many instances in turn (the refill path), and at the default grid.
+- A fleet that holds the generator's largest hulls (16,000 columns, 128 spaces, 600 connections) beside smaller ones
+  equals the oracle instance for instance over 1,200 s, in both numerics (one of those hulls is pinned against the
+  reference's own CPU engine in tests/test_synthetic_reference.py).
 - A fleet of one model equals the plain batch of that model.
 """
 import zlib
@@ -63,6 +66,21 @@ def test_fleet_matches_oracle_with_refill(numerics, tpb):
     models = _models()
     inst_model = np.array([4, 0, 1, 2, 3, 0, 2, 1, 3, 4, 0, 2], np.int32)
     _check_fleet(models, inst_model, numerics, t_max=1200.0, every=30.0, workgroups=3, tpb=tpb)
+
+
+# the generator's largest hulls (the calibration hull is seed 5 at these sizes) and two drawn sizes
+LARGEST = [dict(seed=5, columns=16000, spaces=128, connections=600), dict(seed=31, columns=16000, spaces=128, connections=600),
+           dict(seed=47, columns=16000, spaces=128, connections=600), dict(seed=8), dict(seed=19)]
+
+
+@pytest.mark.parametrize("numerics", ["portable", "portable32"])
+def test_fleet_of_the_largest_hulls_matches_oracle(numerics):
+    from sinksim_amd.synth import synthetic_hull
+    models = [synthetic_hull(**kw) for kw in LARGEST]
+    assert max(m[0].columns for m in models) >= 16000 and max(m[0].nodes for m in models) == 128
+    assert max(m[1].connections for m in models) == 600
+    inst_model = np.array([0, 3, 1, 4, 2, 0, 1], np.int32)
+    _check_fleet(models, inst_model, numerics, t_max=1200.0, every=30.0, workgroups=3)
 
 
 def test_fleet_matches_oracle_full_grid():

@@ -54,3 +54,22 @@ def test_oracle_reproduces_reference_on_synthetic_hulls(tmp_path, seed, numerics
     events, foundered, founderT = S.events()
     assert [(t, sim.event_id(k, i)) for t, i, k in events] == [(t, i) for t, i, _ in ref.events]
     assert foundered == ref.foundered and founderT == ref.founderT
+
+
+@pytest.mark.skipif(not os.path.exists(REF_RUN), reason="reference CPU build absent (make -C oracle ref)")
+def test_oracle_reproduces_reference_on_a_largest_hull(tmp_path):
+    """A hull at the generator's largest sizes (16,000 columns, 128 spaces, 600 connections; seed 31, one of the GPU
+    fleet test's), 1,200 s in portable numerics: every dense and sampled record of the reference's trace."""
+    sp, mp = _files(tmp_path, 31, columns=16000, spaces=128, connections=600)
+    out = os.path.join(tmp_path, "ref.trace.json")
+    r = subprocess.run([REF_RUN, sp, mp, "--numerics", "portable", "--tmax", "1200", "--trace", out, "--dense", "100",
+                        "--sample-every", "30"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    ref = load_trace(out)
+    ship, sim = load_ship(sp), load_sim(mp)
+    assert ship.columns >= 16000 and ship.nodes == 128 and sim.connections == 600
+    S = pyoracle.OracleSim(ship, sim, "portable")
+    dense, samples, steps = S.record_trace(1200.0, 100, 30.0)
+    assert steps == ref.steps
+    assert np.array_equal(dense, ref.dense)
+    assert np.array_equal(samples, ref.samples)

@@ -35,7 +35,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two o
 FLOPS_PER_STEP = 2.34e5
 # HBM bytes per launch of run_batch<false, false, 256> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
 # plain copy kernel: profiles/run_batch_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 235.1e6
+TRAFFIC_BYTES_PER_LAUNCH = 41.8e6
 TRAFFIC_SOURCE = "profiles/run_batch_fp64_titanic64_batch1024.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
@@ -154,7 +154,7 @@ def bench_uneven(args):
     opts = BatchOptions(count=count, mixed_precision=mixed, traced_instances=0, curve_every=30.0, t_max=t_max)
     # the launch shape, chosen once by the autotuner on the largest hulls (untimed)
     b = Batch.fleet(models, inst_model, opts)
-    tune = b.autotune(sample=min(count, 1024), trial_steps=300)
+    tune = b.autotune(sample=min(count, 2048), trial_steps=300)
     b.close()
     print(f"autotune: {tune}", file=sys.stderr, flush=True)
 
@@ -191,10 +191,14 @@ def bench_uneven(args):
                                f"spaces, {int(nc.min())}-{int(nc.max())} connections), one instance each, {args.numerics} "
                                f"numerics, to founder or {t_max / 3600:g} h, curves every 30 s, refilled workgroups",
                    "instances": count, "numerics": args.numerics, "sim_steps_per_bench_step": int(tot_steps // args.steps),
-                   "threads_per_sim": tune["threads_per_sim"], "autotune_rates": [round(tune["rate_256"]), round(tune["rate_512"])],
+                   "threads_per_sim": tune["threads_per_sim"], "workgroups": tune["workgroups"] or "one per instance or all resident",
+                   "autotune_trials": [[t["threads_per_sim"], t["workgroups"], round(t["rate"])] for t in tune["trials"]],
                    "device": _device()},
+        # traffic: FETCH_SIZE + WRITE_SIZE of the timed launch on this workload at 4,096 hulls (FETCH_SIZE unscaled:
+        # profiles/run_batch_fp64_uneven4096.md); a constant here, the run itself is not counted
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 5), "traffic": None,
+                     "frac": round(achieved / 8000.0, 5), "traffic": 52.7e12 if (count == 4096 and not mixed) else None,
+                     "traffic_source": "profiles/run_batch_fp64_uneven4096.md" if (count == 4096 and not mixed) else None,
                      "kernel": "run_batch<%s, true>" % ("true" if mixed else "false"), "kernel_ms": round(tot_kms / max(tot_launch, 1), 3)},
     }
     print(json.dumps(out), flush=True)

@@ -180,16 +180,23 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
  * SS_LOGIC after run. */
 ss_status ss_batch_set_stop_when_stable(ss_batch* b, int32_t on);
 
-/* No counterpart in the reference: what ss_batch_autotune measured and chose. rate[k]: simulation steps per second
- * of the trial at 256 (k = 0) and 512 (k = 1) threads per simulation (0 if that shape does not fit). */
+/* No counterpart in the reference: what ss_batch_autotune measured and chose. threadsPerSim and workgroups: the shape
+ * set on the batch (workgroups 0: one per instance, or as many as are resident at once, the default). rate[k]: the
+ * best simulation steps per second measured at 256 (k = 0) and 512 (k = 1) threads per simulation (0 if that shape
+ * does not fit). trial*: every shape measured, in order (at most 8). */
 typedef struct ss_tune_result {
   int32_t threadsPerSim, workgroups;
   double rate[2];
+  int32_t trials;
+  int32_t trialThreads[8], trialWorkgroups[8];
+  double trialRate[8];
 } ss_tune_result;
 
 /* No counterpart in the reference: the autotuner. Runs the first `sample` instances of the work order (the largest
- * models) for trialSteps steps at each launch shape on a trial copy (this batch's staged data is not touched),
- * and sets the fastest shape on the batch. SS_LOGIC after run. */
+ * models) for trialSteps steps at each launch shape on a trial copy (this batch's staged data is not touched), and
+ * sets the fastest shape on the batch. The shapes: 256 and 512 threads per simulation, each with the default grid and,
+ * where the sample holds more instances than are resident at once (the refilling kernel), with three quarters and
+ * half of the resident workgroups. SS_LOGIC after run. */
 ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the

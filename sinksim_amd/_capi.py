@@ -60,7 +60,8 @@ class SsInstanceResult(C.Structure):
 
 
 class SsTuneResult(C.Structure):
-    _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 2)]
+    _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 2), ("trials", C.c_int32),
+                ("trialThreads", C.c_int32 * 8), ("trialWorkgroups", C.c_int32 * 8), ("trialRate", C.c_double * 8)]
 
 
 class SsEvent(C.Structure):

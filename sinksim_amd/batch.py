@@ -171,7 +171,9 @@ class Batch:
         trial copy) and sets the fastest; returns the choice and the measured rates (steps/s)."""
         r = A.SsTuneResult()
         A.check(self._lib.ss_batch_autotune(self._h, int(sample), int(trial_steps), C.byref(r)))
-        return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "rate_256": r.rate[0], "rate_512": r.rate[1]}
+        trials = [{"threads_per_sim": r.trialThreads[k], "workgroups": r.trialWorkgroups[k], "rate": r.trialRate[k]} for k in range(r.trials)]
+        return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "rate_256": r.rate[0], "rate_512": r.rate[1],
+                "trials": trials}
 
     # ------------------------------------------------------------------ run and results
 

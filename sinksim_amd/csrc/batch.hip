@@ -805,29 +805,53 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
     double best = -1;
     int32_t bestTpb = kThreads, bestBlocks = 0;
     ss_tune_result res{};
+    int cus = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
     for (int c = 0; c < 2; ++c) {
-      std::unique_ptr<ss_batch> t = make_batch(o, b->models, im);
-      for (int32_t k = 0; k < n; ++k) {  // copy the staged tables and states of the sampled instances
-        const size_t src = static_cast<size_t>(b->order[static_cast<size_t>(k)]), dst = static_cast<size_t>(k);
-        const Model& m = b->model(static_cast<int32_t>(src));
-        std::copy_n(b->hEn.begin() + b->connOff[src], m.NC, t->hEn.begin() + t->connOff[dst]);
-        std::copy_n(b->hArea.begin() + b->connOff[src], m.NC, t->hArea.begin() + t->connOff[dst]);
-        std::copy_n(b->hTOn.begin() + b->connOff[src], m.NC, t->hTOn.begin() + t->connOff[dst]);
-        std::copy_n(b->hSt.begin() + 7 * src, 7, t->hSt.begin() + 7 * dst);
-        for (auto [from, to] : {std::make_pair(&b->hVol, &t->hVol), std::make_pair(&b->hLevel, &t->hLevel), std::make_pair(&b->hCx, &t->hCx),
-                                std::make_pair(&b->hCy, &t->hCy), std::make_pair(&b->hCz, &t->hCz)})
-          std::copy_n(from->begin() + b->nodeOff[src], m.NN, to->begin() + t->nodeOff[dst]);
-      }
-      t->tpb = tpbs[c];
       size_t lds = 0;
-      for (const auto& m : t->models) lds = std::max(lds, m->lds_bytes(t->tpb));
+      for (const auto& m : b->models) lds = std::max(lds, m->lds_bytes(tpbs[c]));
       if (lds > lds_limit()) continue;  // this shape does not fit the largest model
-      // one launch of trialSteps steps for every sampled instance
-      double secs = 0;
-      execute(t.get(), &secs, 1);
-      const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
-      res.rate[c] = rate;
-      if (rate > best) { best = rate; bestTpb = t->tpb; bestBlocks = 0; }
+      // the grids: the default (every instance resident, or as many workgroups as are resident at once) and, when the
+      // sample exceeds the resident count, three quarters and half of it (whole workgroups per compute unit)
+      int perCU = 0;
+      hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(batch_kernel(b->mixed, true, tpbs[c])),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)),
+                "hipFuncSetAttribute");
+      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, reinterpret_cast<const void*>(batch_kernel(b->mixed, true, tpbs[c])),
+                                                             tpbs[c], lds),
+                "occupancy");
+      std::vector<int32_t> grids{0};
+      if (perCU * cus < n)
+        for (int num : {3, 2})
+          if (const int per = perCU * num / 4; per >= 1 && per < perCU && per * cus != grids.back()) grids.push_back(per * cus);
+      for (int32_t grid : grids) {
+        std::unique_ptr<ss_batch> t = make_batch(o, b->models, im);
+        for (int32_t k = 0; k < n; ++k) {  // copy the staged tables and states of the sampled instances
+          const size_t src = static_cast<size_t>(b->order[static_cast<size_t>(k)]), dst = static_cast<size_t>(k);
+          const Model& m = b->model(static_cast<int32_t>(src));
+          std::copy_n(b->hEn.begin() + b->connOff[src], m.NC, t->hEn.begin() + t->connOff[dst]);
+          std::copy_n(b->hArea.begin() + b->connOff[src], m.NC, t->hArea.begin() + t->connOff[dst]);
+          std::copy_n(b->hTOn.begin() + b->connOff[src], m.NC, t->hTOn.begin() + t->connOff[dst]);
+          std::copy_n(b->hSt.begin() + 7 * src, 7, t->hSt.begin() + 7 * dst);
+          for (auto [from, to] : {std::make_pair(&b->hVol, &t->hVol), std::make_pair(&b->hLevel, &t->hLevel), std::make_pair(&b->hCx, &t->hCx),
+                                  std::make_pair(&b->hCy, &t->hCy), std::make_pair(&b->hCz, &t->hCz)})
+            std::copy_n(from->begin() + b->nodeOff[src], m.NN, to->begin() + t->nodeOff[dst]);
+        }
+        t->tpb = tpbs[c];
+        t->blocks = grid;
+        // one launch of trialSteps steps for every sampled instance
+        double secs = 0;
+        execute(t.get(), &secs, 1);
+        const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
+        if (res.trials < 8) {
+          res.trialThreads[res.trials] = tpbs[c];
+          res.trialWorkgroups[res.trials] = grid;
+          res.trialRate[res.trials] = rate;
+          ++res.trials;
+        }
+        res.rate[c] = std::max(res.rate[c], rate);
+        if (rate > best) { best = rate; bestTpb = tpbs[c]; bestBlocks = grid; }
+      }
     }
     b->tpb = bestTpb;
     b->blocks = bestBlocks;

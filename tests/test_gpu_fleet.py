@@ -170,3 +170,28 @@ def test_autotune_picks_a_shape_and_keeps_results():
         b.close()
     for a, c in zip(*out):
         assert np.array_equal(a, c)
+
+
+def test_autotune_grid_choice_keeps_results():
+    """With more instances than are resident at once, the autotuner also measures refilling grids of three quarters
+    and half the resident workgroups; whatever it picks, the run equals an untuned one instance for instance."""
+    from sinksim_amd.batch import Batch, BatchOptions
+    models = _models()[:2]
+    n = 6000
+    inst_model = (np.arange(n) % 2).astype(np.int32)
+    out = []
+    for tune in (False, True):
+        b = Batch.fleet(models, inst_model, BatchOptions(count=n, mixed_precision=False, traced_instances=0, curve_every=30.0,
+                                                         t_max=60.0))
+        if tune:
+            r = b.autotune(sample=n, trial_steps=60)
+            grids = {t["workgroups"] for t in r["trials"]}
+            assert len(r["trials"]) >= 3 and any(g > 0 for g in grids), r
+            assert r["workgroups"] in grids and r["threads_per_sim"] in (256, 512)
+        b.run()
+        out.append((b.results(), [b.curve(i) for i in range(0, n, 97)]))
+        b.close()
+    (r0, c0), (r1, c1) = out
+    assert np.array_equal(r0["steps"], r1["steps"])
+    for a, c in zip(c0, c1):
+        assert np.array_equal(a, c)

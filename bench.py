@@ -44,14 +44,16 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint", "calibration", "calibration-titanic"],
+    ap.add_argument("--workload", default="titanic", choices=["titanic", "uneven", "adjoint", "calibration", "calibration-titanic",
+                                                           "calibration-starts"],
                     help="titanic: the 1,024-instance titanic64 batch (the headline); uneven: the persistent batch of "
                          "seeded synthetic hulls of uneven size; adjoint: gradients of the twin-experiment loss of 256 "
                          "titanic64 instances; calibration: twin experiments with 24 kind-zone factors on the 16,680-"
                          "column calibration hull, the 1,320-run search per problem against batched L-BFGS with adjoint "
                          "gradients, device time per problem; calibration-titanic: the reference's search against "
-                         "gradient L-BFGS on the reference objective (one problem)")
-    ap.add_argument("--starts", type=int, default=8, help="calibration-titanic: L-BFGS starts")
+                         "gradient L-BFGS on the reference objective (one problem); calibration-starts: one twin problem "
+                         "of the calibration hull, the search against L-BFGS from 8,192 starts with a portable32 forward")
+    ap.add_argument("--starts", type=int, default=None, help="L-BFGS starts (calibration-titanic: 256; calibration-starts: 8192)")
     ap.add_argument("--problems", type=int, default=256, help="calibration: twin problems the L-BFGS batch calibrates")
     ap.add_argument("--search-problems", type=int, default=8, help="calibration: problems the search also runs on")
     ap.add_argument("--iterations", type=int, default=None, help="calibration: adjoint calls of the L-BFGS at most (default 16; 12 titanic)")
@@ -324,6 +326,39 @@ def bench_twin_calibration(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_starts_calibration(args):
+    """One twin problem of the calibration hull (24 kind-zone factors, 20,000 steps): the reference's search (5 rounds x
+    264 runs) against batched L-BFGS from `starts` starts (default 8,192: scales 1 and seeded points of the search's
+    first box), the forward in portable32 numerics and the reverse in FP64, all starts in every adjoint call. value =
+    the search's device time / the L-BFGS device time to the search's objective (null if not reached within the
+    iterations)."""
+    from sinksim_amd.calibrate import starts_compare
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    starts = args.starts or 8192
+    r = starts_compare(starts=starts, iterations=args.iterations or 2, numerics=args.numerics if args.numerics != "portable" else "portable32",
+                       log=log)
+    sw, lb = r["search"], r["lbfgs"]
+    reached = r["lbfgs_ms_to_target"]
+    out = {
+        "metric": "calibration device time, search / L-BFGS from many starts at equal objective (one problem)",
+        "value": round(r["search_ms"] / reached, 4) if reached else None, "unit": "x", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": round(1e3 * r["wall_s"], 1), "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64", "data": "synthetic: seeded hidden factors of one twin problem, observations from its run",
+        "config": {"workload": f"calibration hull (16,680 columns, 128 spaces, 600 connections), {r['problems'].steps} steps, "
+                               f"24 kind-zone area factors, one problem, {starts} L-BFGS starts, portable32 forward, FP64 reverse",
+                   "search": {"runs": sw["evaluations"], "device_s": round(sw["device_ms"] / 1e3, 3),
+                              "J_by_round": [float(f"{v:.4g}") for v in sw["J_by_round"][:, 0]], "target_J": float(f"{r['target']:.4g}")},
+                   "lbfgs": {"starts": starts, "calls": len(lb.best_J), "device_s_by_call": [round(v / 1e3, 2) for v in lb.device_ms],
+                             "best_J_by_call": [float(f"{v:.4g}") for v in lb.best_J],
+                             "device_s_to_target": round(reached / 1e3, 3) if reached else None},
+                   "device": _device()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_calibration(args):
     """The reference's calibration (tools/py/calibrate_gpu.py: 5 rounds of 264 log-uniform points around the best,
     span 0.3 shrinking by half, seed 20261008, objective J against the 1912 observations) on the GPU batch, then
@@ -337,7 +372,7 @@ def bench_calibration(args):
     def log(msg):
         print(msg, file=sys.stderr, flush=True)
 
-    r = compare(ship, sim, starts=args.starts, iterations=args.iterations or 12, log=log)
+    r = compare(ship, sim, starts=args.starts or 256, iterations=args.iterations or 12, log=log)
     sw, lb = r["search"], r["lbfgs"]
     reached = r["lbfgs_ms_to_search_J"]
     out = {
@@ -349,7 +384,7 @@ def bench_calibration(args):
                    "search": {"evaluations": sw.evaluations[-1], "best_J": round(sw.best_J[-1], 4), "device_s": round(sw.device_ms[-1] / 1e3, 3),
                               "sim_steps": sw.steps[-1], "best_scales": [round(float(np.exp(v)), 4) for v in sw.best_x[-1]],
                               "J_by_round": [round(j, 4) for j in sw.best_J]},
-                   "lbfgs": {"starts": args.starts, "calls": len(lb.best_J), "best_J": round(lb.best_J[-1], 4),
+                   "lbfgs": {"starts": args.starts or 256, "calls": len(lb.best_J), "best_J": round(lb.best_J[-1], 4),
                              "device_s": round(lb.device_ms[-1] / 1e3, 3), "device_s_to_search_J": round(reached / 1e3, 3) if reached else None,
                              "sim_steps": lb.steps[-1], "best_scales": [round(float(np.exp(v)), 4) for v in lb.best_x[-1]],
                              "J_by_call": [round(j, 4) for j in lb.best_J]},
@@ -375,6 +410,9 @@ def main():
         return
     if args.workload == "calibration-titanic":
         bench_calibration(args)
+        return
+    if args.workload == "calibration-starts":
+        bench_starts_calibration(args)
         return
     args.count = args.count or 1024
     from sinksim_amd import sweep as S

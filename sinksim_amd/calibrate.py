@@ -248,7 +248,7 @@ def lbfgs(ship, sim, x0, iterations: int = 12, memory: int = 6, kinds=CALIBRATIO
     return tr
 
 
-def compare(ship, sim, starts: int = 8, seed: int = 20261008, iterations: int = 12, rounds: int = 5, batch: int = 264, log=None) -> dict:
+def compare(ship, sim, starts: int = 256, seed: int = 20261008, iterations: int = 12, rounds: int = 5, batch: int = 264, log=None) -> dict:
     """The reference's search, then L-BFGS (from scales 1 and starts - 1 seeded points of the search's first box) until
     it reaches the search's best objective; device times and steps of both."""
     t0 = time.perf_counter()
@@ -469,6 +469,46 @@ def twin_polish(tp: TwinProblems, x0, f0, rows=None, rounds: int = 8, samples: i
         runner.close()
     trace.final_x = best_x
     return trace
+
+
+def starts_compare(starts: int = 8192, iterations: int = 2, numerics: str = "portable32", problem: int = 0, rounds: int = 5,
+                   batch: int = 264, span: float = 0.3, seed: int = 20261008, log=None) -> dict:
+    """One twin problem on the calibration hull (the hidden factors of problem `problem` of twin_problems), the
+    reference's search on it (rounds x batch runs) against batched L-BFGS from `starts` starts at once (scales 1 and
+    starts - 1 points uniform in the search's first box), the forward in `numerics` and the reverse in FP64, stopping
+    once a start reaches the search's best objective (re-scored with the differentiable loss) or after `iterations`
+    iterations. Device times of both; the L-BFGS starts share every adjoint call."""
+    t0 = time.perf_counter()
+    tp = twin_problems(problem + 1, seed=seed)
+    tp = TwinProblems(tp.ship, tp.sim, tp.params, tp.spec, tp.truth[problem:problem + 1], tp.obs[problem:problem + 1], tp.steps)
+    sw = twin_search(tp, rounds=rounds, batch=batch, span=span, seed=seed, log=log)
+    target = float(objective(tp, sw["best_x"], numerics=numerics)[0])
+    rng = np.random.default_rng(seed + 1)
+    x0 = np.vstack([np.zeros(tp.params.count), rng.uniform(-span, span, (starts - 1, tp.params.count))])
+    rep = TwinProblems(tp.ship, tp.sim, tp.params, tp.spec, np.repeat(tp.truth, starts, axis=0), np.repeat(tp.obs, starts, axis=0),
+                       tp.steps)
+    from .adjoint import GradientOptions, GradientRunner, flood_loss
+    import torch
+    runner = GradientRunner([(rep.ship, rep.sim)], np.zeros(starts, np.int32), rep.params, rep.spec, rep.steps * rep.sim.dt,
+                            GradientOptions(checkpoint_every=0, relaxed=True, max_steps=rep.steps, numerics=numerics))
+
+    def evaluate(x):
+        xt = torch.tensor(x, dtype=torch.float64, requires_grad=True)
+        loss = flood_loss(torch.exp(xt), runner, rep.obs)
+        loss.sum().backward()
+        return loss.detach().numpy().copy(), xt.grad.numpy().copy()
+
+    def progress():
+        if log:
+            log(f"starts lbfgs: call {runner.calls}, device {runner.kernel_ms / 1e3:.2f} s")
+        return runner.kernel_ms, runner.sim_steps
+
+    try:
+        lb = lbfgs_core(evaluate, x0, iterations=iterations, target=target, progress=progress)
+    finally:
+        runner.close()
+    return {"problems": tp, "search": sw, "lbfgs": lb, "target": target, "search_ms": sw["device_ms"],
+            "lbfgs_ms_to_target": lb.time_to(target), "wall_s": time.perf_counter() - t0}
 
 
 def twin_compare(problems: int = 256, search_problems: int = 8, iterations: int = 16, rounds: int = 5, batch: int = 264,

@@ -321,7 +321,7 @@ void check_options(const ss_batch_options* opt, const char* who) {
        w + ": numerics must be portable (1) or portable32 (2); the oracle and std numerics are CPU-only");
   need(opt->traced >= 0 && opt->denseSteps >= 0, w + ": negative trace option");
   need(opt->traced == 0 || opt->sampleEvery > 0, w + ": sampleEvery must be positive when tracing");
-  need(opt->tMax > 0 && std::isfinite(opt->tMax), w + ": tMax must be positive and finite");
+  need(opt->tMax >= 0 && std::isfinite(opt->tMax), w + ": tMax must be finite and not negative");
   need(opt->curveEvery >= 0, w + ": curveEvery must not be negative");
 }
 

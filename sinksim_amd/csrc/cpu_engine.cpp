@@ -829,7 +829,7 @@ ss_status ss_cpu_sweep(const ss_ship* ship, const ss_sim* sim, const ss_cpu_opti
     using ssa::need;
     need(ship && sim && opt, "ss_cpu_sweep: null argument");
     need(count >= 0, "ss_cpu_sweep: negative count");
-    need(opt->tMax > 0 && std::isfinite(opt->tMax), "ss_cpu_sweep: tMax must be positive and finite");
+    need(opt->tMax >= 0 && std::isfinite(opt->tMax), "ss_cpu_sweep: tMax must be finite and not negative");
     need(opt->curveEvery > 0, "ss_cpu_sweep: curveEvery must be positive");
     need(evCap >= 0 && curveCap >= 0, "ss_cpu_sweep: negative capacity");
     const Engine E = engine(opt->numerics);

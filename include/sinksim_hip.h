@@ -86,7 +86,7 @@ typedef struct ss_batch_options {
   double sampleEvery;     /* s between sampled records */
   double curveEvery;      /* s between readout-curve records of every instance; 0 = none */
   double tMax;            /* s; every instance stops here if it has not foundered */
-  int64_t stepsPerLaunch; /* steps per kernel launch; <= 0 runs every instance to its end in one launch */
+  int64_t stepsPerLaunch; /* steps per kernel launch (per epoch of a resident run); <= 0: the library's choice */
 } ss_batch_options;
 
 /* DeviceInfo (engine/include/sinksim/cuda/batch.hpp:19-25) */
@@ -165,11 +165,12 @@ ss_status ss_batch_set_connection_activation(ss_batch* b, int32_t instance, int3
  * applies a whole sweep uses instead of count * nconn single calls. */
 ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double* area, const double* tOn);
 
-/* No counterpart in the reference: the launch shape. threadsPerSim is the workgroup size of one simulation: 256
- * (0 selects it; the default) or 512, whose second half starts the level solve while the first computes the
- * buoyancy partials. workgroups is the number of workgroups of the launch (0, the default: as many as are resident
- * on the device at once, from the kernel's occupancy at this batch's LDS size); fewer than the instance count makes
- * the kernel refill them. SS_LOGIC after run. */
+/* No counterpart in the reference: the launch shape. threadsPerSim is the workgroup size of one simulation: 256,
+ * 512 (whose second half starts the level solve while the first computes the buoyancy partials), or 0, the default:
+ * 256, and 512 for the last epochs of a run once its live instances all fit resident at 512 (see ss_batch_run).
+ * workgroups is the number of workgroups of the launch (0, the default: as many as are resident on the device at
+ * once, from the kernel's occupancy at this batch's LDS size); fewer than the instance count makes the kernel refill
+ * them. The shape changes the time, never the results. SS_LOGIC after run. */
 ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgroups);
 
 /* The oracle's stopWhenStable rule on the device (RunOptions::stopWhenStable, engine/include/sinksim/simulation.hpp:34;
@@ -200,7 +201,12 @@ typedef struct ss_tune_result {
 ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the
- * wall time of the device work (uploads excluded, as in the reference). SS_LOGIC when called twice. */
+ * wall time of the device work (uploads excluded, as in the reference). When every instance has a resident workgroup
+ * of its own, the run is a sequence of epochs (stepsPerLaunch steps each, or by default a 32nd of the longest
+ * remaining run, 256 to 4,096 steps): before each epoch the device lists the instances that have not ended, so that
+ * they spread evenly over the compute units as others end, and, with the automatic launch shape, runs them at 512
+ * threads per simulation once they all fit resident at that size. Otherwise one refilling launch (or one per
+ * stepsPerLaunch steps). Results are the same bit for bit whatever the launches. SS_LOGIC when called twice. */
 ss_status ss_batch_run(ss_batch* b, double* seconds);
 
 /* Layer 1 of the boundary: kernel::step (engine/include/sinksim/kernel/step.hpp:41-54) over the batch, as the
@@ -262,7 +268,8 @@ ss_status ss_batch_curve(const ss_batch* b, int32_t instance, double* rec, int32
  * curve record), computed on the device with the portable numerics. */
 ss_status ss_batch_final_readouts(const ss_batch* b, int32_t instance, double* rec7);
 
-/* Device time of run(): the summed HIP-event duration of every kernel launch (ms) and the number of launches.
+/* Device time of run(): the summed HIP-event duration of its kernel launches (ms) and the number of launches (of
+ * epochs, for a run in epochs; each epoch's graph holds the batch kernel at both workgroup sizes and the next plan).
  * No reference counterpart: the measurement hook of the benchmark. */
 ss_status ss_batch_kernel_time(const ss_batch* b, double* ms, int32_t* launches);
 

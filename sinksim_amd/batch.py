@@ -28,7 +28,7 @@ class BatchOptions:
     sample_every: float = 60.0
     curve_every: float = 0.0
     t_max: float = 5 * 3600.0
-    steps_per_launch: int = 0        # <= 0: one launch runs every instance to its end
+    steps_per_launch: int = 0        # steps per launch or epoch; <= 0: the library's choice
 
 
 @dataclass
@@ -154,8 +154,9 @@ class Batch:
         self._keep = (e, a, t)
 
     def set_launch(self, threads_per_sim: int = 0, workgroups: int = 0) -> None:
-        """The launch shape: 256 (0) or 512 threads per simulation, and the workgroup count (0: as many as are
-        resident at once; fewer makes the kernel refill them)."""
+        """The launch shape: 256 or 512 threads per simulation (0, the default: 256, and 512 for a run's last epochs
+        once its live instances all fit resident at 512), and the workgroup count (0: as many as are resident at once;
+        fewer makes the kernel refill them). The shape changes the time, never the results."""
         A.check(self._lib.ss_batch_set_launch(self._h, int(threads_per_sim), int(workgroups)))
 
     def set_stop_when_stable(self, on: bool = True) -> None:
@@ -178,6 +179,8 @@ class Batch:
     # ------------------------------------------------------------------ run and results
 
     def run(self) -> float:
+        """Batch::run: every instance to its founder or tMax (in epochs when every instance is resident; see
+        ss_batch_run). Returns the wall time of the device work (s)."""
         s = C.c_double()
         A.check(self._lib.ss_batch_run(self._h, C.byref(s)))
         return s.value

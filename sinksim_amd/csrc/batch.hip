@@ -202,22 +202,37 @@ __device__ __attribute__((always_inline)) void run_instance(PC& P, BC& Bt, int i
   for (int m = tid; m < nmark; m += TPB) Bt.markT[iMark + m] = B.markT[m];
 }
 
+// An epoch's plan (ss_batch_run in the resident form): [0] the count of live instances, [1] the workgroup size that
+// runs them in this epoch, then the live instances in work order.
+constexpr int kEpochHead = 2;
+
 // The batch kernel. Persistent: each workgroup takes the next work item from the batch's counter, runs that
 // instance (of its own model) to the end of the launch, and takes another, until the queue is empty; a workgroup
 // whose instance founders or reaches tMax early refills itself with the next one, and every workgroup leaves the
 // loop once the counter has passed the item count. Otherwise (when every instance has a resident workgroup of its
-// own, so there is nothing to refill) workgroup k runs work item k; that form schedules a little better.
+// own, so there is nothing to refill) workgroup k runs work item k; that form schedules a little better. With an
+// epoch plan (resident form only), workgroup k runs the plan's k-th live instance, and a launch whose workgroup size
+// the plan does not name, or a workgroup past the live count, returns at once.
 template <bool Mixed, bool Persistent, int TPB>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void run_batch(const ModelParams* __restrict__ models,
                                                                                                   const DevBatch* __restrict__ batch,
-                                                                                                  long long maxSteps, double tMax) {
+                                                                                                  long long maxSteps, double tMax,
+                                                                                                  const int32_t* __restrict__ epochPlan) {
   BC& Bt = *(BC*)batch;
   extern __shared__ double lds_raw[];
   __shared__ int32_t item;
   SSL double* lds = (SSL double*)lds_raw;
   const int tid = threadIdx.x;
   if constexpr (!Persistent) {
-    const int inst = __builtin_amdgcn_readfirstlane(Bt.order[blockIdx.x]);
+    int inst;
+    if (epochPlan) {
+      const SSG int32_t* plan = (const SSG int32_t*)epochPlan;
+      if (__builtin_amdgcn_readfirstlane(plan[1]) != TPB) return;
+      if (static_cast<int>(blockIdx.x) >= __builtin_amdgcn_readfirstlane(plan[0])) return;
+      inst = __builtin_amdgcn_readfirstlane(plan[kEpochHead + blockIdx.x]);
+    } else {
+      inst = __builtin_amdgcn_readfirstlane(Bt.order[blockIdx.x]);
+    }
     PC& P = *(PC*)((const SSC char*)models + static_cast<size_t>(Bt.instModel[inst]) * sizeof(ModelParams));
     run_instance<Mixed, TPB>(P, Bt, inst, maxSteps, tMax, lds, tid);
     return;
@@ -238,6 +253,46 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // Resets the batch's work counter before a launch (a kernel rather than a memset, so that the pair can be captured
 // in a graph and replayed).
 __global__ void reset_counter(int32_t* counter) { *counter = 0; }
+
+// Plans an epoch of ss_batch_run: the instances of the work order that have not ended by the step loop's own test
+// (foundered, t at or past tMax, or stopped by the calm rule), in work order, and their count; every instance for
+// the first epoch (everyInstance), so that each is loaded and stored at least once, as by a single launch. The epoch
+// runs them on tailTpb threads per simulation once all of them fit resident at that size (at most tailFit), else on
+// mainTpb. One workgroup of 1,024 threads compacts the order a tile at a time (a ballot per wave, the waves' counts
+// in LDS).
+__global__ __launch_bounds__(1024) void plan_epoch(const DevBatch* __restrict__ batch, double tMax, int32_t* __restrict__ epochPlan,
+                                                   int32_t mainTpb, int32_t tailTpb, int32_t tailFit, int32_t everyInstance) {
+  BC& Bt = *(BC*)batch;
+  SSG int32_t* plan = (SSG int32_t*)epochPlan;
+  __shared__ int32_t waveLive[16];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int32_t base = 0;
+  for (int k0 = 0; k0 < Bt.count; k0 += 1024) {
+    const int k = k0 + tid;
+    bool live = false;
+    int32_t inst = 0;
+    if (k < Bt.count) {
+      inst = Bt.order[k];
+      const bool calm = Bt.stopWhenStable && Bt.calm[inst] > kCalmStop;
+      live = everyInstance || !(Bt.foundered[inst] != 0 || Bt.st[static_cast<size_t>(inst) * 7] >= tMax || calm);
+    }
+    const uint64_t m = __ballot(live);
+    if (lane == 0) waveLive[wave] = __popcll(m);
+    __syncthreads();
+    int32_t at = base, total = 0;
+    for (int w = 0; w < 16; ++w) {
+      if (w < wave) at += waveLive[w];
+      total += waveLive[w];
+    }
+    if (live) plan[kEpochHead + at + __popcll(m & ((1ull << lane) - 1))] = inst;
+    base += total;
+    __syncthreads();  // every wave has read the counts before the next tile writes them
+  }
+  if (tid == 0) {
+    plan[0] = base;
+    plan[1] = tailTpb > 0 && base <= tailFit ? tailTpb : mainTpb;
+  }
+}
 
 }  // namespace ssa
 
@@ -610,7 +665,7 @@ void ssa::upload(ss_batch* b) {
 
 namespace {
 
-using BatchKernel = void (*)(const ModelParams*, const DevBatch*, long long, double);
+using BatchKernel = void (*)(const ModelParams*, const DevBatch*, long long, double, const int32_t*);
 
 BatchKernel batch_kernel(bool mixed, bool persistent, int tpb) {
   if (tpb == 512) {
@@ -650,17 +705,116 @@ LaunchPlan plan_launch(ss_batch* b) {
   return LaunchPlan{batch_kernel(b->mixed, b->refill, b->tpb), grid};
 }
 
-// Uploads the staged instances, runs them to their end (one graph launch, or one per chunk of stepsPerLaunch
-// steps; at most maxLaunches) and reads the results back. The launch shape is the batch's (tpb, blocks).
-void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
+// The steps of an epoch: the batch's stepsPerLaunch when set, else a 32nd of the most steps any instance can still
+// take before tMax (at least 256, at most 4,096).
+long long epoch_steps(const ss_batch* b) {
+  if (b->opt.stepsPerLaunch > 0) return static_cast<long long>(b->opt.stepsPerLaunch);
+  double most = 0;
+  for (int32_t i = 0; i < b->opt.count; ++i) {
+    const double left = b->opt.tMax - b->hSt[static_cast<size_t>(i) * 7];
+    if (left > 0) most = std::max(most, std::ceil(left / b->model(i).net.dt) + 2);
+  }
+  return std::clamp(static_cast<long long>(std::ceil(most / 32)), 256LL, 4096LL);
+}
+
+// The resident form of a run as epochs. The batch kernel runs each instance that plan_epoch has listed for up to
+// epoch_steps steps on a workgroup of its own, so that the device's dispatcher spreads the live instances evenly
+// over the compute units (one long instance on a compute unit with three others that finished early otherwise runs
+// alone there while other compute units hold several). With the automatic launch shape (tailForm) at 256 threads,
+// once the live instances fit resident at 512 threads the epoch runs them on the 512-thread form, whose step is
+// shorter for an instance with the compute unit to itself. The epoch (main form, tail form, the next epoch's plan) is
+// one graph, replayed in groups of eight without host waits; the plan after each group counts the instances still
+// live, and the host stops once a group has left none (seen one group later, so the device never waits for the
+// host). Epochs change the launches, not the results: an instance's state is in global memory between launches, as
+// in chunked runs.
+void run_epochs(ss_batch* b, const LaunchPlan& lp, hipStream_t stream, hipEvent_t e0, hipEvent_t e1) {
+  const int32_t count = b->opt.count;
+  const long long steps = epoch_steps(b);
+  b->epochPlan.alloc(static_cast<size_t>(kEpochHead + count));
+  int32_t tailTpb = 0, tailFit = 0;
+  size_t tailLds = 0;
+  if (b->tailForm && b->tpb == kThreads) {
+    for (const auto& m : b->models) tailLds = std::max(tailLds, m->lds_bytes(512));
+    if (tailLds <= lds_limit()) {
+      const void* k = reinterpret_cast<const void*>(batch_kernel(b->mixed, false, 512));
+      if (tailLds > 64 * 1024)
+        hip_check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(tailLds)), "hipFuncSetAttribute");
+      int perCU = 0, cus = 0;
+      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, 512, tailLds), "occupancy");
+      hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+      if (perCU > 0) {
+        tailTpb = 512;
+        tailFit = perCU * cus;
+        std::vector<ModelParams> mp;
+        for (const auto& m : b->models) mp.push_back(m->params(512));
+        b->dModelsTail.upload(mp);
+      }
+    }
+  }
+  const DevBatch* dB = static_cast<const DevBatch*>(b->dBatch.p);
+  const double tMax = b->opt.tMax;
+  int32_t* plan = b->epochPlan.p;
+
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t g[2] = {nullptr, nullptr};
+  int32_t* live = nullptr;  // pinned: the live count after each group, two groups in flight
+  struct Release {
+    hipGraph_t& gr; hipGraphExec_t& x; hipEvent_t* ev; int32_t*& h;
+    ~Release() {
+      if (x) (void)hipGraphExecDestroy(x);
+      if (gr) (void)hipGraphDestroy(gr);
+      for (int i = 0; i < 2; ++i)
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+      if (h) (void)hipHostFree(h);
+    }
+  } release{graph, exec, g, live};
+  for (auto& e : g) hip_check(hipEventCreate(&e), "hipEventCreate");
+  hip_check(hipHostMalloc(reinterpret_cast<void**>(&live), 2 * sizeof(int32_t), hipHostMallocDefault), "hipHostMalloc");
+  live[0] = live[1] = -1;
+
+  hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+  hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p), dB, steps,
+                     tMax, plan);
+  if (tailTpb)
+    hipLaunchKernelGGL(batch_kernel(b->mixed, false, tailTpb), dim3(std::min(count, tailFit)), dim3(tailTpb), tailLds, stream,
+                       static_cast<const ModelParams*>(b->dModelsTail.p), dB, steps, tMax, plan);
+  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, 0);
+  hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+  hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+  hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+
+  constexpr int kGroup = 8;
+  hip_check(hipEventRecord(e0, stream), "hipEventRecord");
+  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, 1);
+  hip_check(hipGetLastError(), "plan_epoch launch");
+  for (int grp = 0;; ++grp) {
+    for (int e = 0; e < kGroup; ++e) hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");
+    b->launches += kGroup;
+    hip_check(hipMemcpyAsync(live + (grp & 1), plan, sizeof(int32_t), hipMemcpyDeviceToHost, stream), "hipMemcpyAsync");
+    hip_check(hipEventRecord(g[grp & 1], stream), "hipEventRecord");
+    if (grp > 0) {
+      hip_check(hipEventSynchronize(g[(grp - 1) & 1]), "kernel execution");
+      if (live[(grp - 1) & 1] == 0) break;
+    }
+  }
+  hip_check(hipEventRecord(e1, stream), "hipEventRecord");
+  hip_check(hipEventSynchronize(e1), "kernel execution");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+  b->kernelMs += ms;
+}
+
+// Uploads the staged instances, runs them to their end and reads the results back. The launch shape is the batch's
+// (tpb, blocks). With `epochs` and every instance resident at once, as epochs (run_epochs); otherwise one graph
+// launch, or one per chunk of stepsPerLaunch steps, at most maxLaunches.
+void execute(ss_batch* b, double* seconds, bool epochs, int maxLaunches = INT_MAX) {
   const size_t count = static_cast<size_t>(b->opt.count);
   upload(b);
 
   const long long maxSteps = b->opt.stepsPerLaunch > 0 ? static_cast<long long>(b->opt.stepsPerLaunch) : LLONG_MAX;
   const LaunchPlan lp = plan_launch(b);
 
-  // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
-  // replayed for every chunk: identical launches, one submission each.
   hipStream_t stream = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -678,16 +832,27 @@ void execute(ss_batch* b, double* seconds, int maxLaunches = INT_MAX) {
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreateWithFlags");
   hip_check(hipEventCreate(&e0), "hipEventCreate");
   hip_check(hipEventCreate(&e1), "hipEventCreate");
+  b->kernelMs = 0;
+  b->launches = 0;
+  if (epochs && !b->refill) {
+    const auto t0 = std::chrono::steady_clock::now();
+    run_epochs(b, lp, stream, e0, e1);
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    read_back(b);
+    if (seconds) *seconds = secs;
+    return;
+  }
+
+  // The launch pair (counter reset, batch kernel) is captured once in a graph on a stream of the batch's own and
+  // replayed for every chunk: identical launches, one submission each.
   hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, stream, b->counter.p);
   hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p),
-                     static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax);
+                     static_cast<const DevBatch*>(b->dBatch.p), maxSteps, b->opt.tMax, nullptr);
   hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
   const auto t0 = std::chrono::steady_clock::now();
-  b->kernelMs = 0;
-  b->launches = 0;
   std::vector<uint8_t> f(count);
   std::vector<double> st(count * 7), calm(count, 0.0);
   for (;;) {
@@ -748,7 +913,7 @@ void step_exact(ss_batch* b, long long k, const int64_t* limits, double* seconds
   hipLaunchKernelGGL(reset_counter, dim3(1), dim3(1), 0, nullptr, b->counter.p);
   if (most > 0)
     hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, nullptr, static_cast<const ModelParams*>(b->dModels.p),
-                       static_cast<const DevBatch*>(b->dBatch.p), most, DBL_MAX);
+                       static_cast<const DevBatch*>(b->dBatch.p), most, DBL_MAX, nullptr);
   hip_check(hipGetLastError(), "kernel launch");
   hip_check(hipEventRecord(e1, nullptr), "hipEventRecord");
   hip_check(hipEventSynchronize(e1), "kernel execution");
@@ -772,6 +937,7 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
     need(threadsPerSim == 0 || threadsPerSim == kThreads || threadsPerSim == 512, "ss_batch_set_launch: threadsPerSim must be 0, 256 or 512");
     need(workgroups >= 0, "ss_batch_set_launch: negative workgroup count");
     b->tpb = threadsPerSim == 0 ? kThreads : threadsPerSim;
+    b->tailForm = threadsPerSim == 0;
     b->blocks = workgroups;
     b->ldsBytes = forward_lds_bytes(*b);
   });
@@ -841,7 +1007,7 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
         t->blocks = grid;
         // one launch of trialSteps steps for every sampled instance
         double secs = 0;
-        execute(t.get(), &secs, 1);
+        execute(t.get(), &secs, false, 1);
         const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
         if (res.trials < 8) {
           res.trialThreads[res.trials] = tpbs[c];
@@ -854,6 +1020,7 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
       }
     }
     b->tpb = bestTpb;
+    b->tailForm = true;
     b->blocks = bestBlocks;
     b->ldsBytes = forward_lds_bytes(*b);
     res.threadsPerSim = bestTpb;
@@ -868,7 +1035,7 @@ ss_status ss_batch_run(ss_batch* b, double* seconds) {
     if (b->stepped) fail(SS_LOGIC, "ss_batch_run: run after ss_batch_step");
     if (b->ran) fail(SS_LOGIC, "ss_batch_run: run twice");
     b->ran = true;
-    execute(b, seconds);
+    execute(b, seconds, true);
   });
 }
 
@@ -1228,7 +1395,7 @@ ss_status ss_batch_step_noalloc(const ss_batch* b, int32_t count, int64_t steps,
     hip_check(hipGetLastError(), "ss_batch_step_noalloc: staging launch");
     if (steps > 0) {
       hipLaunchKernelGGL(kernel, dim3(grid), dim3(b->tpb), lds, s, static_cast<const ModelParams*>(dM), static_cast<const DevBatch*>(dB),
-                         static_cast<long long>(steps), DBL_MAX);
+                         static_cast<long long>(steps), DBL_MAX, nullptr);
       hip_check(hipGetLastError(), "ss_batch_step_noalloc: batch launch");
     }
     hipLaunchKernelGGL(noalloc_pack, dim3(count), dim3(256), 0, s, static_cast<const DevBatch*>(dB), m.NN, recOut);

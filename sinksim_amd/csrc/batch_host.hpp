@@ -186,6 +186,7 @@ struct ss_batch {
   int32_t recSize = 0, maxSamples = 0, curveMax = 0;
   int32_t tpb = kThreads;  // threads per simulation (workgroup size): 256 or 512
   int32_t blocks = 0;   // workgroups of the persistent launch (0: as many as fit on the device at once)
+  bool tailForm = true;  // the automatic shape: a run's last epochs at 512 threads once the live instances fit
   bool refill = false;  // whether the last run used the refilling (persistent) kernel
   int64_t totalSteps = 0;
   bool ran = false;
@@ -202,6 +203,8 @@ struct ss_batch {
   size_t adjLds = 0;
 
   Dev<ModelParams> dModels;
+  Dev<ModelParams> dModelsTail;  // the models at 512 threads, for a run's tail epochs
+  Dev<int32_t> epochPlan;        // the current epoch's live instances (run_epochs)
   Dev<DevBatch> dBatch;
   Dev<int32_t> dInstModel, dOrder, dNodeOff, dConnOff, dEvOff, dMonOff, dMarkOff, counter;
   // per instance

@@ -186,14 +186,14 @@ __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, in
 #pragma unroll
   for (int k = 0; k < 10; ++k) p.S[k] = 0;
   const int s0 = B.nSegStart[n], s1 = s0 + B.nSegCount[n];
-  stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g) {
+  stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) {
     const double x = g.x, y = g.y, a = g.lo, b = g.hi;
     const double base = R20 * x + R21 * y + tz;
     const double wa = base + R22 * a, wb = base + R22 * b;
-    if (wa < p.zmin) { p.zmin = wa; p.xmn = x; p.ymn = y; p.amn = a; }
-    if (wb > p.zmax) { p.zmax = wb; p.xmx = x; p.ymx = y; p.bmx = b; }
+    if (on && wa < p.zmin) { p.zmin = wa; p.xmn = x; p.ymn = y; p.amn = a; }
+    if (on && wb > p.zmax) { p.zmax = wb; p.xmx = x; p.ymx = y; p.bmx = b; }
     const double zp = (h - base) * iR;
-    if (zp <= a) return;
+    if (!on || zp <= a) return;
     const double A = g.A;
     double L;
     if (zp >= b) {

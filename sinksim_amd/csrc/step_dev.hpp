@@ -408,45 +408,67 @@ __device__ __forceinline__ Fr32 fr32(const SSL double* fr) {
 // A column's or a segment's geometry: x, y, bottom, top and footprint (for a segment: those of its column, with
 // the segment's own bottom and top). Records are 48 bytes (double) or 32 bytes (float), addressed by a 32-bit
 // byte offset from the table's base.
+//
+// Loads and their registers: a load writes every register of its destination when it lands, used or not, so a
+// register the compiler believes free (the unused rest of a vector) cannot take another value until the load has
+// landed, and the compiler waits for the load before reusing it. A double record is therefore loaded as two
+// 16-byte loads and one 8-byte load of the footprint (no unused register). A float record keeps two 16-byte loads
+// (measured faster than a 16-byte and a 4-byte load), and the three padding lanes of the second are held live
+// (`hold`, an empty asm that reads them) until the record's wait in `stream`, after which they are free.
+template <class T>
+struct GeoPad {};
+template <>
+struct GeoPad<float> {
+  float p1, p2, p3;
+};
 template <class T>
 struct Geo {
   T x, y, lo, hi, A;
+  GeoPad<T> pad;
 };
+__device__ __forceinline__ void hold(const Geo<double>&) {}
+__device__ __forceinline__ void hold(const Geo<float>& g) { asm volatile("" ::"v"(g.pad.p1), "v"(g.pad.p2), "v"(g.pad.p3)); }
 // record sizes: five doubles padded to 48 bytes, five floats padded to 32
 template <class T>
 constexpr uint32_t kRecBytes = sizeof(T) == sizeof(double) ? 48u : 32u;
+// s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0, expcnt 6:4 and lgkmcnt 11:8 at their maxima, i.e. not
+// waited for) that wait until at most one record's loads are in flight: three loads per double record, two per
+// float record
+template <class T>
+constexpr int kWaitOneRecord = sizeof(T) == sizeof(double) ? 0x0F73 : 0x0F72;
 __device__ __forceinline__ Geo<double> geo_at(const SSG d2v* table, uint32_t off) {
   const SSG d2v* r = (const SSG d2v*)((const SSG char*)table + off);
-  const d2v a = r[0], b = r[1], c = r[2];
-  return Geo<double>{a.x, a.y, b.x, b.y, c.x};
+  const d2v a = r[0], b = r[1];
+  const double A = *(const SSG double*)(r + 2);
+  return Geo<double>{a.x, a.y, b.x, b.y, A, {}};
 }
 __device__ __forceinline__ Geo<float> geo_at(const SSG f4v* table, uint32_t off) {
   const SSG f4v* r = (const SSG f4v*)((const SSG char*)table + off);
   const f4v a = r[0], b = r[1];
-  return Geo<float>{a.x, a.y, a.z, a.w, b.x};
+  return Geo<float>{a.x, a.y, a.z, a.w, b.x, {b.y, b.z, b.w}};
 }
 __device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<double>); }
 __device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<float>); }
 
 // one column into a buoyancy partial (reduce.hpp:39-49 / 131-142)
-__device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, HP<double>& p) {
+__device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, bool on, HP<double>& p) {
   const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
   const double wt = base + F.R22 * hi;
-  if (wt > p.top) p.top = wt;
+  if (on && wt > p.top) p.top = wt;
   const double zp = -base * F.iR;
-  if (zp <= lo) return;
+  if (!on || zp <= lo) return;
   const double s = zp >= hi ? hi - lo : zp - lo;
   const double v = g.A * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
 }
-__device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, HP<float>& p) {
+__device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, bool on, HP<float>& p) {
   const float x = g.x, y = g.y, lo = g.lo, hi = g.hi;
   const float base = f.R20 * x + f.R21 * y + f.tz;
   const float wt = base + f.R22 * hi;
-  if (wt > p.top) p.top = wt;
+  if (on && wt > p.top) p.top = wt;
   const float zp = -base * f.iR;
-  if (zp <= lo) return;
+  if (!on || zp <= lo) return;
   const float s = zp >= hi ? hi - lo : zp - lo;
   const float v = g.A * s;
   p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5f * s);
@@ -456,16 +478,16 @@ __device__ __forceinline__ void column(const Fr32& f, const Geo<float>& g, HP<fl
 // depends on the pose only, not on the level: it is accumulated (Ext) in the first pass of a node in a step and
 // reused by the later ones, which the reference recomputes to the same values.
 template <bool Ext>
-__device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<double>& g, PP<double>& p) {
+__device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<double>& g, bool on, PP<double>& p) {
   const double x = g.x, y = g.y, a = g.lo, b = g.hi;
   const double base = F.R20 * x + F.R21 * y + F.tz;
   if constexpr (Ext) {
     const double wa = base + F.R22 * a, wb = base + F.R22 * b;
-    if (wa < p.zmin) p.zmin = wa;
-    if (wb > p.zmax) p.zmax = wb;
+    if (on && wa < p.zmin) p.zmin = wa;
+    if (on && wb > p.zmax) p.zmax = wb;
   }
   const double zp = (h - base) * F.iR;
-  if (zp <= a) return;
+  if (!on || zp <= a) return;
   const double A = g.A;
   double L;
   if (zp >= b) {
@@ -478,16 +500,16 @@ __device__ __forceinline__ void segment(const Fr64& F, double h, const Geo<doubl
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5 * L);
 }
 template <bool Ext>
-__device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>& g, PP<float>& p) {
+__device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>& g, bool on, PP<float>& p) {
   const float x = g.x, y = g.y, a = g.lo, b = g.hi;
   const float base = f.R20 * x + f.R21 * y + f.tz;
   if constexpr (Ext) {
     const float wa = base + f.R22 * a, wb = base + f.R22 * b;
-    if (wa < p.zmin) p.zmin = wa;
-    if (wb > p.zmax) p.zmax = wb;
+    if (on && wa < p.zmin) p.zmin = wa;
+    if (on && wb > p.zmax) p.zmax = wb;
   }
   const float zp = (h - base) * f.iR;
-  if (zp <= a) return;
+  if (!on || zp <= a) return;
   const float A = g.A;
   float L;
   if (zp >= b) {
@@ -500,37 +522,50 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
   p.vol += v; p.mx += v * x; p.my += v * y; p.mz += v * (a + 0.5f * L);
 }
 
-// A lane's partial over the records s0 + lane, s0 + lane + step, ... below s1, in order, each record's load
-// issued while the previous record is evaluated. Unrolled by two so that the two records in flight keep their
-// registers (no copies between iterations); records are addressed by byte offsets advanced by step records (no
-// multiplication per record). A lane past its last record reloads its current one (in bounds) and stops.
+// A lane's partial over the records first, first + step, ... below end, in order, each record's load issued while
+// the previous record is evaluated. Unrolled by two so that the two records in flight keep their registers (no
+// copies between iterations); records are addressed by byte offsets advanced by step records (no multiplication
+// per record). The loop is a scalar one with its only exit at the bottom: it runs an even number of records, at
+// least as many as the wave's longest lane needs (within each half of the wave `first` ascends with the lane and
+// `end` is common, so the lowest active lane of each half has the most records). A lane past its last record
+// reloads that record (in bounds), a lane without records reads record 0 (in bounds, since another lane of the wave
+// has records), and eval(g, on) leaves the partial unchanged when on is false. Before each evaluation an explicit
+// wait for that record's loads (the other record's stay in flight) is made on every path, also where no lane
+// evaluates. The compiler's own waits would otherwise come at the loop's head, before the next record's loads are
+// issued, on any path that can reach the head with loads in flight (a lane leaving the loop early, a break in the
+// middle, an evaluation skipped by a branch): one record in flight instead of two.
 // The scheduling barriers keep the next record's loads ahead of the current record's arithmetic: without them
 // the compiler sinks the loads below the evaluation and waits for them at once (vmcnt(0)).
 template <class T, class Tab, class Fn>
 __device__ __forceinline__ void stream(const Tab* table, int first, int end, int step, Fn&& eval) {
-  int j = first;
-  if (j >= end) return;
+  const int mine = first < end ? (end - first + step - 1) / step : 0;  // this lane's records
+  const uint64_t ex = __builtin_amdgcn_read_exec();
+  const uint32_t ex0 = static_cast<uint32_t>(ex), ex1 = static_cast<uint32_t>(ex >> 32);
+  int rounds = 0;
+  if (ex0) rounds = __builtin_amdgcn_readlane(mine, __builtin_ctz(ex0));
+  if (ex1) rounds = max(rounds, __builtin_amdgcn_readlane(mine, 32 + __builtin_ctz(ex1)));
+  if (rounds == 0) return;
   constexpr uint32_t rec = kRecBytes<T>;
   const uint32_t stride = static_cast<uint32_t>(step) * rec;
-  uint32_t off = static_cast<uint32_t>(j) * rec;
+  const uint32_t last = mine > 0 ? static_cast<uint32_t>(first + (mine - 1) * step) * rec : 0u;
+  uint32_t off = mine > 0 ? static_cast<uint32_t>(first) * rec : 0u;
   Geo<T> g0 = geo_at(table, off);
-  while (true) {
-    const int j1 = j + step;
-    const bool m1 = j1 < end;
-    const uint32_t off1 = m1 ? off + stride : off;
+  int k = 0;
+  do {
+    const uint32_t off1 = min(off + stride, last);
     const Geo<T> g1 = geo_at(table, off1);
     __builtin_amdgcn_sched_barrier(0);
-    eval(g0);
-    if (!m1) break;
-    const int j2 = j1 + step;
-    const bool m2 = j2 < end;
-    off = m2 ? off1 + stride : off1;
+    __builtin_amdgcn_s_waitcnt(kWaitOneRecord<T>);  // g0 has arrived; g1 stays in flight
+    hold(g0);
+    eval(g0, k < mine);
+    off = min(off1 + stride, last);
     g0 = geo_at(table, off);
     __builtin_amdgcn_sched_barrier(0);
-    eval(g1);
-    if (!m2) break;
-    j = j2;
-  }
+    __builtin_amdgcn_s_waitcnt(kWaitOneRecord<T>);  // g1 has arrived; the next g0 stays in flight
+    hold(g1);
+    eval(g1, k + 1 < mine);
+    k += 2;
+  } while (k < rounds);
 }
 
 struct Pass {
@@ -554,7 +589,7 @@ __device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, 
     const Fr32 f = fr32(fr);
     const float hf = static_cast<float>(h);
     PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
-    stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g) { segment<Ext>(f, hf, g, p); });
+    stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g, bool on) { segment<Ext>(f, hf, g, on, p); });
     const PP<float> t = tree32_all<Ext>(p);
     vol = static_cast<double>(t.vol); dv = static_cast<double>(t.dv);
     mx = static_cast<double>(t.mx); my = static_cast<double>(t.my); mz = static_cast<double>(t.mz);
@@ -562,7 +597,7 @@ __device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, 
   } else {
     const Fr64 F = fr64(fr);
     PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
-    stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g) { segment<Ext>(F, h, g, p); });
+    stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) { segment<Ext>(F, h, g, on, p); });
     const PP<double> t = tree32_all<Ext>(p);
     vol = t.vol; dv = t.dv; mx = t.mx; my = t.my; mz = t.mz;
     if constexpr (Ext) { zmin = t.zmin; zmax = t.zmax; }
@@ -1145,7 +1180,7 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   if constexpr (Mixed) {
     const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
-    stream<float>(P.S.colF, tid, P.S.ncol, kThreads, [&](const Geo<float>& g) { column(f, g, p); });
+    stream<float>(P.S.colF, tid, P.S.ncol, kThreads, [&](const Geo<float>& g, bool on) { column(f, g, on, p); });
     float vx, yz, top;
     tree32_packed(p, vx, yz, top);
     if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }
@@ -1153,7 +1188,7 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   } else {
     const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
-    stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g) { column(F, g, p); });
+    stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) { column(F, g, on, p); });
     double vx, yz, top;
     tree32_packed(p, vx, yz, top);
     if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }

@@ -80,7 +80,7 @@ __device__ __attribute__((always_inline)) void run_instance(PC& P, BC& Bt, int i
   }
   for (int n = tid; n < NN; n += TPB) {
     B.vol[n] = Bt.vol[iN + n]; B.level[n] = Bt.level[iN + n];
-    B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n];
+    B.cx[n] = Bt.cx[iN + n]; B.cy[n] = Bt.cy[iN + n]; B.cz[n] = Bt.cz[iN + n]; B.cvol[n] = 0;
     B.area[n] = 0; B.zmin[n] = 0; B.zmax[n] = 0; B.hEff[n] = 0; B.aEff[n] = 0; B.excess[n] = 0; B.deg[n] = 0;
     B.tX[n] = 0; B.tY[n] = 0; B.tF[n] = 0; B.tM[n] = 0; B.tIp[n] = 0; B.tIr[n] = 0;
   }

@@ -119,7 +119,7 @@ inline LdsLayout lds_layout(int32_t NN, int32_t NC, int32_t NG, int32_t nmon, in
   auto take = [&](int32_t doubles) { const int32_t at = off; off += std::max<int32_t>(doubles, 1); return at; };
   auto ints = [](int32_t n) { return (n + 1) / 2; };
   auto bytes = [](int32_t n) { return (n + 7) / 8; };
-  L.vol = take(NN); L.level = take(NN); L.cx = take(NN); L.cy = take(NN); L.cz = take(NN); L.area = take(NN); L.zmin = take(NN);
+  L.vol = take(NN); L.level = take(NN); L.cx = take(NN); L.cy = take(NN); L.cz = take(NN); L.cvol = take(NN); L.area = take(NN); L.zmin = take(NN);
   L.zmax = take(NN); L.hEff = take(NN); L.aEff = take(NN); L.excess = take(NN);
   L.tX = take(NN); L.tY = take(NN); L.tF = take(NN); L.tM = take(NN); L.tIp = take(NN); L.tIr = take(NN);
   L.dV = take(NC); L.q = take(NC); L.carea = take(NC); L.ctOn = take(NC); L.seaDV = take(nsea);

@@ -160,7 +160,7 @@ struct DevBatch {
 
 // LDS layout of one simulation, as offsets in doubles (the host computes it: lds_layout in batch.hip).
 struct LdsLayout {
-  int32_t vol, level, cx, cy, cz, area, zmin, zmax, hEff, aEff, excess;
+  int32_t vol, level, cx, cy, cz, cvol, area, zmin, zmax, hEff, aEff, excess;
   int32_t tX, tY, tF, tM, tIp, tIr;
   int32_t dV, q, carea, ctOn, seaDV, passes, warpHyd, frame, scal, over, overT, markT, evT, nextGroup;
   int32_t nMu, nVmax, nAov, nAmin, gpx, gpy, gpz;  // copies of the node and group tables
@@ -187,7 +187,10 @@ enum : int { S_T = 0, S_ZO, S_PITCH, S_ROLL, S_VZ, S_WTH, S_WPH, O_INFLOW, O_VB,
              S_FOUNDERED, S_NEXTCURVE, S_CURVECOUNT, S_CALM, S_COUNT };
 
 struct Blk {
-  SSL double *vol, *level, *cx, *cy, *cz, *area, *zmin, *zmax, *hEff, *aEff, *excess;
+  // cx, cy, cz hold a node's centroid between steps; from a node's last pass in the level solve to the heads phase
+  // they hold that pass's moments and cvol its volume (0: no new centroid this step), and the heads phase divides
+  // (cvol is 0 at every step's start)
+  SSL double *vol, *level, *cx, *cy, *cz, *cvol, *area, *zmin, *zmax, *hEff, *aEff, *excess;
   SSL double *tX, *tY, *tF, *tM, *tIp, *tIr;
   SSL double *dV, *q, *carea, *ctOn, *seaDV, *passes, *warpHyd, *frame, *scal, *over, *overT, *markT, *evT;
   SSL int32_t* nextGroup;  // the next free-surface group a solver unit takes (reset to 0 before each step's solve)
@@ -199,7 +202,7 @@ struct Blk {
 __device__ inline Blk carve(SSL double* base, PC& P) {
   const SSC LdsLayout& L = P.L;
   Blk b;
-  b.vol = base + L.vol; b.level = base + L.level; b.cx = base + L.cx; b.cy = base + L.cy; b.cz = base + L.cz;
+  b.vol = base + L.vol; b.level = base + L.level; b.cx = base + L.cx; b.cy = base + L.cy; b.cz = base + L.cz; b.cvol = base + L.cvol;
   b.area = base + L.area; b.zmin = base + L.zmin; b.zmax = base + L.zmax; b.hEff = base + L.hEff; b.aEff = base + L.aEff;
   b.excess = base + L.excess;
   b.tX = base + L.tX; b.tY = base + L.tY; b.tF = base + L.tF; b.tM = base + L.tM; b.tIp = base + L.tIp; b.tIr = base + L.tIr;
@@ -614,8 +617,18 @@ __device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, 
 
 // ------------------------------------------------------------------------------------------- (b) levels
 
+// A node's new centroid (hydro.hpp:101-109: the pass's moments over its volume, kept when the volume is too small),
+// left as the moments and the volume for the heads phase to divide (finish_centroid): the three divisions leave the
+// level solve's chain of dependent work.
 __device__ __forceinline__ void put_centroid(const Blk& B, int n, const Pass& p) {
-  if (p.vol > kCentroidMinVolume) { B.cx[n] = p.mx / p.vol; B.cy[n] = p.my / p.vol; B.cz[n] = p.mz / p.vol; }
+  if (p.vol > kCentroidMinVolume) { B.cx[n] = p.mx; B.cy[n] = p.my; B.cz[n] = p.mz; B.cvol[n] = p.vol; }
+}
+__device__ __forceinline__ void finish_centroid(const Blk& B, int n) {
+  const double v = B.cvol[n];
+  if (v > 0) {
+    B.cx[n] = B.cx[n] / v; B.cy[n] = B.cy[n] / v; B.cz[n] = B.cz[n] / v;
+    B.cvol[n] = 0;
+  }
 }
 
 // Pp (a unit's pass scratch for the members of a merged group) is written by every lane of the unit (each lane
@@ -1033,6 +1046,7 @@ __device__ __forceinline__ void node_heads(PC& P, const Blk& B, int n, int j) {
     B.aEff[n] = v >= B.nVmax[n] ? B.nAov[n] : jmax(B.area[n], B.nAmin[n]);
     B.deg[n] = d;
   }
+  if (j == 1) finish_centroid(B, n);
 }
 
 // The Villemonte factor of a submerged weir, (1 - r^1.5)^0.385 with the base clipped at 0 (flows.hpp:59): both
@@ -1241,14 +1255,31 @@ __device__ void step_tail(PC& P, const Blk& B, int lane) {
     // node_update stored each chain's terms gated and signed: the chains only add
     const SSL double* src = lane == 0 ? B.tX : lane == 1 ? B.tY : lane == 2 ? B.tF : lane == 3 ? B.tM : lane == 4 ? B.tIp : lane == 5 ? B.tIr : B.excess;
     acc = lane == 0 ? (byw - P.M.YO) * Fb : lane == 1 ? -(bxw - P.M.XO) * Fb : lane == 2 ? Fb - P.M.g * P.M.ms : lane == 6 ? B.scal[S_SEA] : 0.0;
-    for (int n0 = 0; n0 < NN; n0 += 8) {
-      double sv[8];
+    // whole batches of eight nodes, each batch's terms read while the previous batch is added (the next batch's
+    // start clamped in bounds: the last batch is read twice and added once), then the remaining nodes one by one
+    const int full = NN & ~7;
+    if (full > 0) {
+      double a[8], b[8];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) sv[k] = src[n0 + k < NN ? n0 + k : NN - 1];
+      for (int k = 0; k < 8; ++k) a[k] = src[k];
+      for (int n0 = 0;;) {
+        const int nb = min(n0 + 8, full - 8);
 #pragma unroll
-      for (int k = 0; k < 8; ++k)
-        if (n0 + k < NN) acc += sv[k];
+        for (int k = 0; k < 8; ++k) b[k] = src[nb + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += a[k];
+        n0 += 8;
+        if (n0 >= full) break;
+        const int na = min(n0 + 8, full - 8);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) a[k] = src[na + k];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += b[k];
+        n0 += 8;
+        if (n0 >= full) break;
+      }
     }
+    for (int n = full; n < NN; ++n) acc += src[n];
   }
   const double tauX = bcast(acc, 0), tauY = bcast(acc, 1), Fz = bcast(acc, 2), mw = bcast(acc, 3);
   const double Iw_p = bcast(acc, 4), Iw_r = bcast(acc, 5), seaIn = bcast(acc, 6);

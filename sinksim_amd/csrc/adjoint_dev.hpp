@@ -581,14 +581,13 @@ __device__ inline void buoyancy_rec(PC& P, const TapeT<GT>& T, int tid) {
   HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
   double S[10] = {};
   double tx = 0, ty = 0, th = 0;
-  for (int i = tid; i < P.S.ncol; i += kThreads) {
-    const Geo<double> g = geo(P.S.col, i);
+  stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) {
     const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
     const double base = F.R20 * x + F.R21 * y + F.tz;
     const double wt = base + F.R22 * hi;
-    if (wt > p.top) { p.top = wt; tx = x; ty = y; th = hi; }
+    if (on && wt > p.top) { p.top = wt; tx = x; ty = y; th = hi; }
     const double zp = -base * F.iR;
-    if (zp <= lo) continue;
+    if (!on || zp <= lo) return;
     double s;
     if (zp >= hi) {
       s = hi - lo;
@@ -600,7 +599,7 @@ __device__ inline void buoyancy_rec(PC& P, const TapeT<GT>& T, int tid) {
     }
     const double v = g.A * s;
     p.V += v; p.Mx += v * x; p.My += v * y; p.Mz += v * (lo + 0.5 * s);
-  }
+  });
   auto level = [&](auto off) {
     constexpr int OFF = decltype(off)::value;
     const double V = down<OFF>(p.V), Mx = down<OFF>(p.Mx), My = down<OFF>(p.My), Mz = down<OFF>(p.Mz), top = down<OFF>(p.top);

@@ -257,11 +257,12 @@ __global__ void reset_counter(int32_t* counter) { *counter = 0; }
 // Plans an epoch of ss_batch_run: the instances of the work order that have not ended by the step loop's own test
 // (foundered, t at or past tMax, or stopped by the calm rule), in work order, and their count; every instance for
 // the first epoch (everyInstance), so that each is loaded and stored at least once, as by a single launch. The epoch
-// runs them on tailTpb threads per simulation once all of them fit resident at that size (at most tailFit), else on
-// mainTpb. One workgroup of 1,024 threads compacts the order a tile at a time (a ballot per wave, the waves' counts
+// runs them on lastTpb threads per simulation once all of them fit resident at that size (at most lastFit), else on
+// tailTpb once they fit at that size (at most tailFit), else on mainTpb (a size of 0: that form is not used). One workgroup of 1,024 threads compacts the order a tile at a time (a ballot per wave, the waves' counts
 // in LDS).
 __global__ __launch_bounds__(1024) void plan_epoch(const DevBatch* __restrict__ batch, double tMax, int32_t* __restrict__ epochPlan,
-                                                   int32_t mainTpb, int32_t tailTpb, int32_t tailFit, int32_t everyInstance) {
+                                                   int32_t mainTpb, int32_t tailTpb, int32_t tailFit, int32_t lastTpb, int32_t lastFit,
+                                                   int32_t everyInstance) {
   BC& Bt = *(BC*)batch;
   SSG int32_t* plan = (SSG int32_t*)epochPlan;
   __shared__ int32_t waveLive[16];
@@ -290,7 +291,7 @@ __global__ __launch_bounds__(1024) void plan_epoch(const DevBatch* __restrict__ 
   }
   if (tid == 0) {
     plan[0] = base;
-    plan[1] = tailTpb > 0 && base <= tailFit ? tailTpb : mainTpb;
+    plan[1] = lastTpb > 0 && base <= lastFit ? lastTpb : tailTpb > 0 && base <= tailFit ? tailTpb : mainTpb;
   }
 }
 
@@ -668,6 +669,7 @@ namespace {
 using BatchKernel = void (*)(const ModelParams*, const DevBatch*, long long, double, const int32_t*);
 
 BatchKernel batch_kernel(bool mixed, bool persistent, int tpb) {
+  if (tpb == 1024) return mixed ? run_batch<true, false, 1024> : run_batch<false, false, 1024>;  // resident epochs only
   if (tpb == 512) {
     if (persistent) return mixed ? run_batch<true, true, 512> : run_batch<false, true, 512>;
     return mixed ? run_batch<true, false, 512> : run_batch<false, false, 512>;
@@ -721,9 +723,10 @@ long long epoch_steps(const ss_batch* b) {
 // epoch_steps steps on a workgroup of its own, so that the device's dispatcher spreads the live instances evenly
 // over the compute units (one long instance on a compute unit with three others that finished early otherwise runs
 // alone there while other compute units hold several). With the automatic launch shape (tailForm) at 256 threads,
-// once the live instances fit resident at 512 threads the epoch runs them on the 512-thread form, whose step is
-// shorter for an instance with the compute unit to itself. The epoch (main form, tail form, the next epoch's plan) is
-// one graph, replayed in groups of eight without host waits; the plan after each group counts the instances still
+// once the live instances fit resident at 512 threads the epoch runs them on the 512-thread form, and once they fit
+// one per compute unit on the 1,024-thread form: the step of an instance with a compute unit (or half of one) to
+// itself is shorter on more threads, its level solve spread over more waves. The epoch (main form, the two tail forms,
+// the next epoch's plan) is one graph, replayed in groups of eight without host waits; the plan after each group counts the instances still
 // live, and the host stops once a group has left none (seen one group later, so the device never waits for the
 // host). Epochs change the launches, not the results: an instance's state is in global memory between launches, as
 // in chunked runs.
@@ -731,26 +734,35 @@ void run_epochs(ss_batch* b, const LaunchPlan& lp, hipStream_t stream, hipEvent_
   const int32_t count = b->opt.count;
   const long long steps = epoch_steps(b);
   b->epochPlan.alloc(static_cast<size_t>(kEpochHead + count));
-  int32_t tailTpb = 0, tailFit = 0;
-  size_t tailLds = 0;
+  // the tail forms: 512 threads per simulation once the live instances fit resident at that size, 1,024 once they fit
+  // one per compute unit (each form only where every model's layout fits its LDS and a workgroup is resident)
+  struct Form {
+    int32_t tpb = 0, fit = 0;
+    size_t lds = 0;
+  } forms[2];
   if (b->tailForm && b->tpb == kThreads) {
-    for (const auto& m : b->models) tailLds = std::max(tailLds, m->lds_bytes(512));
-    if (tailLds <= lds_limit()) {
-      const void* k = reinterpret_cast<const void*>(batch_kernel(b->mixed, false, 512));
-      if (tailLds > 64 * 1024)
-        hip_check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(tailLds)), "hipFuncSetAttribute");
+    Dev<ModelParams>* params[2] = {&b->dModelsTail, &b->dModelsLast};
+    const int32_t sizes[2] = {512, 1024};
+    for (int f = 0; f < 2; ++f) {
+      size_t lds = 0;
+      for (const auto& m : b->models) lds = std::max(lds, m->lds_bytes(sizes[f]));
+      if (lds > lds_limit()) break;
+      const void* k = reinterpret_cast<const void*>(batch_kernel(b->mixed, false, sizes[f]));
+      if (lds > 64 * 1024)
+        hip_check(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)), "hipFuncSetAttribute");
       int perCU = 0, cus = 0;
-      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, 512, tailLds), "occupancy");
+      hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, k, sizes[f], lds), "occupancy");
       hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
-      if (perCU > 0) {
-        tailTpb = 512;
-        tailFit = perCU * cus;
-        std::vector<ModelParams> mp;
-        for (const auto& m : b->models) mp.push_back(m->params(512));
-        b->dModelsTail.upload(mp);
-      }
+      if (perCU <= 0) break;
+      forms[f].tpb = sizes[f];
+      forms[f].fit = perCU * cus;
+      forms[f].lds = lds;
+      std::vector<ModelParams> mp;
+      for (const auto& m : b->models) mp.push_back(m->params(sizes[f]));
+      params[f]->upload(mp);
     }
   }
+  const int32_t tailTpb = forms[0].tpb, tailFit = forms[0].fit, lastTpb = forms[1].tpb, lastFit = forms[1].fit;
   const DevBatch* dB = static_cast<const DevBatch*>(b->dBatch.p);
   const double tMax = b->opt.tMax;
   int32_t* plan = b->epochPlan.p;
@@ -777,16 +789,19 @@ void run_epochs(ss_batch* b, const LaunchPlan& lp, hipStream_t stream, hipEvent_
   hipLaunchKernelGGL(lp.kernel, dim3(lp.grid), dim3(b->tpb), b->ldsBytes, stream, static_cast<const ModelParams*>(b->dModels.p), dB, steps,
                      tMax, plan);
   if (tailTpb)
-    hipLaunchKernelGGL(batch_kernel(b->mixed, false, tailTpb), dim3(std::min(count, tailFit)), dim3(tailTpb), tailLds, stream,
+    hipLaunchKernelGGL(batch_kernel(b->mixed, false, tailTpb), dim3(std::min(count, tailFit)), dim3(tailTpb), forms[0].lds, stream,
                        static_cast<const ModelParams*>(b->dModelsTail.p), dB, steps, tMax, plan);
-  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, 0);
+  if (lastTpb)
+    hipLaunchKernelGGL(batch_kernel(b->mixed, false, lastTpb), dim3(std::min(count, lastFit)), dim3(lastTpb), forms[1].lds, stream,
+                       static_cast<const ModelParams*>(b->dModelsLast.p), dB, steps, tMax, plan);
+  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, lastTpb, lastFit, 0);
   hip_check(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
   hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
   hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
 
   constexpr int kGroup = 8;
   hip_check(hipEventRecord(e0, stream), "hipEventRecord");
-  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, 1);
+  hipLaunchKernelGGL(plan_epoch, dim3(1), dim3(1024), 0, stream, dB, tMax, plan, b->tpb, tailTpb, tailFit, lastTpb, lastFit, 1);
   hip_check(hipGetLastError(), "plan_epoch launch");
   for (int grp = 0;; ++grp) {
     for (int e = 0; e < kGroup; ++e) hip_check(hipGraphLaunch(exec, stream), "hipGraphLaunch");

@@ -204,6 +204,7 @@ struct ss_batch {
 
   Dev<ModelParams> dModels;
   Dev<ModelParams> dModelsTail;  // the models at 512 threads, for a run's tail epochs
+  Dev<ModelParams> dModelsLast;  // the models at 1,024 threads, for a run's last epochs
   Dev<int32_t> epochPlan;        // the current epoch's live instances (run_epochs)
   Dev<DevBatch> dBatch;
   Dev<int32_t> dInstModel, dOrder, dNodeOff, dConnOff, dEvOff, dMonOff, dMarkOff, counter;

@@ -2,11 +2,13 @@
 
 ss_batch_run splits a resident run into epochs: before each one the device lists the instances that have not ended,
 so that the live ones spread evenly over the compute units, and the automatic shape runs them at 512 threads per
-simulation once they all fit resident at that size. None of this may change a result. The batch here holds more
-instances than fit resident at 512 threads, so the automatic run starts at 256 threads and switches part way; it is
-compared with runs held at 256 and at 512 threads (the latter in the refilling form while the instances exceed the
-resident count) and with short epochs, in both numerics: steps, founder flags and times, end times, events, every
-curve record, the final readouts and two traced instances' dense and sampled records.
+simulation once they all fit resident at that size and at 1,024 once they fit one per compute unit. None of this may
+change a result. The batch here holds more instances than fit resident at 512 threads, so the automatic run starts at
+256 threads and switches part way (to 512, then to 1,024 as instances end); it is compared with runs held at 256 and
+at 512 threads (the latter in the refilling form while the instances exceed the resident count) and with short
+epochs, in both numerics: steps, founder flags and times, end times, events, every curve record, the final readouts
+and two traced instances' dense and sampled records. The calm-rule batch (96 instances) runs at 1,024 threads
+throughout.
 """
 import numpy as np
 import pytest

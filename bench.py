@@ -33,9 +33,11 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector/matrix peak (FMA counted as two o
 # and priced by oracle/pyoracle.py:flops_per_step: 2.34e5. tests/test_bench_constants.py pins the counter against an
 # independent count of the reference's own kernels on the 1912 run and re-counts a sample of this workload.
 FLOPS_PER_STEP = 2.34e5
-# HBM bytes per launch of run_batch<false, false, 256> on this workload (FETCH_SIZE + WRITE_SIZE, calibrated against a
-# plain copy kernel: profiles/run_batch_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 41.8e6
+# HBM bytes per launch on this workload, a launch being one epoch of the resident run (one graph replay: the batch
+# kernel at 256, 512 and 1,024 threads per simulation, all but one of them idle, and plan_epoch), from FETCH_SIZE +
+# WRITE_SIZE summed over the epoch's kernels of a profiled run and calibrated against a plain copy kernel (FETCH_SIZE
+# x 2) (profiles/run_batch_fp64_titanic64_batch1024.md).
+TRAFFIC_BYTES_PER_LAUNCH = 13.53e6
 TRAFFIC_SOURCE = "profiles/run_batch_fp64_titanic64_batch1024.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
@@ -486,7 +488,10 @@ def main():
                      # workload, scaled by a copy kernel's reading (a constant here; the run itself is not counted)
                      "traffic": TRAFFIC_BYTES_PER_LAUNCH if (args.count == 1024 and not mixed) else None,
                      "traffic_source": TRAFFIC_SOURCE if (args.count == 1024 and not mixed) else None,
-                     "kernel": "run_batch<false>" if not mixed else "run_batch<true>", "kernel_ms": round(per_launch_ms, 3)},
+                     # a launch is one epoch: run_batch at 256, 512 and 1,024 threads per simulation (all but one
+                     # idle) and plan_epoch, one graph replay; kernel_ms is the launches' HIP-event time over their count
+                     "kernel": f"run_batch<{'true' if mixed else 'false'}, false, 256|512|1024> + plan_epoch, per epoch",
+                     "launches": tot_launch // max(args.steps, 1), "kernel_ms": round(per_launch_ms, 3)},
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sw, sim, args.cpu_sample, args.numerics)

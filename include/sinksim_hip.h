@@ -167,7 +167,8 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
 
 /* No counterpart in the reference: the launch shape. threadsPerSim is the workgroup size of one simulation: 256,
  * 512 (whose second half starts the level solve while the first computes the buoyancy partials), or 0, the default:
- * 256, and 512 for the last epochs of a run once its live instances all fit resident at 512 (see ss_batch_run).
+ * 256, then 512 and 1,024 for the last epochs of a run once its live instances fit resident at those sizes (see
+ * ss_batch_run).
  * workgroups is the number of workgroups of the launch (0, the default: as many as are resident on the device at
  * once, from the kernel's occupancy at this batch's LDS size); fewer than the instance count makes the kernel refill
  * them. The shape changes the time, never the results. SS_LOGIC after run. */
@@ -205,7 +206,8 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
  * of its own, the run is a sequence of epochs (stepsPerLaunch steps each, or by default a 32nd of the longest
  * remaining run, 256 to 4,096 steps): before each epoch the device lists the instances that have not ended, so that
  * they spread evenly over the compute units as others end, and, with the automatic launch shape, runs them at 512
- * threads per simulation once they all fit resident at that size. Otherwise one refilling launch (or one per
+ * threads per simulation once they all fit resident at that size and at 1,024 once they fit one per compute unit.
+ * Otherwise one refilling launch (or one per
  * stepsPerLaunch steps). Results are the same bit for bit whatever the launches. SS_LOGIC when called twice. */
 ss_status ss_batch_run(ss_batch* b, double* seconds);
 

@@ -362,8 +362,14 @@ __device__ __forceinline__ void min_level(T& v) {
 // The node pass's halving trees (the reference's sums and comparisons, operand for operand), with (vol, dv) and
 // (mx, my) packed in pairs, and with Ext (zmin, -zmax) as a pair of minimum trees: -max(-a, -b) is max(a, b)
 // bit for bit, ties and NaN keeping the lane's own value in both. The totals go to every lane of the group.
+// the trees' results, packed: vol in lane 0 and dv in lane 16 of each group (vd), mx / my likewise (xy), mz in lane 0
+// (z), with Ext zmin in lane 0 and -zmax in lane 16 (mm)
+template <class T>
+struct PackedPass {
+  T vd, xy, z, mm;
+};
 template <bool Ext, class T>
-__device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
+__device__ __forceinline__ PackedPass<T> tree32_levels(const PP<T>& p) {
   T vd = pack_sum16(p.vol, p.dv), xy = pack_sum16(p.mx, p.my), z = p.mz;
   sum_level<16>(z);
   T mm = 0;
@@ -376,10 +382,37 @@ __device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
   if constexpr (Ext) min_level<2>(mm);
   sum_level<1>(vd); sum_level<1>(xy); sum_level<1>(z);
   if constexpr (Ext) min_level<1>(mm);
+  return PackedPass<T>{vd, xy, z, mm};
+}
+template <bool Ext, class T>
+__device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
+  const PackedPass<T> q = tree32_levels<Ext>(p);
   PP<T> r;
-  r.vol = lead(vd); r.dv = lead16(vd); r.mx = lead(xy); r.my = lead16(xy); r.mz = lead(z);
+  r.vol = lead(q.vd); r.dv = lead16(q.vd); r.mx = lead(q.xy); r.my = lead16(q.xy); r.mz = lead(q.z);
   r.zmin = 0; r.zmax = 0;
-  if constexpr (Ext) { r.zmin = lead(mm); r.zmax = -lead16(mm); }
+  if constexpr (Ext) { r.zmin = lead(q.mm); r.zmax = -lead16(q.mm); }
+  return r;
+}
+// lane L's value (the same in every lane, in scalar registers)
+template <int L>
+__device__ __forceinline__ float at_lane(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), L));
+}
+template <int L>
+__device__ __forceinline__ double at_lane(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), L));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), L));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+// the group starting at lane L0's totals, read from the packed trees
+template <int L0, bool Ext, class T>
+__device__ __forceinline__ PP<T> group_totals(const PackedPass<T>& q) {
+  PP<T> r;
+  r.vol = at_lane<L0>(q.vd); r.dv = at_lane<L0 + 16>(q.vd); r.mx = at_lane<L0>(q.xy); r.my = at_lane<L0 + 16>(q.xy);
+  r.mz = at_lane<L0>(q.z);
+  r.zmin = 0; r.zmax = 0;
+  if constexpr (Ext) { r.zmin = at_lane<L0>(q.mm); r.zmax = -at_lane<L0 + 16>(q.mm); }
   return r;
 }
 
@@ -579,40 +612,57 @@ struct Pass {
 // hydro.hpp:101-109); the result is in every lane of the group. Both groups of a wave call it together, each with
 // its own request; n < 0 is no request (an empty pass whose result is not used). zmin and zmax are computed only
 // with Ext.
+// A lane's partials of the node pass of node n at level h (n < 0: none), in the numerics' type.
 template <bool Mixed, bool Ext>
-__device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, double h) {
+__device__ __forceinline__ auto pass_partials(PC& P, const Blk& B, int lane, int n, double h) {
   const SSL double* fr = B.frame;
   int s0 = 0, s1 = 0;
   if (n >= 0) {
     s0 = B.nSegStart[n];
     s1 = s0 + B.nSegCount[n];
   }
-  double vol, dv, mx, my, mz, zmin = 0, zmax = 0;
   if constexpr (Mixed) {
     const Fr32 f = fr32(fr);
     const float hf = static_cast<float>(h);
     PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
     stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g, bool on) { segment<Ext>(f, hf, g, on, p); });
-    const PP<float> t = tree32_all<Ext>(p);
-    vol = static_cast<double>(t.vol); dv = static_cast<double>(t.dv);
-    mx = static_cast<double>(t.mx); my = static_cast<double>(t.my); mz = static_cast<double>(t.mz);
-    if constexpr (Ext) { zmin = static_cast<double>(t.zmin); zmax = static_cast<double>(t.zmax); }
+    return p;
   } else {
     const Fr64 F = fr64(fr);
     PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
     stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) { segment<Ext>(F, h, g, on, p); });
-    const PP<double> t = tree32_all<Ext>(p);
-    vol = t.vol; dv = t.dv; mx = t.mx; my = t.my; mz = t.mz;
-    if constexpr (Ext) { zmin = t.zmin; zmax = t.zmax; }
+    return p;
   }
-  const double mu = B.nMu[n >= 0 ? n : 0];
-  const double iR = fr[F_IR];
+}
+// finish_pass (hydro.hpp:101-109) of a node's totals t with its mu and the frame's iR
+template <class T>
+__device__ __forceinline__ Pass finish_pass(const PP<T>& t, double mu, double iR) {
   Pass r;
-  r.vol = vol * mu;
-  r.dv = dv * mu * iR;
-  r.mx = mx * mu; r.my = my * mu; r.mz = mz * mu;
-  r.zmin = zmin; r.zmax = zmax;
+  r.vol = static_cast<double>(t.vol) * mu;
+  r.dv = static_cast<double>(t.dv) * mu * iR;
+  r.mx = static_cast<double>(t.mx) * mu; r.my = static_cast<double>(t.my) * mu; r.mz = static_cast<double>(t.mz) * mu;
+  r.zmin = static_cast<double>(t.zmin); r.zmax = static_cast<double>(t.zmax);
   return r;
+}
+
+// The node pass of node n at level h by one logical 32-lane group (hydro.hpp:112-122 / 144-155; finish_pass
+// hydro.hpp:101-109); the result is in every lane of the group. Both groups of a wave call it together, each with
+// its own request; n < 0 is no request (an empty pass whose result is not used). zmin and zmax are computed only
+// with Ext.
+template <bool Mixed, bool Ext>
+__device__ __forceinline__ Pass node_pass(PC& P, const Blk& B, int lane, int n, double h) {
+  const auto t = tree32_all<Ext>(pass_partials<Mixed, Ext>(P, B, lane, n, h));
+  return finish_pass(t, B.nMu[n >= 0 ? n : 0], B.frame[F_IR]);
+}
+// The passes of a wave's two groups at one level, group 0 on node n0 and group 1 on node n1 (each group's lane
+// gives its own node, n), their totals read from the trees' lanes into scalar registers (no broadcast): both results
+// in every lane.
+template <bool Mixed, bool Ext>
+__device__ __forceinline__ void node_pass_pair(PC& P, const Blk& B, int lane, int n, double h, int n0, int n1, Pass& p0, Pass& p1) {
+  const auto q = tree32_levels<Ext>(pass_partials<Mixed, Ext>(P, B, lane, n, h));
+  const double iR = B.frame[F_IR];
+  p0 = finish_pass(group_totals<0, Ext>(q), B.nMu[n0], iR);
+  p1 = finish_pass(group_totals<32, Ext>(q), B.nMu[n1], iR);
 }
 
 // ------------------------------------------------------------------------------------------- (b) levels
@@ -661,30 +711,14 @@ __device__ __forceinline__ void solve_round(int& round) {
   if (round == 5) __builtin_amdgcn_s_setprio(2);
 }
 
-// lane L's value of v, the same in every lane (held in scalar registers)
-template <int L>
-__device__ __forceinline__ double from_lane(double v) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b)), L));
-  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(b >> 32)), L));
-  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-template <int L>
-__device__ __forceinline__ Pass pass_from_lane(const Pass& p) {
-  return Pass{from_lane<L>(p.vol), from_lane<L>(p.dv), from_lane<L>(p.mx), from_lane<L>(p.my), from_lane<L>(p.mz), from_lane<L>(p.zmin),
-              from_lane<L>(p.zmax)};
-}
-
 // The passes of the two members of a merged pair at level h in one round, member k on half k: both results, the same in
 // every lane (with Ext also each member's extent into zmin / zmax).
 template <bool Mixed, bool Ext>
 __device__ __forceinline__ void pair_passes(PC& P, const Blk& B, int base, double h, int lane, int half, int& round, Pass& p0, Pass& p1) {
-  const int n = B.gNodes[base + half];
-  const Pass p = node_pass<Mixed, Ext>(P, B, lane, n, h);
+  const int n0 = B.gNodes[base], n1 = B.gNodes[base + 1];
+  node_pass_pair<Mixed, Ext>(P, B, lane, half ? n1 : n0, h, n0, n1, p0, p1);
   if constexpr (Ext)
-    if (lane == 0) { B.zmin[n] = p.zmin; B.zmax[n] = p.zmax; }
-  p0 = pass_from_lane<0>(p);
-  p1 = pass_from_lane<32>(p);
+    if ((lane | half) == 0) { B.zmin[n0] = p0.zmin; B.zmax[n0] = p0.zmax; B.zmin[n1] = p1.zmin; B.zmax[n1] = p1.zmax; }
   solve_round(round);
 }
 

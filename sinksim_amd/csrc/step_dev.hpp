@@ -361,9 +361,9 @@ __device__ __forceinline__ void min_level(T& v) {
 
 // The node pass's halving trees (the reference's sums and comparisons, operand for operand), with (vol, dv) and
 // (mx, my) packed in pairs, and with Ext (zmin, -zmax) as a pair of minimum trees: -max(-a, -b) is max(a, b)
-// bit for bit, ties and NaN keeping the lane's own value in both. The totals go to every lane of the group.
-// the trees' results, packed: vol in lane 0 and dv in lane 16 of each group (vd), mx / my likewise (xy), mz in lane 0
-// (z), with Ext zmin in lane 0 and -zmax in lane 16 (mm)
+// bit for bit, ties and NaN keeping the lane's own value in both. The results stay packed: vol in lane 0 and dv in
+// lane 16 of each group (vd), mx and my likewise (xy), mz in lane 0 (z), with Ext zmin in lane 0 and -zmax in lane 16
+// (mm).
 template <class T>
 struct PackedPass {
   T vd, xy, z, mm;
@@ -384,6 +384,7 @@ __device__ __forceinline__ PackedPass<T> tree32_levels(const PP<T>& p) {
   if constexpr (Ext) min_level<1>(mm);
   return PackedPass<T>{vd, xy, z, mm};
 }
+// the trees' totals in every lane of the group
 template <bool Ext, class T>
 __device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
   const PackedPass<T> q = tree32_levels<Ext>(p);
@@ -608,10 +609,6 @@ struct Pass {
   double vol, dv, mx, my, mz, zmin, zmax;
 };
 
-// The node pass of node n at level h by one logical 32-lane group (hydro.hpp:112-122 / 144-155; finish_pass
-// hydro.hpp:101-109); the result is in every lane of the group. Both groups of a wave call it together, each with
-// its own request; n < 0 is no request (an empty pass whose result is not used). zmin and zmax are computed only
-// with Ext.
 // A lane's partials of the node pass of node n at level h (n < 0: none), in the numerics' type.
 template <bool Mixed, bool Ext>
 __device__ __forceinline__ auto pass_partials(PC& P, const Blk& B, int lane, int n, double h) {

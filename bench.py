@@ -165,6 +165,7 @@ def bench_uneven(args):
     def one_run():
         b = Batch.fleet(models, inst_model, opts)
         b.set_launch(tune["threads_per_sim"], tune["workgroups"])
+        b.set_record_layout(tune["record_bytes"])
         secs = b.run()
         steps = b.results()["steps"]
         kms, launches = b.kernel_time()
@@ -196,7 +197,8 @@ def bench_uneven(args):
                                f"numerics, to founder or {t_max / 3600:g} h, curves every 30 s, refilled workgroups",
                    "instances": count, "numerics": args.numerics, "sim_steps_per_bench_step": int(tot_steps // args.steps),
                    "threads_per_sim": tune["threads_per_sim"], "workgroups": tune["workgroups"] or "one per instance or all resident",
-                   "autotune_trials": [[t["threads_per_sim"], t["workgroups"], round(t["rate"])] for t in tune["trials"]],
+                   "record_bytes": tune["record_bytes"],
+                   "autotune_trials": [[t["threads_per_sim"], t["workgroups"], t["record_bytes"], round(t["rate"])] for t in tune["trials"]],
                    "device": _device()},
         # traffic: FETCH_SIZE + WRITE_SIZE of the timed launch on this workload at 4,096 hulls (FETCH_SIZE unscaled:
         # profiles/run_batch_fp64_uneven4096.md); a constant here, the run itself is not counted

@@ -174,6 +174,14 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
  * them. The shape changes the time, never the results. SS_LOGIC after run. */
 ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgroups);
 
+/* No counterpart in the reference: the device layout of the hull's double geometry records (portable numerics).
+ * recordBytes 48 (the default) pads each record's five doubles to 48 bytes, 16-byte aligned; 40 packs them, 8-byte
+ * aligned. The padded records are read faster while a hull stays in the caches (one hull shared by the batch), the
+ * packed ones carry a sixth fewer bytes where hulls stream from memory (fleets of large hulls); ss_batch_autotune
+ * measures both. The layout changes the time, never the results. ss_batch_step_noalloc always reads the padded
+ * records. SS_INVALID_ARGUMENT for another size; SS_LOGIC after run or step. */
+ss_status ss_batch_set_record_layout(ss_batch* b, int32_t recordBytes);
+
 /* The oracle's stopWhenStable rule on the device (RunOptions::stopWhenStable, engine/include/sinksim/simulation.hpp:34;
  * engine/src/simulation.cpp:179-182): past 30 minutes an instance accumulates calm time (dt per step) while the
  * step's sea inflow stays under 0.02, the pitch rate under 1e-5 and the heave velocity under 1e-3 in magnitude,
@@ -192,13 +200,16 @@ typedef struct ss_tune_result {
   int32_t trials;
   int32_t trialThreads[8], trialWorkgroups[8];
   double trialRate[8];
+  int32_t recordBytes;          /* the double geometry records' layout chosen (48 or 40; see ss_batch_set_record_layout) */
+  int32_t trialRecordBytes[8];  /* each trial's record layout */
 } ss_tune_result;
 
 /* No counterpart in the reference: the autotuner. Runs the first `sample` instances of the work order (the largest
  * models) for trialSteps steps at each launch shape on a trial copy (this batch's staged data is not touched), and
  * sets the fastest shape on the batch. The shapes: 256 and 512 threads per simulation, each with the default grid and,
  * where the sample holds more instances than are resident at once (the refilling kernel), with three quarters and
- * half of the resident workgroups. SS_LOGIC after run. */
+ * half of the resident workgroups; then, for a portable (FP64) batch, the other layout of the double geometry records
+ * at the fastest shape, kept when faster. SS_LOGIC after run. */
 ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the

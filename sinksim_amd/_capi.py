@@ -61,7 +61,8 @@ class SsInstanceResult(C.Structure):
 
 class SsTuneResult(C.Structure):
     _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 2), ("trials", C.c_int32),
-                ("trialThreads", C.c_int32 * 8), ("trialWorkgroups", C.c_int32 * 8), ("trialRate", C.c_double * 8)]
+                ("trialThreads", C.c_int32 * 8), ("trialWorkgroups", C.c_int32 * 8), ("trialRate", C.c_double * 8),
+                ("recordBytes", C.c_int32), ("trialRecordBytes", C.c_int32 * 8)]
 
 
 class SsEvent(C.Structure):
@@ -98,6 +99,7 @@ SIGNATURES = {
     "ss_batch_set_connections": (C.c_int, [C.c_void_p, _bp, _dp, _dp]),
     "ss_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
     "ss_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.POINTER(SsTuneResult)]),
+    "ss_batch_set_record_layout": (C.c_int, [C.c_void_p, C.c_int32]),
     "ss_batch_run": (C.c_int, [C.c_void_p, _dp]),
     "ss_batch_step": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64), _dp]),
     "ss_batch_state_records": (C.c_int, [C.c_void_p, _dp]),

@@ -167,14 +167,20 @@ class Batch:
     def set_workgroups(self, n: int) -> None:
         self.set_launch(0, n)
 
+    def set_record_layout(self, record_bytes: int) -> None:
+        """The double geometry records' device layout: 48 bytes (padded, the default) or 40 (packed); the time
+        changes, the results do not (ss_batch_set_record_layout)."""
+        A.check(self._lib.ss_batch_set_record_layout(self._h, int(record_bytes)))
+
     def autotune(self, sample: int = 1024, trial_steps: int = 400) -> dict:
         """Measures each launch shape on the first `sample` instances of the work order for trial_steps steps (on a
         trial copy) and sets the fastest; returns the choice and the measured rates (steps/s)."""
         r = A.SsTuneResult()
         A.check(self._lib.ss_batch_autotune(self._h, int(sample), int(trial_steps), C.byref(r)))
-        trials = [{"threads_per_sim": r.trialThreads[k], "workgroups": r.trialWorkgroups[k], "rate": r.trialRate[k]} for k in range(r.trials)]
-        return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "rate_256": r.rate[0], "rate_512": r.rate[1],
-                "trials": trials}
+        trials = [{"threads_per_sim": r.trialThreads[k], "workgroups": r.trialWorkgroups[k], "record_bytes": r.trialRecordBytes[k],
+                   "rate": r.trialRate[k]} for k in range(r.trials)]
+        return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "record_bytes": r.recordBytes, "rate_256": r.rate[0],
+                "rate_512": r.rate[1], "trials": trials}
 
     # ------------------------------------------------------------------ run and results
 

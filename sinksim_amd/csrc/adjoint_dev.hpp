@@ -186,7 +186,7 @@ __device__ inline PassRec pass_rec(PC& P, const Blk& B, const SSL double* fr, in
 #pragma unroll
   for (int k = 0; k < 10; ++k) p.S[k] = 0;
   const int s0 = B.nSegStart[n], s1 = s0 + B.nSegCount[n];
-  stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) {
+  stream<double>(P.S.seg, rec_bytes<double>(P), s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) {
     const double x = g.x, y = g.y, a = g.lo, b = g.hi;
     const double base = R20 * x + R21 * y + tz;
     const double wa = base + R22 * a, wb = base + R22 * b;
@@ -581,7 +581,7 @@ __device__ inline void buoyancy_rec(PC& P, const TapeT<GT>& T, int tid) {
   HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
   double S[10] = {};
   double tx = 0, ty = 0, th = 0;
-  stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) {
+  stream<double>(P.S.col, rec_bytes<double>(P), tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) {
     const double x = g.x, y = g.y, lo = g.lo, hi = g.hi;
     const double base = F.R20 * x + F.R21 * y + F.tz;
     const double wt = base + F.R22 * hi;

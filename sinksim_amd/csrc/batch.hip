@@ -254,6 +254,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // in a graph and replayed).
 __global__ void reset_counter(int32_t* counter) { *counter = 0; }
 
+// The packed double records: record i's five values from its 48-byte slot (6 doubles) to its 40-byte one.
+__global__ void pack_records(const double* __restrict__ src, double* __restrict__ dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    for (int k = 0; k < 5; ++k) dst[5 * i + k] = src[6 * i + k];
+}
+
 // Plans an epoch of ss_batch_run: the instances of the work order that have not ended by the step loop's own test
 // (foundered, t at or past tMax, or stopped by the calm rule), in work order, and their count; every instance for
 // the first epoch (everyInstance), so that each is loaded and stored at least once, as by a single launch. The epoch
@@ -627,6 +633,20 @@ ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double*
 // initial states), clears every result array, and uploads the models at the batch's workgroup size and the batch
 // view. The products are formed here exactly as the kernel formed them (one rounded multiplication), so the flows
 // are unchanged bit for bit.
+void Model::pack40() const {
+  auto pack = [](const Dev<double>& from, Dev<double>& to) {
+    if (to.p || !from.p) return;
+    const size_t n = from.n / 6;
+    to.alloc(n * 5);
+    hipLaunchKernelGGL(pack_records, dim3(static_cast<unsigned>(std::min<size_t>((n + 255) / 256, 4096))), dim3(256), 0, nullptr,
+                       from.p, to.p, static_cast<int64_t>(n));
+    hip_check(hipGetLastError(), "pack_records launch");
+  };
+  pack(col, col40);
+  pack(seg, seg40);
+  hip_check(hipDeviceSynchronize(), "pack_records");
+}
+
 void ssa::upload(ss_batch* b) {
   const size_t count = static_cast<size_t>(b->opt.count);
   b->ldsBytes = 0;
@@ -658,7 +678,10 @@ void ssa::upload(ss_batch* b) {
   b->dMonOff.upload(b->monOff); b->dMarkOff.upload(b->markOff);
   b->counter.fill(0, 1);
   std::vector<ModelParams> mp;
-  for (const auto& m : b->models) mp.push_back(m->params(b->tpb));
+  for (const auto& m : b->models) {
+    if (b->recBytes == 40) m->pack40();
+    mp.push_back(m->params(b->tpb, b->recBytes));
+  }
   b->dModels.upload(mp);
   const DevBatch bv = b->batchView();
   b->dBatch.upload(&bv, 1);
@@ -758,7 +781,7 @@ void run_epochs(ss_batch* b, const LaunchPlan& lp, hipStream_t stream, hipEvent_
       forms[f].fit = perCU * cus;
       forms[f].lds = lds;
       std::vector<ModelParams> mp;
-      for (const auto& m : b->models) mp.push_back(m->params(sizes[f]));
+      for (const auto& m : b->models) mp.push_back(m->params(sizes[f], b->recBytes));
       params[f]->upload(mp);
     }
   }
@@ -958,6 +981,15 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
   });
 }
 
+ss_status ss_batch_set_record_layout(ss_batch* b, int32_t recordBytes) {
+  return guarded([&] {
+    need(b != nullptr, "ss_batch_set_record_layout: null batch");
+    if (b->ran || b->stepped) fail(SS_LOGIC, "ss_batch_set_record_layout: set after run or step");
+    need(recordBytes == 48 || recordBytes == 40, "ss_batch_set_record_layout: recordBytes must be 48 or 40");
+    b->recBytes = recordBytes;
+  });
+}
+
 ss_status ss_batch_set_stop_when_stable(ss_batch* b, int32_t on) {
   return guarded([&] {
     need(b != nullptr, "ss_batch_set_stop_when_stable: null batch");
@@ -984,10 +1016,39 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
     o.stepsPerLaunch = trialSteps;
     const int tpbs[2] = {kThreads, 512};
     double best = -1;
-    int32_t bestTpb = kThreads, bestBlocks = 0;
+    int32_t bestTpb = kThreads, bestBlocks = 0, bestRec = b->recBytes;
     ss_tune_result res{};
     int cus = 0;
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0), "hipDeviceGetAttribute");
+    // one launch of trialSteps steps for every sampled instance at a shape and record layout: steps/s
+    auto trial = [&](int tpb, int32_t grid, int32_t rec) {
+      std::unique_ptr<ss_batch> t = make_batch(o, b->models, im);
+      for (int32_t k = 0; k < n; ++k) {  // copy the staged tables and states of the sampled instances
+        const size_t src = static_cast<size_t>(b->order[static_cast<size_t>(k)]), dst = static_cast<size_t>(k);
+        const Model& m = b->model(static_cast<int32_t>(src));
+        std::copy_n(b->hEn.begin() + b->connOff[src], m.NC, t->hEn.begin() + t->connOff[dst]);
+        std::copy_n(b->hArea.begin() + b->connOff[src], m.NC, t->hArea.begin() + t->connOff[dst]);
+        std::copy_n(b->hTOn.begin() + b->connOff[src], m.NC, t->hTOn.begin() + t->connOff[dst]);
+        std::copy_n(b->hSt.begin() + 7 * src, 7, t->hSt.begin() + 7 * dst);
+        for (auto [from, to] : {std::make_pair(&b->hVol, &t->hVol), std::make_pair(&b->hLevel, &t->hLevel), std::make_pair(&b->hCx, &t->hCx),
+                                std::make_pair(&b->hCy, &t->hCy), std::make_pair(&b->hCz, &t->hCz)})
+          std::copy_n(from->begin() + b->nodeOff[src], m.NN, to->begin() + t->nodeOff[dst]);
+      }
+      t->tpb = tpb;
+      t->blocks = grid;
+      t->recBytes = rec;
+      double secs = 0;
+      execute(t.get(), &secs, false, 1);
+      const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
+      if (res.trials < 8) {
+        res.trialThreads[res.trials] = tpb;
+        res.trialWorkgroups[res.trials] = grid;
+        res.trialRate[res.trials] = rate;
+        res.trialRecordBytes[res.trials] = rec;
+        ++res.trials;
+      }
+      return rate;
+    };
     for (int c = 0; c < 2; ++c) {
       size_t lds = 0;
       for (const auto& m : b->models) lds = std::max(lds, m->lds_bytes(tpbs[c]));
@@ -1006,40 +1067,24 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
         for (int num : {3, 2})
           if (const int per = perCU * num / 4; per >= 1 && per < perCU && per * cus != grids.back()) grids.push_back(per * cus);
       for (int32_t grid : grids) {
-        std::unique_ptr<ss_batch> t = make_batch(o, b->models, im);
-        for (int32_t k = 0; k < n; ++k) {  // copy the staged tables and states of the sampled instances
-          const size_t src = static_cast<size_t>(b->order[static_cast<size_t>(k)]), dst = static_cast<size_t>(k);
-          const Model& m = b->model(static_cast<int32_t>(src));
-          std::copy_n(b->hEn.begin() + b->connOff[src], m.NC, t->hEn.begin() + t->connOff[dst]);
-          std::copy_n(b->hArea.begin() + b->connOff[src], m.NC, t->hArea.begin() + t->connOff[dst]);
-          std::copy_n(b->hTOn.begin() + b->connOff[src], m.NC, t->hTOn.begin() + t->connOff[dst]);
-          std::copy_n(b->hSt.begin() + 7 * src, 7, t->hSt.begin() + 7 * dst);
-          for (auto [from, to] : {std::make_pair(&b->hVol, &t->hVol), std::make_pair(&b->hLevel, &t->hLevel), std::make_pair(&b->hCx, &t->hCx),
-                                  std::make_pair(&b->hCy, &t->hCy), std::make_pair(&b->hCz, &t->hCz)})
-            std::copy_n(from->begin() + b->nodeOff[src], m.NN, to->begin() + t->nodeOff[dst]);
-        }
-        t->tpb = tpbs[c];
-        t->blocks = grid;
-        // one launch of trialSteps steps for every sampled instance
-        double secs = 0;
-        execute(t.get(), &secs, false, 1);
-        const double rate = t->kernelMs > 0 ? static_cast<double>(t->totalSteps) / (t->kernelMs * 1e-3) : 0;
-        if (res.trials < 8) {
-          res.trialThreads[res.trials] = tpbs[c];
-          res.trialWorkgroups[res.trials] = grid;
-          res.trialRate[res.trials] = rate;
-          ++res.trials;
-        }
+        const double rate = trial(tpbs[c], grid, b->recBytes);
         res.rate[c] = std::max(res.rate[c], rate);
         if (rate > best) { best = rate; bestTpb = tpbs[c]; bestBlocks = grid; }
       }
     }
+    // the other layout of the double records at the fastest shape (the float records have one layout)
+    if (!b->mixed) {
+      const int32_t other = b->recBytes == 48 ? 40 : 48;
+      if (const double rate = trial(bestTpb, bestBlocks, other); rate > best) { best = rate; bestRec = other; }
+    }
     b->tpb = bestTpb;
     b->tailForm = true;
     b->blocks = bestBlocks;
+    b->recBytes = bestRec;
     b->ldsBytes = forward_lds_bytes(*b);
     res.threadsPerSim = bestTpb;
     res.workgroups = bestBlocks;
+    res.recordBytes = bestRec;
     if (out) *out = res;
   });
 }
@@ -1365,7 +1410,7 @@ ss_status ss_batch_step_noalloc(const ss_batch* b, int32_t count, int64_t steps,
     const hipStream_t s = static_cast<hipStream_t>(stream);
 
     // the views the kernel reads, pointing into the scratch array and at the caller's tables
-    ModelParams mp = m.params(b->tpb);
+    ModelParams mp = m.params(b->tpb, 48);  // the padded records, which every model has (the packed ones would be allocated)
     DevBatch bv{};
     bv.count = count;
     bv.instModel = G(reinterpret_cast<const int32_t*>(base + plan.instModel));

@@ -142,7 +142,9 @@ struct Model {
   int32_t NN = 0, NC = 0, NG = 0, nmon = 0, nmark = 0, evCap = 0, maxGroup = 1, ncol = 0, nseg = 0, members = 0, nsea = 0;
   double cost = 0;  // work per step, relative: the order in which a batch hands its instances out
   // ship
-  Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record
+  Dev<double> col, seg, mu, vmax, amin, aov;  // col, seg: 6 doubles per record (48 bytes)
+  mutable Dev<double> col40, seg40;           // the same records packed in 5 doubles (40 bytes), made on first use
+  void pack40() const;                        // makes col40 / seg40 from col / seg on the device
   Dev<float> colF, segF;                      // 8 floats per record
   Dev<int32_t> segStart, segCount;
   // scenario
@@ -161,12 +163,14 @@ struct Model {
   LdsLayout layout(int tpb) const { return lds_layout(NN, NC, NG, nmon, nmark, maxGroup, evCap, members, tpb / kLanes, nsea); }
   size_t lds_bytes(int tpb) const { return static_cast<size_t>(layout(tpb).total) * sizeof(double); }
 
-  ModelParams params(int tpb) const {
+  // the model's views at a workgroup size and a double-record layout (48 or 40 bytes; 40 needs pack40 first)
+  ModelParams params(int tpb, int recBytes) const {
     ModelParams m{};
     DevShip& s = m.S;
     s.ncol = ncol; s.nseg = nseg; s.nn = NN;
-    s.col = (const SSG d2v*)col.p; s.colF = (const SSG f4v*)colF.p;
-    s.seg = (const SSG d2v*)seg.p; s.segF = (const SSG f4v*)segF.p;
+    s.recD = recBytes;
+    s.col = (const SSG d2v*)(recBytes == 40 ? col40.p : col.p); s.colF = (const SSG f4v*)colF.p;
+    s.seg = (const SSG d2v*)(recBytes == 40 ? seg40.p : seg.p); s.segF = (const SSG f4v*)segF.p;
     s.segStart = G(segStart.p); s.segCount = G(segCount.p);
     s.mu = G(mu.p); s.vmax = G(vmax.p); s.amin = G(amin.p); s.aov = G(aov.p);
     m.M = net;
@@ -185,6 +189,7 @@ struct ss_batch {
   size_t ldsBytes = 0;  // the largest model's, at the launch's workgroup size
   int32_t recSize = 0, maxSamples = 0, curveMax = 0;
   int32_t tpb = kThreads;  // threads per simulation (workgroup size): 256 or 512
+  int32_t recBytes = 48;   // the double geometry records' layout: 48 bytes (padded) or 40 (packed)
   int32_t blocks = 0;   // workgroups of the persistent launch (0: as many as fit on the device at once)
   bool tailForm = true;  // the automatic shape: a run's last epochs at 512 threads once the live instances fit
   bool refill = false;  // whether the last run used the refilling (persistent) kernel

@@ -77,10 +77,11 @@ using gcb = const SSG uint8_t;
 
 // The hull, shared by every simulation, as records: per column (x, y, zlo, zhi, dxdy) and per segment the same
 // five values with the segment's own bottom and top in place of the column's (x, y and footprint pre-gathered from
-// the segment's column), as 48-byte double records and 32-byte float records. The float records hold the same
-// values rounded once on the host, which is what the portable32 arithmetic converts them to anyway.
+// the segment's column), as 48-byte (or packed 40-byte) double records and 32-byte float records. The float records
+// hold the same values rounded once on the host, which is what the portable32 arithmetic converts them to anyway.
 struct DevShip {
   int32_t ncol, nseg, nn;
+  int32_t recD;  // bytes per double record: 48 (16-byte aligned) or 40 (packed); the float records are 32 bytes
   const SSG d2v* col;
   const SSG f4v* colF;
   const SSG d2v* seg;
@@ -443,7 +444,7 @@ __device__ __forceinline__ Fr32 fr32(const SSL double* fr) {
 }
 
 // A column's or a segment's geometry: x, y, bottom, top and footprint (for a segment: those of its column, with
-// the segment's own bottom and top). Records are 48 bytes (double) or 32 bytes (float), addressed by a 32-bit
+// the segment's own bottom and top). Records are 48 or 40 bytes (double) or 32 bytes (float), addressed by a 32-bit
 // byte offset from the table's base.
 //
 // Loads and their registers: a load writes every register of its destination when it lands, used or not, so a
@@ -465,17 +466,20 @@ struct Geo {
 };
 __device__ __forceinline__ void hold(const Geo<double>&) {}
 __device__ __forceinline__ void hold(const Geo<float>& g) { asm volatile("" ::"v"(g.pad.p1), "v"(g.pad.p2), "v"(g.pad.p3)); }
-// record sizes: five doubles padded to 48 bytes, five floats padded to 32
+// record sizes: five doubles padded to 48 bytes or packed in 40 (a batch's choice: the padded records are read
+// faster when a hull stays in the caches, the packed ones when hulls stream from memory), five floats padded to 32
 template <class T>
-constexpr uint32_t kRecBytes = sizeof(T) == sizeof(double) ? 48u : 32u;
+__device__ __forceinline__ uint32_t rec_bytes(PC& P) { return sizeof(T) == sizeof(double) ? static_cast<uint32_t>(P.S.recD) : 32u; }
 // s_waitcnt immediates (gfx9 encoding: vmcnt in bits 3:0, expcnt 6:4 and lgkmcnt 11:8 at their maxima, i.e. not
 // waited for) that wait until at most one record's loads are in flight: three loads per double record, two per
 // float record
 template <class T>
 constexpr int kWaitOneRecord = sizeof(T) == sizeof(double) ? 0x0F73 : 0x0F72;
+// (a packed double record is 8-byte aligned: its pairs are loaded as 16-byte loads of 8-byte alignment)
+typedef double d2v8 __attribute__((ext_vector_type(2), aligned(8)));
 __device__ __forceinline__ Geo<double> geo_at(const SSG d2v* table, uint32_t off) {
-  const SSG d2v* r = (const SSG d2v*)((const SSG char*)table + off);
-  const d2v a = r[0], b = r[1];
+  const SSG d2v8* r = (const SSG d2v8*)((const SSG char*)table + off);
+  const d2v8 a = r[0], b = r[1];
   const double A = *(const SSG double*)(r + 2);
   return Geo<double>{a.x, a.y, b.x, b.y, A, {}};
 }
@@ -484,8 +488,6 @@ __device__ __forceinline__ Geo<float> geo_at(const SSG f4v* table, uint32_t off)
   const f4v a = r[0], b = r[1];
   return Geo<float>{a.x, a.y, a.z, a.w, b.x, {b.y, b.z, b.w}};
 }
-__device__ __forceinline__ Geo<double> geo(const SSG d2v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<double>); }
-__device__ __forceinline__ Geo<float> geo(const SSG f4v* table, int i) { return geo_at(table, static_cast<uint32_t>(i) * kRecBytes<float>); }
 
 // one column into a buoyancy partial (reduce.hpp:39-49 / 131-142)
 __device__ __forceinline__ void column(const Fr64& F, const Geo<double>& g, bool on, HP<double>& p) {
@@ -574,7 +576,7 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
 // The scheduling barriers keep the next record's loads ahead of the current record's arithmetic: without them
 // the compiler sinks the loads below the evaluation and waits for them at once (vmcnt(0)).
 template <class T, class Tab, class Fn>
-__device__ __forceinline__ void stream(const Tab* table, int first, int end, int step, Fn&& eval) {
+__device__ __forceinline__ void stream(const Tab* table, uint32_t rec, int first, int end, int step, Fn&& eval) {
   const int mine = first < end ? (end - first + step - 1) / step : 0;  // this lane's records
   const uint64_t ex = __builtin_amdgcn_read_exec();
   const uint32_t ex0 = static_cast<uint32_t>(ex), ex1 = static_cast<uint32_t>(ex >> 32);
@@ -582,7 +584,6 @@ __device__ __forceinline__ void stream(const Tab* table, int first, int end, int
   if (ex0) rounds = __builtin_amdgcn_readlane(mine, __builtin_ctz(ex0));
   if (ex1) rounds = max(rounds, __builtin_amdgcn_readlane(mine, 32 + __builtin_ctz(ex1)));
   if (rounds == 0) return;
-  constexpr uint32_t rec = kRecBytes<T>;
   const uint32_t stride = static_cast<uint32_t>(step) * rec;
   const uint32_t last = mine > 0 ? static_cast<uint32_t>(first + (mine - 1) * step) * rec : 0u;
   uint32_t off = mine > 0 ? static_cast<uint32_t>(first) * rec : 0u;
@@ -622,12 +623,12 @@ __device__ __forceinline__ auto pass_partials(PC& P, const Blk& B, int lane, int
     const Fr32 f = fr32(fr);
     const float hf = static_cast<float>(h);
     PP<float> p{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 1e9f, -1e9f};
-    stream<float>(P.S.segF, s0 + lane, s1, kLanes, [&](const Geo<float>& g, bool on) { segment<Ext>(f, hf, g, on, p); });
+    stream<float>(P.S.segF, rec_bytes<float>(P), s0 + lane, s1, kLanes, [&](const Geo<float>& g, bool on) { segment<Ext>(f, hf, g, on, p); });
     return p;
   } else {
     const Fr64 F = fr64(fr);
     PP<double> p{0.0, 0.0, 0.0, 0.0, 0.0, 1e9, -1e9};
-    stream<double>(P.S.seg, s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) { segment<Ext>(F, h, g, on, p); });
+    stream<double>(P.S.seg, rec_bytes<double>(P), s0 + lane, s1, kLanes, [&](const Geo<double>& g, bool on) { segment<Ext>(F, h, g, on, p); });
     return p;
   }
 }
@@ -1225,7 +1226,7 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   if constexpr (Mixed) {
     const Fr32 f = fr32(B.frame);
     HP<float> p{0.0f, 0.0f, 0.0f, 0.0f, -1e9f};
-    stream<float>(P.S.colF, tid, P.S.ncol, kThreads, [&](const Geo<float>& g, bool on) { column(f, g, on, p); });
+    stream<float>(P.S.colF, rec_bytes<float>(P), tid, P.S.ncol, kThreads, [&](const Geo<float>& g, bool on) { column(f, g, on, p); });
     float vx, yz, top;
     tree32_packed(p, vx, yz, top);
     if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }
@@ -1233,7 +1234,7 @@ __device__ __forceinline__ void buoyancy_partials(PC& P, const Blk& B, int tid) 
   } else {
     const Fr64 F = fr64(B.frame);
     HP<double> p{0.0, 0.0, 0.0, 0.0, -1e9};
-    stream<double>(P.S.col, tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) { column(F, g, on, p); });
+    stream<double>(P.S.col, rec_bytes<double>(P), tid, P.S.ncol, kThreads, [&](const Geo<double>& g, bool on) { column(F, g, on, p); });
     double vx, yz, top;
     tree32_packed(p, vx, yz, top);
     if (lane == 0) { w[0] = vx; w[2] = yz; w[4] = top; }

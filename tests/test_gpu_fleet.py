@@ -8,6 +8,8 @@ handed out by a work counter and refilled as simulations end.
   equals the oracle instance for instance over 1,200 s, in both numerics (one of those hulls is pinned against the
   reference's own CPU engine in tests/test_synthetic_reference.py).
 - A fleet of one model equals the plain batch of that model.
+- The packed 40-byte double records give the same runs as the padded 48-byte ones (the oracle's, for a fleet), and
+  the autotuner measures both layouts.
 """
 import zlib
 
@@ -35,7 +37,7 @@ def _tables(models, inst_model, seed):
     return en, area, tOn
 
 
-def _check_fleet(models, inst_model, numerics, t_max, every, workgroups, tpb=0):
+def _check_fleet(models, inst_model, numerics, t_max, every, workgroups, tpb=0, record_bytes=48):
     import pyoracle
     from sinksim_amd.batch import Batch, BatchOptions
     en, area, tOn = _tables(models, inst_model, seed=zlib.crc32(numerics.encode()))
@@ -43,6 +45,7 @@ def _check_fleet(models, inst_model, numerics, t_max, every, workgroups, tpb=0):
                                                      traced_instances=0, curve_every=every, t_max=t_max))
     b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
     b.set_launch(tpb, workgroups)
+    b.set_record_layout(record_bytes)
     b.run()
     for i, m in enumerate(inst_model):
         ship, sim = models[m]
@@ -165,6 +168,8 @@ def test_autotune_picks_a_shape_and_keeps_results():
         if tune:
             r = b.autotune(sample=8, trial_steps=200)
             assert r["threads_per_sim"] in (256, 512) and r["rate_256"] > 0 and r["rate_512"] > 0
+            # the double records' two layouts are measured (the last trial: the other layout at the fastest shape)
+            assert r["record_bytes"] in (48, 40) and {t["record_bytes"] for t in r["trials"]} == {48, 40}, r
         b.run()
         out.append([b.curve(i) for i in range(len(inst_model))])
         b.close()
@@ -194,4 +199,49 @@ def test_autotune_grid_choice_keeps_results():
     (r0, c0), (r1, c1) = out
     assert np.array_equal(r0["steps"], r1["steps"])
     for a, c in zip(c0, c1):
+        assert np.array_equal(a, c)
+
+
+def test_packed_records_equal_the_oracle():
+    """The packed 40-byte double geometry records (ss_batch_set_record_layout) give the oracle's runs bit for bit, as
+    the padded 48-byte ones do: a fleet through the refilling kernel (three workgroups for ten instances) and at the
+    default grid, portable numerics (the float records have one layout)."""
+    models = _models()
+    inst_model = np.array([2, 1, 0, 3, 4, 2, 0, 1, 3, 4], np.int32)
+    _check_fleet(models, inst_model, "portable", 900.0, 60.0, workgroups=3, record_bytes=40)
+    _check_fleet(models, inst_model, "portable", 900.0, 60.0, workgroups=0, record_bytes=40)
+
+
+def test_packed_records_keep_a_titanic_batch_in_epochs():
+    """A titanic64 batch in epochs (every instance resident, the 256-, 512- and 1,024-thread forms) with the packed
+    records equals the padded records' run: steps, events, curves, and a traced instance's dense and sampled records;
+    other sizes are refused, and the layout cannot change after a run."""
+    from sinksim_amd import sweep as S
+    from sinksim_amd._capi import EngineError
+    from sinksim_amd.batch import Batch, BatchOptions
+    from sinksim_amd.data_paths import titanic
+    from sinksim_amd.workloads import titanic_batch_sweep
+    ship, sim = titanic("titanic")
+    sw = titanic_batch_sweep(sim, count=300, t_max=3 * 3600.0)
+    en, area, tOn = S.connection_tables(sw, sim)
+    out = []
+    for rec in (48, 40):
+        b = Batch(ship, sim, BatchOptions(count=300, mixed_precision=False, traced_instances=1, dense_steps=50, sample_every=300.0,
+                                          curve_every=sw.every, t_max=sw.tMax))
+        b.set_connections(en, area, tOn)
+        with pytest.raises(ValueError):  # SS_INVALID_ARGUMENT
+            b.set_record_layout(44)
+        b.set_record_layout(rec)
+        b.run()
+        with pytest.raises(EngineError):  # SS_LOGIC
+            b.set_record_layout(48)
+        out.append((b.results(), [b.curve(i) for i in range(300)], b.trace(0), [b.result(i).events for i in range(300)]))
+        b.close()
+    (r0, c0, t0, e0), (r1, c1, t1, e1) = out
+    for k in ("steps", "foundered", "founderT", "tEnd"):
+        assert np.array_equal(r0[k], r1[k]), k
+    assert e0 == e1
+    for a, c in zip(c0, c1):
+        assert np.array_equal(a, c)
+    for a, c in zip(t0, t1):
         assert np.array_equal(a, c)

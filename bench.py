@@ -200,11 +200,12 @@ def bench_uneven(args):
                    "record_bytes": tune["record_bytes"],
                    "autotune_trials": [[t["threads_per_sim"], t["workgroups"], t["record_bytes"], round(t["rate"])] for t in tune["trials"]],
                    "device": _device()},
-        # traffic: FETCH_SIZE + WRITE_SIZE of the timed launch on this workload at 4,096 hulls (FETCH_SIZE unscaled:
-        # profiles/run_batch_fp64_uneven4096.md); a constant here, the run itself is not counted
+        # traffic: FETCH_SIZE + WRITE_SIZE of the timed launch on this workload at 4,096 hulls with the packed records
+        # (FETCH_SIZE unscaled: profiles/run_batch_fp64_uneven4096.md); a constant here, the run itself is not counted
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
-                     "frac": round(achieved / 8000.0, 5), "traffic": 52.7e12 if (count == 4096 and not mixed) else None,
-                     "traffic_source": "profiles/run_batch_fp64_uneven4096.md" if (count == 4096 and not mixed) else None,
+                     "frac": round(achieved / 8000.0, 5),
+                     "traffic": 43.23e12 if (count == 4096 and not mixed and tune["record_bytes"] == 40) else None,
+                     "traffic_source": "profiles/run_batch_fp64_uneven4096.md" if (count == 4096 and not mixed and tune["record_bytes"] == 40) else None,
                      "kernel": "run_batch<%s, true>" % ("true" if mixed else "false"), "kernel_ms": round(tot_kms / max(tot_launch, 1), 3)},
     }
     print(json.dumps(out), flush=True)

@@ -273,11 +273,13 @@ __device__ __forceinline__ double world_z(const SSL double* fr, double x, double
 // ------------------------------------------------------------------------------------------- cross-lane
 
 // Lane l of the logical 32-lane group receives lane l + off (off = 8, 4, 2, 1 by DPP row shifts inside a row of
-// 16; off = 16 by an LDS-unit swizzle that swaps the two rows of each group). Only lanes l < off use the result.
+// 16; off = 16 by an LDS-unit swizzle that swaps the two rows of each group). Only lanes l < off use the result;
+// the row shift gives the other lanes of a row whose source lies past the row's end 0 (bound control), so the move
+// writes every lane and needs no copy of the destination's old value first.
 template <int OFF>
 __device__ __forceinline__ uint32_t down32(uint32_t v) {
   if constexpr (OFF == 16) return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), 0x401F));
-  else return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(v), static_cast<int>(v), 0x100 + OFF, 0xF, 0xF, false));
+  else return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x100 + OFF, 0xF, 0xF, true));
 }
 template <int OFF>
 __device__ __forceinline__ double down(double v) {
@@ -475,6 +477,7 @@ __device__ __forceinline__ uint32_t rec_bytes(PC& P) { return sizeof(T) == sizeo
 // float record
 template <class T>
 constexpr int kWaitOneRecord = sizeof(T) == sizeof(double) ? 0x0F73 : 0x0F72;
+constexpr int kWaitAll = 0x0F70;  // no vector memory load in flight
 // (a packed double record is 8-byte aligned: its pairs are loaded as 16-byte loads of 8-byte alignment)
 typedef double d2v8 __attribute__((ext_vector_type(2), aligned(8)));
 __device__ __forceinline__ Geo<double> geo_at(const SSG d2v* table, uint32_t off) {
@@ -564,15 +567,18 @@ __device__ __forceinline__ void segment(const Fr32& f, float h, const Geo<float>
 // A lane's partial over the records first, first + step, ... below end, in order, each record's load issued while
 // the previous record is evaluated. Unrolled by two so that the two records in flight keep their registers (no
 // copies between iterations); records are addressed by byte offsets advanced by step records (no multiplication
-// per record). The loop is a scalar one with its only exit at the bottom: it runs an even number of records, at
-// least as many as the wave's longest lane needs (within each half of the wave `first` ascends with the lane and
-// `end` is common, so the lowest active lane of each half has the most records). A lane past its last record
+// per record). The loop is a scalar one with its only exit at the bottom: it runs the records in pairs while at
+// least two of the wave's rounds remain (the rounds are the wave's longest lane's record count: within each half of
+// the wave `first` ascends with the lane and `end` is common, so the lowest active lane of each half has the most
+// records), and an odd last round is evaluated after the loop, from the record already in flight. A lane past its last record
 // reloads that record (in bounds), a lane without records reads record 0 (in bounds, since another lane of the wave
 // has records), and eval(g, on) leaves the partial unchanged when on is false. Before each evaluation an explicit
 // wait for that record's loads (the other record's stay in flight) is made on every path, also where no lane
 // evaluates. The compiler's own waits would otherwise come at the loop's head, before the next record's loads are
 // issued, on any path that can reach the head with loads in flight (a lane leaving the loop early, a break in the
-// middle, an evaluation skipped by a branch): one record in flight instead of two.
+// middle, an evaluation skipped by a branch): one record in flight instead of two. The last round's second record
+// (an odd round count) would be a whole evaluation with no lane on; a break before it is such a path (the compiler
+// routes it through the loop's latch), hence the evaluation after the loop.
 // The scheduling barriers keep the next record's loads ahead of the current record's arithmetic: without them
 // the compiler sinks the loads below the evaluation and waits for them at once (vmcnt(0)).
 template <class T, class Tab, class Fn>
@@ -589,7 +595,7 @@ __device__ __forceinline__ void stream(const Tab* table, uint32_t rec, int first
   uint32_t off = mine > 0 ? static_cast<uint32_t>(first) * rec : 0u;
   Geo<T> g0 = geo_at(table, off);
   int k = 0;
-  do {
+  while (k + 1 < rounds) {
     const uint32_t off1 = min(off + stride, last);
     const Geo<T> g1 = geo_at(table, off1);
     __builtin_amdgcn_sched_barrier(0);
@@ -603,7 +609,13 @@ __device__ __forceinline__ void stream(const Tab* table, uint32_t rec, int first
     hold(g1);
     eval(g1, k + 1 < mine);
     k += 2;
-  } while (k < rounds);
+  }
+  if (k < rounds) {
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(kWaitAll);  // the last record, the only one in flight
+    hold(g0);
+    eval(g0, k < mine);
+  }
 }
 
 struct Pass {

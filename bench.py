@@ -37,7 +37,7 @@ FLOPS_PER_STEP = 2.34e5
 # kernel at 256, 512 and 1,024 threads per simulation, all but one of them idle, and plan_epoch), from FETCH_SIZE +
 # WRITE_SIZE summed over the epoch's kernels of a profiled run and calibrated against a plain copy kernel (FETCH_SIZE
 # x 2) (profiles/run_batch_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 13.53e6
+TRAFFIC_BYTES_PER_LAUNCH = 13.58e6
 TRAFFIC_SOURCE = "profiles/run_batch_fp64_titanic64_batch1024.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
@@ -204,7 +204,7 @@ def bench_uneven(args):
         # (FETCH_SIZE unscaled: profiles/run_batch_fp64_uneven4096.md); a constant here, the run itself is not counted
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
                      "frac": round(achieved / 8000.0, 5),
-                     "traffic": 43.23e12 if (count == 4096 and not mixed and tune["record_bytes"] == 40) else None,
+                     "traffic": 43.20e12 if (count == 4096 and not mixed and tune["record_bytes"] == 40) else None,
                      "traffic_source": "profiles/run_batch_fp64_uneven4096.md" if (count == 4096 and not mixed and tune["record_bytes"] == 40) else None,
                      "kernel": "run_batch<%s, true>" % ("true" if mixed else "false"), "kernel_ms": round(tot_kms / max(tot_launch, 1), 3)},
     }

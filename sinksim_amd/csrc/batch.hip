@@ -49,6 +49,16 @@ __device__ __forceinline__ void write_record(SSG double* rec, const Blk& B, int 
   write_record_arrays(rec, B.scal, B.vol, B.level, B.cx, B.cy, B.cz, nn);
 }
 
+// The thread's index for one step. In the portable32 kernel it is hidden from the compiler at each step (an empty
+// asm), so that the lane conditions the step derives from it are evaluated inside the step, one compare each,
+// instead of being hoisted out of the step loop into scalar registers that spill to vector lanes and cost two lane
+// reads per use; the FP64 kernel keeps them hoisted (measured faster there).
+template <bool Mixed>
+__device__ __forceinline__ int step_thread(int tid) {
+  if constexpr (Mixed) asm volatile("" : "+v"(tid));
+  return tid;
+}
+
 // One instance on one workgroup: load it into LDS, step it until it founders, reaches tMax or has taken maxSteps
 // steps in this launch, store it back. Traces and curves are recorded as in the reference
 // (device_step.cuh:379-416); the readouts of the state at the end of the launch go to final7.
@@ -129,6 +139,7 @@ __device__ __attribute__((always_inline)) void run_instance(PC& P, BC& Bt, int i
   const long long limit = exact && Bt.stepLimit ? Bt.stepLimit[inst] : maxSteps;
   uint64_t pa = uniform64(reinterpret_cast<uint64_t>(&P));  // the same in every lane, held in scalar registers
   for (long long s = 0; s < limit; ++s) {
+    const int t = step_thread<Mixed>(tid);
     PC& Q = model_at(pa);
     const Blk Bq = carve(lds, Q);
     // uniform: LDS values after a barrier
@@ -137,30 +148,30 @@ __device__ __attribute__((always_inline)) void run_instance(PC& P, BC& Bt, int i
     // LDS values; the barrier inside orders all of those reads (and the record's reads of the volumes) before
     // thread 0 advances the bookkeeping, so no wave can see the advanced values and skip the barrier.
     if (curves && Bq.scal[S_T] >= Bq.scal[S_NEXTCURVE] && Bq.scal[S_CURVECOUNT] < Bt.curveMax) {
-      if (tid == 0) {
+      if (t == 0) {
         SSG double* rec = Bt.curveRec + (static_cast<size_t>(inst) * Bt.curveMax + static_cast<size_t>(Bq.scal[S_CURVECOUNT])) * kCurveStride;
         readouts7(Q, Bq.scal, Bq.vol, rec, 1);
       }
       __syncthreads();
-      if (tid == 0) {
+      if (t == 0) {
         Bq.scal[S_CURVECOUNT] += 1;
         Bq.scal[S_NEXTCURVE] += Bt.curveEvery;
       }
       // read again only at the next iteration, after the step's barriers
     }
-    device_step<Mixed, TPB>(pa, lds, tid);
+    device_step<Mixed, TPB>(pa, lds, t);
     ++done;
     if (Bt.stopWhenStable) {
       // simulation.cpp:179-182: past 30 minutes, calm accumulates while the inflow, the pitch rate and the heave
       // velocity after the step stay small, and resets otherwise; the loop ends once it exceeds 20 minutes
-      if (tid == 0 && Bq.scal[S_T] > kCalmStart) {
+      if (t == 0 && Bq.scal[S_T] > kCalmStart) {
         const bool still = __builtin_fabs(Bq.scal[O_INFLOW]) < 0.02 && __builtin_fabs(Bq.scal[S_WTH]) < 1e-5 && __builtin_fabs(Bq.scal[S_VZ]) < 1e-3;
         Bq.scal[S_CALM] = still ? Bq.scal[S_CALM] + Q.M.dt : 0.0;
       }
       __syncthreads();
     }
     if (traced) {
-      if (tid == 0) {
+      if (t == 0) {
         const size_t rs = static_cast<size_t>(Bt.recSize);
         if (done <= Bt.denseSteps) {
           const int k = Bt.denseCount[inst]++;

@@ -166,9 +166,9 @@ ss_status ss_batch_set_connection_activation(ss_batch* b, int32_t instance, int3
 ss_status ss_batch_set_connections(ss_batch* b, const uint8_t* en, const double* area, const double* tOn);
 
 /* No counterpart in the reference: the launch shape. threadsPerSim is the workgroup size of one simulation: 256,
- * 512 (whose second half starts the level solve while the first computes the buoyancy partials), or 0, the default:
- * 256, then 512 and 1,024 for the last epochs of a run once its live instances fit resident at those sizes (see
- * ss_batch_run).
+ * 512 or 1,024 (beyond 256 the other waves start the level solve while the first 256 threads compute the buoyancy
+ * partials), or 0, the default: 256, then 512 and 1,024 for the last epochs of a run once its live instances fit
+ * resident at those sizes (see ss_batch_run).
  * workgroups is the number of workgroups of the launch (0, the default: as many as are resident on the device at
  * once, from the kernel's occupancy at this batch's LDS size); fewer than the instance count makes the kernel refill
  * them. The shape changes the time, never the results. SS_LOGIC after run. */
@@ -192,11 +192,11 @@ ss_status ss_batch_set_stop_when_stable(ss_batch* b, int32_t on);
 
 /* No counterpart in the reference: what ss_batch_autotune measured and chose. threadsPerSim and workgroups: the shape
  * set on the batch (workgroups 0: one per instance, or as many as are resident at once, the default). rate[k]: the
- * best simulation steps per second measured at 256 (k = 0) and 512 (k = 1) threads per simulation (0 if that shape
- * does not fit). trial*: every shape measured, in order (at most 8). */
+ * best simulation steps per second measured at 256 (k = 0), 512 (k = 1) and 1,024 (k = 2) threads per simulation (0
+ * if that shape does not fit). trial*: every shape measured, in order (at most 8). */
 typedef struct ss_tune_result {
   int32_t threadsPerSim, workgroups;
-  double rate[2];
+  double rate[3];
   int32_t trials;
   int32_t trialThreads[8], trialWorkgroups[8];
   double trialRate[8];
@@ -206,10 +206,10 @@ typedef struct ss_tune_result {
 
 /* No counterpart in the reference: the autotuner. Runs the first `sample` instances of the work order (the largest
  * models) for trialSteps steps at each launch shape on a trial copy (this batch's staged data is not touched), and
- * sets the fastest shape on the batch. The shapes: 256 and 512 threads per simulation, each with the default grid and,
- * where the sample holds more instances than are resident at once (the refilling kernel), with three quarters and
- * half of the resident workgroups; then, for a portable (FP64) batch, the other layout of the double geometry records
- * at the fastest shape, kept when faster. SS_LOGIC after run. */
+ * sets the fastest shape on the batch. The shapes: 256, 512 and 1,024 threads per simulation, each with the default
+ * grid and, where the sample holds more instances than are resident at once (the refilling kernel) and a compute unit
+ * holds more than one workgroup, with three quarters and half of the resident workgroups; then, for a portable (FP64)
+ * batch, the other layout of the double geometry records at the fastest shape, kept when faster. SS_LOGIC after run. */
 ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_tune_result* out);
 
 /* Batch::run (engine/cuda/batch.cu:245-308): runs every instance to its founder or tMax; *seconds receives the

@@ -60,7 +60,7 @@ class SsInstanceResult(C.Structure):
 
 
 class SsTuneResult(C.Structure):
-    _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 2), ("trials", C.c_int32),
+    _fields_ = [("threadsPerSim", C.c_int32), ("workgroups", C.c_int32), ("rate", C.c_double * 3), ("trials", C.c_int32),
                 ("trialThreads", C.c_int32 * 8), ("trialWorkgroups", C.c_int32 * 8), ("trialRate", C.c_double * 8),
                 ("recordBytes", C.c_int32), ("trialRecordBytes", C.c_int32 * 8)]
 

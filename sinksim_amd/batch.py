@@ -154,9 +154,9 @@ class Batch:
         self._keep = (e, a, t)
 
     def set_launch(self, threads_per_sim: int = 0, workgroups: int = 0) -> None:
-        """The launch shape: 256 or 512 threads per simulation (0, the default: 256, and 512 for a run's last epochs
-        once its live instances all fit resident at 512), and the workgroup count (0: as many as are resident at once;
-        fewer makes the kernel refill them). The shape changes the time, never the results."""
+        """The launch shape: 256, 512 or 1,024 threads per simulation (0, the default: 256, and 512 and 1,024 for a
+        run's last epochs once its live instances all fit resident at those sizes), and the workgroup count (0: as many
+        as are resident at once; fewer makes the kernel refill them). The shape changes the time, never the results."""
         A.check(self._lib.ss_batch_set_launch(self._h, int(threads_per_sim), int(workgroups)))
 
     def set_stop_when_stable(self, on: bool = True) -> None:
@@ -180,7 +180,7 @@ class Batch:
         trials = [{"threads_per_sim": r.trialThreads[k], "workgroups": r.trialWorkgroups[k], "record_bytes": r.trialRecordBytes[k],
                    "rate": r.trialRate[k]} for k in range(r.trials)]
         return {"threads_per_sim": r.threadsPerSim, "workgroups": r.workgroups, "record_bytes": r.recordBytes, "rate_256": r.rate[0],
-                "rate_512": r.rate[1], "trials": trials}
+                "rate_512": r.rate[1], "rate_1024": r.rate[2], "trials": trials}
 
     # ------------------------------------------------------------------ run and results
 

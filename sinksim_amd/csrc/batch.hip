@@ -703,7 +703,10 @@ namespace {
 using BatchKernel = void (*)(const ModelParams*, const DevBatch*, long long, double, const int32_t*);
 
 BatchKernel batch_kernel(bool mixed, bool persistent, int tpb) {
-  if (tpb == 1024) return mixed ? run_batch<true, false, 1024> : run_batch<false, false, 1024>;  // resident epochs only
+  if (tpb == 1024) {
+    if (persistent) return mixed ? run_batch<true, true, 1024> : run_batch<false, true, 1024>;
+    return mixed ? run_batch<true, false, 1024> : run_batch<false, false, 1024>;
+  }
   if (tpb == 512) {
     if (persistent) return mixed ? run_batch<true, true, 512> : run_batch<false, true, 512>;
     return mixed ? run_batch<true, false, 512> : run_batch<false, false, 512>;
@@ -721,7 +724,7 @@ struct LaunchPlan {
 
 LaunchPlan plan_launch(ss_batch* b) {
   const size_t count = static_cast<size_t>(b->opt.count);
-  for (int tpb : {kThreads, 512})
+  for (int tpb : {kThreads, 512, 1024})
     for (bool pers : {false, true})
       for (bool mixed : {false, true})
         if (b->ldsBytes > 64 * 1024)
@@ -983,7 +986,8 @@ ss_status ss_batch_set_launch(ss_batch* b, int32_t threadsPerSim, int32_t workgr
   return guarded([&] {
     need(b != nullptr, "ss_batch_set_launch: null batch");
     if (b->ran) fail(SS_LOGIC, "ss_batch_set_launch: set after run");
-    need(threadsPerSim == 0 || threadsPerSim == kThreads || threadsPerSim == 512, "ss_batch_set_launch: threadsPerSim must be 0, 256 or 512");
+    need(threadsPerSim == 0 || threadsPerSim == kThreads || threadsPerSim == 512 || threadsPerSim == 1024,
+         "ss_batch_set_launch: threadsPerSim must be 0, 256, 512 or 1024");
     need(workgroups >= 0, "ss_batch_set_launch: negative workgroup count");
     b->tpb = threadsPerSim == 0 ? kThreads : threadsPerSim;
     b->tailForm = threadsPerSim == 0;
@@ -1025,7 +1029,7 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
     o.traced = 0;
     o.curveEvery = 0;
     o.stepsPerLaunch = trialSteps;
-    const int tpbs[2] = {kThreads, 512};
+    const int tpbs[3] = {kThreads, 512, 1024};
     double best = -1;
     int32_t bestTpb = kThreads, bestBlocks = 0, bestRec = b->recBytes;
     ss_tune_result res{};
@@ -1060,7 +1064,7 @@ ss_status ss_batch_autotune(ss_batch* b, int32_t sample, int64_t trialSteps, ss_
       }
       return rate;
     };
-    for (int c = 0; c < 2; ++c) {
+    for (int c = 0; c < 3; ++c) {
       size_t lds = 0;
       for (const auto& m : b->models) lds = std::max(lds, m->lds_bytes(tpbs[c]));
       if (lds > lds_limit()) continue;  // this shape does not fit the largest model

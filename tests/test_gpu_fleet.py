@@ -63,7 +63,7 @@ def _check_fleet(models, inst_model, numerics, t_max, every, workgroups, tpb=0, 
     b.close()
 
 
-@pytest.mark.parametrize("tpb", [256, 512])
+@pytest.mark.parametrize("tpb", [256, 512, 1024])
 @pytest.mark.parametrize("numerics", ["portable", "portable32"])
 def test_fleet_matches_oracle_with_refill(numerics, tpb):
     models = _models()
@@ -154,8 +154,8 @@ def test_512_threads_reproduce_golden_trace(titanic, goldens, numerics):
 
 
 def test_autotune_picks_a_shape_and_keeps_results():
-    """The autotuner measures both shapes on a trial copy, sets the faster one, and the tuned run equals an
-    untuned one instance for instance."""
+    """The autotuner measures the three workgroup sizes on a trial copy, sets the fastest shape, and the tuned run
+    equals an untuned one instance for instance."""
     from sinksim_amd.batch import Batch, BatchOptions
     models = _models()
     inst_model = np.array([2, 1, 0, 3, 4, 2, 0, 1, 3, 4], np.int32)
@@ -167,7 +167,8 @@ def test_autotune_picks_a_shape_and_keeps_results():
         b.set_connections(np.concatenate(en), np.concatenate(area), np.concatenate(tOn))
         if tune:
             r = b.autotune(sample=8, trial_steps=200)
-            assert r["threads_per_sim"] in (256, 512) and r["rate_256"] > 0 and r["rate_512"] > 0
+            assert r["threads_per_sim"] in (256, 512, 1024) and r["rate_256"] > 0 and r["rate_512"] > 0 and r["rate_1024"] > 0
+            assert {t["threads_per_sim"] for t in r["trials"]} == {256, 512, 1024}, r
             # the double records' two layouts are measured (the last trial: the other layout at the fastest shape)
             assert r["record_bytes"] in (48, 40) and {t["record_bytes"] for t in r["trials"]} == {48, 40}, r
         b.run()
@@ -192,7 +193,7 @@ def test_autotune_grid_choice_keeps_results():
             r = b.autotune(sample=n, trial_steps=60)
             grids = {t["workgroups"] for t in r["trials"]}
             assert len(r["trials"]) >= 3 and any(g > 0 for g in grids), r
-            assert r["workgroups"] in grids and r["threads_per_sim"] in (256, 512)
+            assert r["workgroups"] in grids and r["threads_per_sim"] in (256, 512, 1024)
         b.run()
         out.append((b.results(), [b.curve(i) for i in range(0, n, 97)]))
         b.close()

@@ -310,6 +310,22 @@ __device__ __forceinline__ double lead16(double v) {
 }
 __device__ __forceinline__ float lead16(float v) { return __builtin_bit_cast(float, lead16_32(__builtin_bit_cast(uint32_t, v))); }
 
+// Lanes 8 and 24 of each logical 32-lane group to every lane of the group: source lane ((l & 0) | 8) and
+// ((l & 0) | 24) within each group of 32.
+template <int OR>
+__device__ __forceinline__ uint32_t lane_of32(uint32_t v) { return static_cast<uint32_t>(__builtin_amdgcn_ds_swizzle(static_cast<int>(v), OR << 5)); }
+template <int OR>
+__device__ __forceinline__ double lane_of(double v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(lane_of32<OR>(static_cast<uint32_t>(b >> 32))) << 32) | lane_of32<OR>(static_cast<uint32_t>(b)));
+}
+template <int OR>
+__device__ __forceinline__ float lane_of(float v) { return __builtin_bit_cast(float, lane_of32<OR>(__builtin_bit_cast(uint32_t, v))); }
+template <class T>
+__device__ __forceinline__ T lead8(T v) { return lane_of<8>(v); }
+template <class T>
+__device__ __forceinline__ T lead24(T v) { return lane_of<24>(v); }
+
 // Two values of a logical 32-lane group, a and b, after the first level of the halving tree (l absorbs l + 16)
 // packed into one register: lanes 0..15 of the group end with a's sums and lanes 16..31 with b's. The gfx950
 // v_permlane16_swap exchanges rows 1 and 3 (16 lanes each) of its first operand with rows 0 and 2 of its second:
@@ -364,13 +380,27 @@ __device__ __forceinline__ void min_level(T& v) {
 
 // The node pass's halving trees (the reference's sums and comparisons, operand for operand), with (vol, dv) and
 // (mx, my) packed in pairs, and with Ext (zmin, -zmax) as a pair of minimum trees: -max(-a, -b) is max(a, b)
-// bit for bit, ties and NaN keeping the lane's own value in both. The results stay packed: vol in lane 0 and dv in
-// lane 16 of each group (vd), mx and my likewise (xy), mz in lane 0 (z), with Ext zmin in lane 0 and -zmax in lane 16
+// bit for bit, ties and NaN keeping the lane's own value in both. The results stay packed: vol in lane 0, dv in lane
+// 16, mx in lane 8 and my in lane 24 of each group (q), mz in lane 0 (z), with Ext zmin in lane 0 and -zmax in lane 16
 // (mm).
 template <class T>
 struct PackedPass {
-  T vd, xy, z, mm;
+  T q, z, mm;
 };
+// a's lanes 0..7 of each row of 16 with b's lanes 0..7 moved into lanes 8..15: a row shift right by 8 that writes
+// banks 2 and 3 (lanes 8..15) only, the other lanes keeping a
+__device__ __forceinline__ uint32_t merge8_32(uint32_t a, uint32_t b) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(a), static_cast<int>(b), 0x118, 0xF, 0xC, false));
+}
+__device__ __forceinline__ double merge8(double a, double b) {
+  const uint64_t ua = __builtin_bit_cast(uint64_t, a), ub = __builtin_bit_cast(uint64_t, b);
+  const uint32_t lo = merge8_32(static_cast<uint32_t>(ua), static_cast<uint32_t>(ub));
+  const uint32_t hi = merge8_32(static_cast<uint32_t>(ua >> 32), static_cast<uint32_t>(ub >> 32));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+__device__ __forceinline__ float merge8(float a, float b) {
+  return __builtin_bit_cast(float, merge8_32(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
+}
 template <bool Ext, class T>
 __device__ __forceinline__ PackedPass<T> tree32_levels(const PP<T>& p) {
   T vd = pack_sum16(p.vol, p.dv), xy = pack_sum16(p.mx, p.my), z = p.mz;
@@ -379,20 +409,23 @@ __device__ __forceinline__ PackedPass<T> tree32_levels(const PP<T>& p) {
   if constexpr (Ext) mm = pack_min16(p.zmin, -p.zmax);
   sum_level<8>(vd); sum_level<8>(xy); sum_level<8>(z);
   if constexpr (Ext) min_level<8>(mm);
-  sum_level<4>(vd); sum_level<4>(xy); sum_level<4>(z);
+  // after +8 a tree lives in lanes 0..7 of its row: the (mx, my) trees move beside (vol, dv), and the +4, +2 and +1
+  // levels serve all four (lanes 0..3 of a row absorb 4..7 and lanes 8..11 absorb 12..15, and so on)
+  T q = merge8(vd, xy);
+  sum_level<4>(q); sum_level<4>(z);
   if constexpr (Ext) min_level<4>(mm);
-  sum_level<2>(vd); sum_level<2>(xy); sum_level<2>(z);
+  sum_level<2>(q); sum_level<2>(z);
   if constexpr (Ext) min_level<2>(mm);
-  sum_level<1>(vd); sum_level<1>(xy); sum_level<1>(z);
+  sum_level<1>(q); sum_level<1>(z);
   if constexpr (Ext) min_level<1>(mm);
-  return PackedPass<T>{vd, xy, z, mm};
+  return PackedPass<T>{q, z, mm};
 }
 // the trees' totals in every lane of the group
 template <bool Ext, class T>
 __device__ __forceinline__ PP<T> tree32_all(const PP<T>& p) {
   const PackedPass<T> q = tree32_levels<Ext>(p);
   PP<T> r;
-  r.vol = lead(q.vd); r.dv = lead16(q.vd); r.mx = lead(q.xy); r.my = lead16(q.xy); r.mz = lead(q.z);
+  r.vol = lead(q.q); r.dv = lead16(q.q); r.mx = lead8(q.q); r.my = lead24(q.q); r.mz = lead(q.z);
   r.zmin = 0; r.zmax = 0;
   if constexpr (Ext) { r.zmin = lead(q.mm); r.zmax = -lead16(q.mm); }
   return r;
@@ -413,7 +446,7 @@ __device__ __forceinline__ double at_lane(double v) {
 template <int L0, bool Ext, class T>
 __device__ __forceinline__ PP<T> group_totals(const PackedPass<T>& q) {
   PP<T> r;
-  r.vol = at_lane<L0>(q.vd); r.dv = at_lane<L0 + 16>(q.vd); r.mx = at_lane<L0>(q.xy); r.my = at_lane<L0 + 16>(q.xy);
+  r.vol = at_lane<L0>(q.q); r.dv = at_lane<L0 + 16>(q.q); r.mx = at_lane<L0 + 8>(q.q); r.my = at_lane<L0 + 24>(q.q);
   r.mz = at_lane<L0>(q.z);
   r.zmin = 0; r.zmax = 0;
   if constexpr (Ext) { r.zmin = at_lane<L0>(q.mm); r.zmax = -at_lane<L0 + 16>(q.mm); }

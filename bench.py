@@ -37,7 +37,7 @@ FLOPS_PER_STEP = 2.34e5
 # kernel at 256, 512 and 1,024 threads per simulation, all but one of them idle, and plan_epoch), from FETCH_SIZE +
 # WRITE_SIZE summed over the epoch's kernels of a profiled run and calibrated against a plain copy kernel (FETCH_SIZE
 # x 2) (profiles/run_batch_fp64_titanic64_batch1024.md).
-TRAFFIC_BYTES_PER_LAUNCH = 13.58e6
+TRAFFIC_BYTES_PER_LAUNCH = 13.55e6
 TRAFFIC_SOURCE = "profiles/run_batch_fp64_titanic64_batch1024.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 
